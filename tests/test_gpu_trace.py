@@ -1,0 +1,144 @@
+"""GPU parity of the acceleration structures and trace kernels.
+
+The oracle is brute force over every triangle of every instance with the same
+FP32 watertight test; the HIP path builds its BVH on the device and must
+return the same closest (instance, primitive) and bit-identical t and
+barycentrics, equal-t ties included (lowest instance, then primitive).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from vork_amd import abi, native, scene as S
+from tests.helpers import random_rays
+
+pytestmark = pytest.mark.gpu
+
+
+def _compare(g, o):
+    gh, oh = g["t"] >= 0, o["t"] >= 0
+    assert np.array_equal(gh, oh), f"hit/miss differs on {np.sum(gh != oh)} rays"
+    assert np.array_equal(g["instance"][gh], o["instance"][oh])
+    assert np.array_equal(g["primitive"][gh], o["primitive"][oh])
+    assert np.array_equal(g["custom_index"][gh], o["custom_index"][oh])
+    assert np.array_equal(g["t"].view(np.uint32), o["t"].view(np.uint32)), "t differs"
+    assert np.array_equal(g["u"][gh].view(np.uint32), o["u"][oh].view(np.uint32))
+    assert np.array_equal(g["v"][gh].view(np.uint32), o["v"][oh].view(np.uint32))
+
+
+def test_closest_and_any_match_brute_force_config0(renderer):
+    sc = S.config0()
+    inst = renderer.install(sc)
+    ora = oracle.OracleScene(sc)
+    rays = random_rays(sc, 6000, seed=1)
+    _compare(inst.tlas.trace_closest(rays), ora.trace_closest(rays, brute_force=True))
+    occ = inst.tlas.trace_any(rays)
+    assert np.array_equal(occ, ora.trace_any(rays, brute_force=True))
+
+
+def test_camera_rays_match_oracle_bvh(renderer):
+    sc = S.config0(width=96, height=96)
+    inst = renderer.install(sc)
+    ora = oracle.OracleScene(sc)
+    p = sc.render_params()
+    rays = np.array([ora.camera_ray(p, x, y, 0) for y in range(96) for x in range(96)], np.float32)
+    rays = rays[:, [0, 1, 2, 7, 4, 5, 6, 3]]  # origin, tmin(0), direction, tmax
+    rays[:, 3] = 0.0
+    _compare(inst.tlas.trace_closest(rays, mask=abi.VH_RAY_CAMERA), ora.trace_closest(rays, abi.VH_RAY_CAMERA))
+
+
+def _rot(axis, angle):
+    axis = np.asarray(axis, np.float64) / np.linalg.norm(axis)
+    x, y, z = axis
+    c, s, C = math.cos(angle), math.sin(angle), 1 - math.cos(angle)
+    return np.array([[c + x * x * C, x * y * C - z * s, x * z * C + y * s],
+                     [y * x * C + z * s, c + y * y * C, y * z * C - x * s],
+                     [z * x * C - y * s, z * y * C + x * s, c + z * z * C]])
+
+
+def test_instanced_transforms_masks_and_culling(renderer):
+    rng = np.random.default_rng(5)
+    meshes = [S.uv_sphere(1.0, 24, 12), S.box((1.0, 2.0, 0.5)), S.icosphere_mesh(2)]
+    instances = []
+    for k in range(40):
+        R = _rot(rng.normal(size=3), rng.uniform(0, 6.28)) * rng.uniform(0.3, 1.7, size=3)
+        T = rng.uniform(-6, 6, size=3)
+        m = np.concatenate([R, T[:, None]], 1).reshape(-1)
+        instances.append(dict(transform=m.tolist(), blas=int(k % 3), material=0, custom=1000 + k,
+                              mask=int(rng.choice([0xFF, 0x01, 0x04])),
+                              flags=abi.VH_INSTANCE_CULL_BACK_FACES if k % 5 == 0 else 0))
+    # a singular instance must be ignored
+    instances.append(dict(transform=[0.0] * 12, blas=0, material=0))
+    sc = S.Scene(meshes, instances, [dict(base_color=(0.5, 0.5, 0.5))], [], dict(eye=(0, 0, 10), target=(0, 0, 0)),
+                 dict(kind=0, color=(0, 0, 0)))
+    inst = renderer.install(sc)
+    ora = oracle.OracleScene(sc)
+    rays = random_rays(S.Scene(meshes, instances[:-1], [], [], {}, {}), 4000, seed=3)
+    for mask in (0xFF, 0x01, 0x04):
+        _compare(inst.tlas.trace_closest(rays, mask), ora.trace_closest(rays, mask, brute_force=True))
+        assert np.array_equal(inst.tlas.trace_any(rays, mask), ora.trace_any(rays, mask, brute_force=True))
+
+
+def test_shared_edges_are_hit_exactly_once(renderer):
+    """A ray through a shared edge of a closed mesh crosses it exactly once per
+    side (watertightness): with culling off, the closest hit exists for every
+    ray aimed at an interior edge point."""
+    m = S.icosphere_mesh(3)
+    sc = S.Scene([m], [dict(transform=S.translate(0, 0, 0), blas=0, material=0)], [dict(base_color=(1, 1, 1))], [],
+                 dict(eye=(0, 0, 5), target=(0, 0, 0)), dict(kind=0, color=(0, 0, 0)))
+    inst = renderer.install(sc)
+    ora = oracle.OracleScene(sc)
+    rng = np.random.default_rng(11)
+    P, I = m.positions.astype(np.float64), m.indices
+    n = 3000
+    t = I[rng.integers(0, I.shape[0], n)]
+    w = rng.uniform(0.05, 0.95, size=(n, 1))
+    target = P[t[:, 0]] * (1 - w) + P[t[:, 1]] * w
+    origin = target * 3.0 + rng.normal(size=(n, 3)) * 0.3
+    d = target - origin
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, :3], rays[:, 4:7], rays[:, 7] = origin, d, 1e30
+    g = inst.tlas.trace_closest(rays)
+    assert (g["t"] > 0).all(), "a ray through an edge missed both triangles"
+    _compare(g, ora.trace_closest(rays, brute_force=True))
+
+
+def test_traversal_counts_are_consistent(renderer):
+    sc = S.config0()
+    inst = renderer.install(sc)
+    rays = random_rays(sc, 2000, seed=9)
+    counts = inst.tlas.trace_counts(rays)
+    hits = inst.tlas.trace_closest(rays)
+    assert (counts[:, 0] >= 1).all()
+    assert (counts[hits["t"] >= 0, 1] >= 1).all()
+    info = inst.blases[0].info()
+    assert info["triangle_count"] == 20000
+    assert info["node_count"] >= 20000 // (8 * 4)
+
+
+def test_blas_from_device_pointers_matches_host(renderer):
+    torch = pytest.importorskip("torch")
+    m = S.uv_sphere(1.0, 32, 16)
+    b_host = native.Blas(renderer.device, m.positions, m.indices, m.normals)
+    pos = torch.from_numpy(m.positions).cuda()
+    idx = torch.from_numpy(m.indices.astype(np.int32)).cuda()
+    nrm = torch.from_numpy(m.normals).cuda()
+    b_dev = native.Blas(renderer.device, pos, idx, nrm, device_pointers=True)
+    sc = S.Scene([m], [dict(transform=S.translate(0, 0, 0), blas=0, material=0)], [dict(base_color=(1, 1, 1))], [],
+                 dict(eye=(0, 0, 5), target=(0, 0, 0)), dict(kind=0, color=(0, 0, 0)))
+    rays = random_rays(sc, 1000, seed=2)
+    t1 = native.Tlas(renderer.device, [b_host], sc.instance_array())
+    t2 = native.Tlas(renderer.device, [b_dev], sc.instance_array())
+    _compare(t1.trace_closest(rays), t2.trace_closest(rays))
+
+
+def test_invalid_inputs_raise(renderer):
+    with pytest.raises(native.VorkError) as e:
+        native.Blas(renderer.device, np.zeros((3, 3), np.float32), np.array([[0, 1, 5]], np.uint32))
+    assert e.value.status == abi.VH_ERR_SCENE
+    with pytest.raises(native.VorkError) as e:
+        native.Blas(renderer.device, np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32))
+    assert e.value.status == abi.VH_ERR_SCENE

@@ -1,0 +1,601 @@
+// GPU construction of the 8-wide BVH used by both BLAS (triangles) and TLAS
+// (instances).
+//
+//  1. primitive boxes and centroid bounds
+//  2. 63-bit Morton codes of the centroids, radix-sorted with their ids
+//  3. binary radix tree over the sorted codes (Karras 2012), parents linked
+//  4. bottom-up box refit, one thread per leaf, the second child to arrive at
+//     a node continues upward
+//  5. top-down collapse into 8-wide nodes, one kernel per wide level: each
+//     node opens its largest-area children until it holds 8; subtrees of at
+//     most `leaf_max` primitives become leaf children whose items are written
+//     contiguously per node
+//
+// The rocprim radix sort is the only library call; all other passes are
+// kernels written here.
+#include <cstring>
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "vh_internal.h"
+
+namespace vh {
+namespace {
+
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kNone = 0xffffffffu;
+
+__device__ __forceinline__ uint32_t float_to_ordered(float f) {
+    uint32_t u = f2u(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ordered_to_float(uint32_t u) {
+    return u2f((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+__global__ void k_init_bounds(uint32_t* b) {
+    if (threadIdx.x < 3) { b[threadIdx.x] = 0xffffffffu; b[3 + threadIdx.x] = 0u; }
+}
+
+// Centroid bounds: per-block reduction then ordered-uint atomics.
+__global__ void k_centroid_bounds(const float* lo, const float* hi, uint32_t n, uint32_t* bounds) {
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        for (int k = 0; k < 3; k++) {
+            float c = 0.5f * (lo[3 * i + k] + hi[3 * i + k]);
+            mn[k] = fminf(mn[k], c);
+            mx[k] = fmaxf(mx[k], c);
+        }
+    }
+    __shared__ float smn[3][256], smx[3][256];
+    for (int k = 0; k < 3; k++) { smn[k][threadIdx.x] = mn[k]; smx[k][threadIdx.x] = mx[k]; }
+    __syncthreads();
+    for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int k = 0; k < 3; k++) {
+                smn[k][threadIdx.x] = fminf(smn[k][threadIdx.x], smn[k][threadIdx.x + s]);
+                smx[k][threadIdx.x] = fmaxf(smx[k][threadIdx.x], smx[k][threadIdx.x + s]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; k++) {
+            atomicMin(&bounds[k], float_to_ordered(smn[k][0]));
+            atomicMax(&bounds[3 + k], float_to_ordered(smx[k][0]));
+        }
+}
+
+__device__ __forceinline__ uint64_t spread21(uint64_t x) {
+    x &= 0x1fffffull;
+    x = (x | (x << 32)) & 0x1f00000000ffffull;
+    x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+    x = (x | (x << 8)) & 0x100f00f00f00f00full;
+    x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+
+__global__ void k_morton(const float* lo, const float* hi, uint32_t n, const uint32_t* bounds, uint64_t* keys,
+                         uint32_t* ids) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t q[3];
+    for (int k = 0; k < 3; k++) {
+        float bmin = ordered_to_float(bounds[k]), bmax = ordered_to_float(bounds[3 + k]);
+        float ext = bmax - bmin;
+        float c = 0.5f * (lo[3 * i + k] + hi[3 * i + k]);
+        float f = ext > 0.f ? (c - bmin) / ext : 0.5f;
+        f = fminf(fmaxf(f, 0.f), 1.f);
+        q[k] = (uint64_t)fminf(f * 2097152.f, 2097151.f);
+    }
+    keys[i] = spread21(q[0]) | (spread21(q[1]) << 1) | (spread21(q[2]) << 2);
+    ids[i] = i;
+}
+
+// Karras 2012 prefix length with ties broken by index.
+__device__ __forceinline__ int delta(const uint64_t* keys, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    uint64_t a = keys[i], b = keys[j];
+    if (a == b) return 64 + __clz((uint32_t)(i ^ j));
+    return __clzll(a ^ b);
+}
+
+// Internal node arrays (n-1 of them): left/right child refs (kLeafBit marks a
+// leaf = sorted position), parent, covered range [first, last].
+__global__ void k_karras(const uint64_t* keys, int n, uint32_t* left, uint32_t* right, uint32_t* parent,
+                         uint32_t* leaf_parent, uint32_t* first, uint32_t* last) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+    int dmin = delta(keys, n, i, i - d);
+    int lmax = 2;
+    while (delta(keys, n, i, i + lmax * d) > dmin) lmax *= 2;
+    int l = 0;
+    for (int t = lmax / 2; t >= 1; t /= 2)
+        if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    int j = i + l * d;
+    int dnode = delta(keys, n, i, j);
+    int s = 0;
+    for (int div = 2;; div *= 2) {
+        int t = (l + div - 1) / div;
+        if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+        if (t <= 1) break;
+    }
+    int gamma = i + s * d + min(d, 0);
+    int lo = min(i, j), hi = max(i, j);
+    uint32_t lc = (lo == gamma) ? (kLeafBit | (uint32_t)gamma) : (uint32_t)gamma;
+    uint32_t rc = (hi == gamma + 1) ? (kLeafBit | (uint32_t)(gamma + 1)) : (uint32_t)(gamma + 1);
+    left[i] = lc;
+    right[i] = rc;
+    first[i] = (uint32_t)lo;
+    last[i] = (uint32_t)hi;
+    if (lc & kLeafBit) leaf_parent[lc & ~kLeafBit] = (uint32_t)i; else parent[lc] = (uint32_t)i;
+    if (rc & kLeafBit) leaf_parent[rc & ~kLeafBit] = (uint32_t)i; else parent[rc] = (uint32_t)i;
+    if (i == 0) parent[0] = kNone;
+}
+
+__device__ __forceinline__ float aload(const float* p) {
+    return u2f(__hip_atomic_load((const uint32_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+__device__ __forceinline__ void astore(float* p, float v) {
+    __hip_atomic_store((uint32_t*)p, f2u(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Bottom-up refit: node boxes are 6 floats (lo xyz, hi xyz).
+__global__ void k_refit(const float* plo, const float* phi, const uint32_t* sorted_ids, uint32_t n,
+                        const uint32_t* left, const uint32_t* right, const uint32_t* parent,
+                        const uint32_t* leaf_parent, uint32_t* flags, float* nb) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t p = leaf_parent[i];
+    while (p != kNone) {
+        __threadfence();
+        uint32_t prev = atomicAdd(&flags[p], 1u);
+        if (prev == 0) return;  // the sibling subtree is not finished yet
+        __threadfence();
+        float lo[3], hi[3];
+        uint32_t c[2] = {left[p], right[p]};
+        for (int k = 0; k < 3; k++) { lo[k] = INFINITY; hi[k] = -INFINITY; }
+        for (int s = 0; s < 2; s++) {
+            if (c[s] & kLeafBit) {
+                uint32_t prim = sorted_ids[c[s] & ~kLeafBit];
+                for (int k = 0; k < 3; k++) {
+                    lo[k] = fminf(lo[k], plo[3 * prim + k]);
+                    hi[k] = fmaxf(hi[k], phi[3 * prim + k]);
+                }
+            } else {
+                for (int k = 0; k < 3; k++) {
+                    lo[k] = fminf(lo[k], aload(&nb[6 * c[s] + k]));
+                    hi[k] = fmaxf(hi[k], aload(&nb[6 * c[s] + 3 + k]));
+                }
+            }
+        }
+        for (int k = 0; k < 3; k++) { astore(&nb[6 * p + k], lo[k]); astore(&nb[6 * p + 3 + k], hi[k]); }
+        p = parent[p];
+    }
+}
+
+struct Tree {
+    const uint32_t *left, *right, *first, *last, *sorted_ids;
+    const float *nb, *plo, *phi;
+};
+
+__device__ __forceinline__ uint32_t ref_size(const Tree& t, uint32_t r) {
+    return (r & kLeafBit) ? 1u : t.last[r] - t.first[r] + 1u;
+}
+__device__ __forceinline__ uint32_t ref_first(const Tree& t, uint32_t r) {
+    return (r & kLeafBit) ? (r & ~kLeafBit) : t.first[r];
+}
+__device__ __forceinline__ void ref_box(const Tree& t, uint32_t r, float lo[3], float hi[3]) {
+    if (r & kLeafBit) {
+        uint32_t prim = t.sorted_ids[r & ~kLeafBit];
+        for (int k = 0; k < 3; k++) { lo[k] = t.plo[3 * prim + k]; hi[k] = t.phi[3 * prim + k]; }
+    } else {
+        for (int k = 0; k < 3; k++) { lo[k] = t.nb[6 * r + k]; hi[k] = t.nb[6 * r + 3 + k]; }
+    }
+}
+__device__ __forceinline__ float half_area(const float lo[3], const float hi[3]) {
+    float dx = fmaxf(hi[0] - lo[0], 0.f), dy = fmaxf(hi[1] - lo[1], 0.f), dz = fmaxf(hi[2] - lo[2], 0.f);
+    return dx * dy + dy * dz + dz * dx;
+}
+
+// Quantise one axis of up to 8 child boxes against the node's lower corner,
+// making sure the decoded boxes enclose the exact ones.
+__device__ void quantise_axis(float origin, float node_hi, int m, const float* clo, const float* chi, uint8_t* ebyte,
+                              uint8_t* qlo, uint8_t* qhi) {
+    float extent = node_hi - origin;
+    int e = -126;
+    if (extent > 0.f) {
+        int ex;
+        float mant = frexpf(extent / 255.f, &ex);  // extent/255 = mant * 2^ex, mant in [0.5,1)
+        e = (mant == 0.5f) ? ex - 1 : ex;
+        if (e < -126) e = -126;
+    }
+    for (;; e++) {
+        if (e > 127) e = 127;
+        float scale = u2f(uint32_t(e + 127) << 23);
+        bool ok = true;
+        for (int i = 0; i < m; i++) {
+            float a = floorf((clo[i] - origin) / scale), b = ceilf((chi[i] - origin) / scale);
+            int ql = (int)fminf(fmaxf(a, 0.f), 255.f), qh = (int)fminf(fmaxf(b, 0.f), 255.f);
+            while (ql > 0 && decode(origin, (uint8_t)ql, scale) > clo[i]) ql--;
+            while (qh < 255 && decode(origin, (uint8_t)qh, scale) < chi[i]) qh++;
+            if (decode(origin, (uint8_t)ql, scale) > clo[i] || decode(origin, (uint8_t)qh, scale) < chi[i]) {
+                ok = false;
+                break;
+            }
+            qlo[i] = (uint8_t)ql;
+            qhi[i] = (uint8_t)qh;
+        }
+        if (ok || e == 127) { *ebyte = (uint8_t)(e + 127); return; }
+    }
+}
+
+struct QueueItem { uint32_t binary; uint32_t wide; };
+
+__global__ void k_collapse(Tree t, const QueueItem* in, const uint32_t* in_count, QueueItem* out, uint32_t* out_count,
+                           uint32_t* node_counter, uint32_t* item_counter, WideNode* nodes, uint32_t* order,
+                           uint32_t leaf_max, uint32_t n_prims, uint32_t* max_level, uint32_t level) {
+    uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= *in_count) return;
+    QueueItem it = in[idx];
+    uint32_t c[kWidth];
+    int m = 0;
+    if (it.binary == kNone) {
+        // a single-primitive tree: one leaf child
+        c[m++] = kLeafBit | 0u;
+    } else {
+        c[m++] = t.left[it.binary];
+        c[m++] = t.right[it.binary];
+        while (m < kWidth) {
+            int best = -1;
+            float best_area = -1.f;
+            for (int k = 0; k < m; k++) {
+                if ((c[k] & kLeafBit) || ref_size(t, c[k]) <= leaf_max) continue;
+                float lo[3], hi[3];
+                ref_box(t, c[k], lo, hi);
+                float a = half_area(lo, hi);
+                if (a > best_area) { best_area = a; best = k; }
+            }
+            if (best < 0) break;
+            uint32_t r = c[best];
+            c[best] = t.left[r];
+            c[m++] = t.right[r];
+        }
+    }
+    float clo[3][kWidth], chi[3][kWidth];
+    float nlo[3] = {INFINITY, INFINITY, INFINITY}, nhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int k = 0; k < m; k++) {
+        float lo[3], hi[3];
+        ref_box(t, c[k], lo, hi);
+        for (int a = 0; a < 3; a++) {
+            clo[a][k] = lo[a]; chi[a][k] = hi[a];
+            nlo[a] = fminf(nlo[a], lo[a]); nhi[a] = fmaxf(nhi[a], hi[a]);
+        }
+    }
+    WideNode node;
+    for (int a = 0; a < 3; a++) node.origin[a] = nlo[a];
+    uint8_t e[3];
+    for (int a = 0; a < 3; a++) {
+        uint8_t ql[kWidth], qh[kWidth];
+        quantise_axis(nlo[a], nhi[a], m, clo[a], chi[a], &e[a], ql, qh);
+        for (int k = 0; k < kWidth; k++) {
+            node.qlo[a][k] = k < m ? ql[k] : 255;
+            node.qhi[a][k] = k < m ? qh[k] : 0;
+        }
+    }
+    node.ex = e[0]; node.ey = e[1]; node.ez = e[2];
+    node.count = (uint8_t)m;
+    uint32_t n_internal = 0, n_items = 0;
+    for (int k = 0; k < m; k++) {
+        bool internal = !(c[k] & kLeafBit) && ref_size(t, c[k]) > leaf_max;
+        if (internal) n_internal++; else n_items += ref_size(t, c[k]);
+    }
+    uint32_t child_base = n_internal ? atomicAdd(node_counter, n_internal) : 0u;
+    uint32_t item_base = n_items ? atomicAdd(item_counter, n_items) : 0u;
+    node.child_base = child_base;
+    node.item_base = item_base;
+    uint32_t ki = 0, off = 0;
+    for (int k = 0; k < kWidth; k++) node.meta[k] = 0;
+    for (int k = 0; k < m; k++) {
+        bool internal = !(c[k] & kLeafBit) && ref_size(t, c[k]) > leaf_max;
+        if (internal) {
+            node.meta[k] = (uint8_t)(kMetaInternal | ki);
+            uint32_t slot = atomicAdd(out_count, 1u);
+            out[slot].binary = c[k];
+            out[slot].wide = child_base + ki;
+            ki++;
+        } else {
+            uint32_t cnt = ref_size(t, c[k]), f0 = ref_first(t, c[k]);
+            node.meta[k] = (uint8_t)(off | ((cnt - 1) << 5));
+            for (uint32_t j = 0; j < cnt; j++) order[item_base + off + j] = t.sorted_ids[f0 + j];
+            off += cnt;
+        }
+    }
+    nodes[it.wide] = node;
+    if (ki) atomicMax(max_level, level + 1);
+}
+
+// SAH cost of the binary tree (C_node = 1, C_prim = 1), relative to the root.
+__global__ void k_sah(const float* nb, uint32_t n_internal, double* out) {
+    __shared__ double s[256];
+    double acc = 0;
+    float rlo[3] = {nb[0], nb[1], nb[2]}, rhi[3] = {nb[3], nb[4], nb[5]};
+    double root = half_area(rlo, rhi);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_internal; i += gridDim.x * blockDim.x) {
+        float lo[3] = {nb[6 * i], nb[6 * i + 1], nb[6 * i + 2]}, hi[3] = {nb[6 * i + 3], nb[6 * i + 4], nb[6 * i + 5]};
+        acc += root > 0 ? half_area(lo, hi) / root : 0.0;
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t k = blockDim.x / 2; k > 0; k >>= 1) {
+        if (threadIdx.x < k) s[threadIdx.x] += s[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(out, s[0]);
+}
+
+inline uint32_t blocks(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
+
+}  // namespace
+
+int build_wide_bvh(vh_device* dev, hipStream_t s, const float* d_lo, const float* d_hi, uint32_t n, uint32_t leaf_max,
+                   WideBuild* out) {
+    if (n == 0) return fail(VH_ERR_USAGE, "cannot build a BVH over zero primitives");
+    const uint32_t n_int = n > 1 ? n - 1 : 0;
+    DeviceBuffer bounds, keys, keys_sorted, ids, ids_sorted, left, right, parent, leaf_parent, first, last, flags, nb,
+        tmp, qa, qb, counters, sah;
+    VH_HIP(bounds.allocate(6 * sizeof(uint32_t)));
+    VH_HIP(keys.allocate(sizeof(uint64_t) * n));
+    VH_HIP(keys_sorted.allocate(sizeof(uint64_t) * n));
+    VH_HIP(ids.allocate(sizeof(uint32_t) * n));
+    VH_HIP(ids_sorted.allocate(sizeof(uint32_t) * n));
+    VH_HIP(left.allocate(sizeof(uint32_t) * (n_int + 1)));
+    VH_HIP(right.allocate(sizeof(uint32_t) * (n_int + 1)));
+    VH_HIP(parent.allocate(sizeof(uint32_t) * (n_int + 1)));
+    VH_HIP(leaf_parent.allocate(sizeof(uint32_t) * n));
+    VH_HIP(first.allocate(sizeof(uint32_t) * (n_int + 1)));
+    VH_HIP(last.allocate(sizeof(uint32_t) * (n_int + 1)));
+    VH_HIP(flags.allocate(sizeof(uint32_t) * (n_int + 1)));
+    VH_HIP(nb.allocate(sizeof(float) * 6 * (n_int + 1)));
+    VH_HIP(counters.allocate(sizeof(uint32_t) * 8));
+    VH_HIP(sah.allocate(sizeof(double)));
+
+    hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, s, bounds.as<uint32_t>());
+    uint32_t rb = std::min<uint32_t>(blocks(n, 256), 1024);
+    hipLaunchKernelGGL(k_centroid_bounds, dim3(rb), dim3(256), 0, s, d_lo, d_hi, n, bounds.as<uint32_t>());
+    hipLaunchKernelGGL(k_morton, dim3(blocks(n, 256)), dim3(256), 0, s, d_lo, d_hi, n, bounds.as<uint32_t>(),
+                       keys.as<uint64_t>(), ids.as<uint32_t>());
+    size_t tmp_bytes = 0;
+    VH_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, keys.as<uint64_t>(), keys_sorted.as<uint64_t>(),
+                                     ids.as<uint32_t>(), ids_sorted.as<uint32_t>(), n, 0, 63, s));
+    VH_HIP(tmp.allocate(tmp_bytes));
+    VH_HIP(rocprim::radix_sort_pairs(tmp.ptr, tmp_bytes, keys.as<uint64_t>(), keys_sorted.as<uint64_t>(),
+                                     ids.as<uint32_t>(), ids_sorted.as<uint32_t>(), n, 0, 63, s));
+    if (n > 1) {
+        hipLaunchKernelGGL(k_karras, dim3(blocks(n_int, 256)), dim3(256), 0, s, keys_sorted.as<uint64_t>(), (int)n,
+                           left.as<uint32_t>(), right.as<uint32_t>(), parent.as<uint32_t>(),
+                           leaf_parent.as<uint32_t>(), first.as<uint32_t>(), last.as<uint32_t>());
+        VH_HIP(hipMemsetAsync(flags.ptr, 0, sizeof(uint32_t) * n_int, s));
+        hipLaunchKernelGGL(k_refit, dim3(blocks(n, 256)), dim3(256), 0, s, d_lo, d_hi, ids_sorted.as<uint32_t>(), n,
+                           left.as<uint32_t>(), right.as<uint32_t>(), parent.as<uint32_t>(),
+                           leaf_parent.as<uint32_t>(), flags.as<uint32_t>(), nb.as<float>());
+    }
+    // Upper bound of wide nodes: every wide node but the root has > leaf_max
+    // primitives under it except through the tree shape; n internal binary
+    // nodes bound it.
+    uint32_t max_nodes = std::max<uint32_t>(n_int, 1) + 1;
+    VH_HIP(out->nodes.allocate(sizeof(WideNode) * max_nodes));
+    VH_HIP(out->order.allocate(sizeof(uint32_t) * n));
+    VH_HIP(qa.allocate(sizeof(QueueItem) * max_nodes));
+    VH_HIP(qb.allocate(sizeof(QueueItem) * max_nodes));
+    // counters: 0 node_counter, 1 item_counter, 2 count_a, 3 count_b, 4 max_level
+    uint32_t init[8] = {1, 0, 1, 0, 0, 0, 0, 0};
+    VH_HIP(hipMemcpyAsync(counters.ptr, init, sizeof(init), hipMemcpyHostToDevice, s));
+    QueueItem root = {n > 1 ? 0u : kNone, 0u};
+    VH_HIP(hipMemcpyAsync(qa.ptr, &root, sizeof(root), hipMemcpyHostToDevice, s));
+    Tree t{left.as<uint32_t>(), right.as<uint32_t>(), first.as<uint32_t>(), last.as<uint32_t>(),
+           ids_sorted.as<uint32_t>(), nb.as<float>(), d_lo, d_hi};
+    uint32_t* cnt = counters.as<uint32_t>();
+    uint32_t count = 1, level = 0;
+    QueueItem *qin = qa.as<QueueItem>(), *qout = qb.as<QueueItem>();
+    uint32_t *cin = cnt + 2, *cout = cnt + 3;
+    while (count) {
+        VH_HIP(hipMemsetAsync(cout, 0, sizeof(uint32_t), s));
+        hipLaunchKernelGGL(k_collapse, dim3(blocks(count, 128)), dim3(128), 0, s, t, qin, cin, qout, cout, cnt, cnt + 1,
+                           out->nodes.as<WideNode>(), out->order.as<uint32_t>(), leaf_max, n, cnt + 4, level);
+        VH_HIP(hipMemcpyAsync(&count, cout, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        VH_HIP(hipStreamSynchronize(s));
+        std::swap(qin, qout);
+        std::swap(cin, cout);
+        level++;
+    }
+    uint32_t host_counters[8];
+    VH_HIP(hipMemcpyAsync(host_counters, cnt, sizeof(host_counters), hipMemcpyDeviceToHost, s));
+    double sah_sum = 0;
+    if (n_int) {
+        VH_HIP(hipMemsetAsync(sah.ptr, 0, sizeof(double), s));
+        hipLaunchKernelGGL(k_sah, dim3(std::min<uint32_t>(blocks(n_int, 256), 512)), dim3(256), 0, s, nb.as<float>(),
+                           n_int, sah.as<double>());
+        VH_HIP(hipMemcpyAsync(&sah_sum, sah.ptr, sizeof(double), hipMemcpyDeviceToHost, s));
+    }
+    VH_HIP(hipStreamSynchronize(s));
+    if (host_counters[1] != n) return fail(VH_ERR_DEVICE, "BVH collapse lost primitives");
+    out->node_count = host_counters[0];
+    out->binary_nodes = n_int;
+    out->max_depth = level;
+    out->sah_cost = sah_sum + (double)n;
+    return VH_OK;
+}
+
+// ------------------------------------------------------------ mesh kernels
+namespace {
+__global__ void k_compact_positions(const float* positions, uint32_t stride, uint32_t vertex_count, float* out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= vertex_count) return;
+    const float* p = positions + (size_t)i * stride;
+    out[3 * i] = p[0]; out[3 * i + 1] = p[1]; out[3 * i + 2] = p[2];
+}
+__global__ void k_tri_bounds(const float* pos, const uint32_t* idx, uint32_t n, uint32_t vertex_count, float* lo,
+                             float* hi, uint32_t* bad) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t a = idx[3 * i], b = idx[3 * i + 1], c = idx[3 * i + 2];
+    if (a >= vertex_count || b >= vertex_count || c >= vertex_count) {
+        atomicOr(bad, 1u);
+        a = b = c = 0;
+    }
+    for (int k = 0; k < 3; k++) {
+        float x = pos[3 * a + k], y = pos[3 * b + k], z = pos[3 * c + k];
+        if (!isfinite(x) || !isfinite(y) || !isfinite(z)) atomicOr(bad, 2u);
+        lo[3 * i + k] = fminf(x, fminf(y, z));
+        hi[3 * i + k] = fmaxf(x, fmaxf(y, z));
+    }
+}
+__global__ void k_gather_triangles(const float* pos, const uint32_t* idx, const uint32_t* order, uint32_t n, float* tris,
+                                   uint32_t* tri_ids) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t prim = order[i];
+    for (int v = 0; v < 3; v++) {
+        uint32_t vi = idx[3 * prim + v];
+        for (int k = 0; k < 3; k++) tris[9 * (size_t)i + 3 * v + k] = pos[3 * vi + k];
+    }
+    tri_ids[i] = prim;
+}
+__global__ void k_instance_bounds(const InstanceRecord* inst, const BlasRecord* blases, uint32_t n, float* lo, float* hi) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const InstanceRecord& r = inst[i];
+    float l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
+    if (r.mask_flags & (1u << 9)) {
+        const BlasRecord& b = blases[r.blas];
+        for (int corner = 0; corner < 8; corner++) {
+            f3 p = mk(corner & 1 ? b.hi[0] : b.lo[0], corner & 2 ? b.hi[1] : b.lo[1], corner & 4 ? b.hi[2] : b.lo[2]);
+            f3 w = xform_point(r.o2w, p);
+            for (int k = 0; k < 3; k++) { l[k] = fminf(l[k], comp(w, k)); h[k] = fmaxf(h[k], comp(w, k)); }
+        }
+        // The box only guides traversal into the instance: widen it well past
+        // the rounding of the world-to-object ray transform.
+        for (int k = 0; k < 3; k++) {
+            float e = (fabsf(l[k]) + fabsf(h[k])) * 64.f * FLT_EPSILON + 1e-30f;
+            l[k] -= e; h[k] += e;
+        }
+    } else {
+        for (int k = 0; k < 3; k++) { l[k] = h[k] = 0.f; }
+    }
+    for (int k = 0; k < 3; k++) { lo[3 * i + k] = l[k]; hi[3 * i + k] = h[k]; }
+}
+__global__ void k_compact3(const float* src, uint32_t stride, uint32_t count, float* dst) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const float* p = src + (size_t)i * stride;
+    dst[3 * i] = p[0]; dst[3 * i + 1] = p[1]; dst[3 * i + 2] = p[2];
+}
+__global__ void k_bounds_reduce(const float* lo, const float* hi, uint32_t n, uint32_t* out) {
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        for (int k = 0; k < 3; k++) { mn[k] = fminf(mn[k], lo[3 * i + k]); mx[k] = fmaxf(mx[k], hi[3 * i + k]); }
+    __shared__ float smn[3][256], smx[3][256];
+    for (int k = 0; k < 3; k++) { smn[k][threadIdx.x] = mn[k]; smx[k][threadIdx.x] = mx[k]; }
+    __syncthreads();
+    for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int k = 0; k < 3; k++) {
+                smn[k][threadIdx.x] = fminf(smn[k][threadIdx.x], smn[k][threadIdx.x + s]);
+                smx[k][threadIdx.x] = fmaxf(smx[k][threadIdx.x], smx[k][threadIdx.x + s]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; k++) {
+            atomicMin(&out[k], float_to_ordered(smn[k][0]));
+            atomicMax(&out[3 + k], float_to_ordered(smx[k][0]));
+        }
+}
+__global__ void k_ordered_to_float(const uint32_t* in, float* out) {
+    if (threadIdx.x < 6) out[threadIdx.x] = ordered_to_float(in[threadIdx.x]);
+}
+// Instance records: world-to-object rows (double-precision inverse), mask 0
+// for a singular or non-finite transform or an unknown BLAS, as the placement
+// kernel of crates/vork-render/src/gpu_tlas.rs:70-118 writes for such rows.
+__global__ void k_instance_records(const vh_instance* inst, uint32_t n, uint32_t blas_count, InstanceRecord* out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const vh_instance& s = inst[i];
+    InstanceRecord r;
+    for (int k = 0; k < 12; k++) r.o2w[k] = s.transform[k];
+    bool valid = invert_transform(s.transform, r.w2o) && s.blas < blas_count;
+    if (!valid)
+        for (int k = 0; k < 12; k++) r.w2o[k] = 0.f;
+    r.blas = valid ? s.blas : 0u;
+    r.custom_index = s.custom_index & 0xffffffu;
+    r.material = s.material;
+    r.mask_flags = (valid ? (uint32_t)s.mask : 0u) | ((s.flags & VH_INSTANCE_CULL_BACK_FACES) ? (1u << 8) : 0u) |
+                   (valid ? (1u << 9) : 0u);
+    out[i] = r;
+}
+}  // namespace
+
+int launch_compact3(hipStream_t s, const float* src, uint32_t stride_floats, uint32_t count, float* dst) {
+    hipLaunchKernelGGL(k_compact3, dim3(blocks(count, 256)), dim3(256), 0, s, src, stride_floats, count, dst);
+    VH_HIP(hipGetLastError());
+    return VH_OK;
+}
+
+int launch_bounds_reduce(hipStream_t s, const float* lo, const float* hi, uint32_t n, float out[6]) {
+    DeviceBuffer b, f;
+    VH_HIP(b.allocate(6 * sizeof(uint32_t)));
+    VH_HIP(f.allocate(6 * sizeof(float)));
+    hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, s, b.as<uint32_t>());
+    hipLaunchKernelGGL(k_bounds_reduce, dim3(std::min<uint32_t>(blocks(n, 256), 1024)), dim3(256), 0, s, lo, hi, n,
+                       b.as<uint32_t>());
+    hipLaunchKernelGGL(k_ordered_to_float, dim3(1), dim3(64), 0, s, b.as<uint32_t>(), f.as<float>());
+    VH_HIP(hipMemcpyAsync(out, f.ptr, 6 * sizeof(float), hipMemcpyDeviceToHost, s));
+    VH_HIP(hipStreamSynchronize(s));
+    return VH_OK;
+}
+
+int launch_instance_records(hipStream_t s, const vh_instance* inst, uint32_t n, uint32_t blas_count,
+                            InstanceRecord* out) {
+    hipLaunchKernelGGL(k_instance_records, dim3(blocks(n, 256)), dim3(256), 0, s, inst, n, blas_count, out);
+    VH_HIP(hipGetLastError());
+    return VH_OK;
+}
+
+int launch_triangle_bounds(hipStream_t s, const float* positions, uint32_t stride_floats, const uint32_t* indices,
+                           uint32_t n, float* lo, float* hi, float* compact_positions, uint32_t vertex_count) {
+    DeviceBuffer bad;
+    VH_HIP(bad.allocate(sizeof(uint32_t)));
+    VH_HIP(hipMemsetAsync(bad.ptr, 0, sizeof(uint32_t), s));
+    hipLaunchKernelGGL(k_compact_positions, dim3(blocks(vertex_count, 256)), dim3(256), 0, s, positions, stride_floats,
+                       vertex_count, compact_positions);
+    hipLaunchKernelGGL(k_tri_bounds, dim3(blocks(n, 256)), dim3(256), 0, s, compact_positions, indices, n, vertex_count,
+                       lo, hi, bad.as<uint32_t>());
+    uint32_t h = 0;
+    VH_HIP(hipMemcpyAsync(&h, bad.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    VH_HIP(hipStreamSynchronize(s));
+    if (h & 1u) return fail(VH_ERR_SCENE, "triangle index out of range of the vertex count");
+    if (h & 2u) return fail(VH_ERR_SCENE, "non-finite vertex position");
+    return VH_OK;
+}
+
+int launch_gather_triangles(hipStream_t s, const float* positions, const uint32_t* indices, const uint32_t* order,
+                            uint32_t n, float* tris, uint32_t* tri_ids) {
+    hipLaunchKernelGGL(k_gather_triangles, dim3(blocks(n, 256)), dim3(256), 0, s, positions, indices, order, n, tris,
+                       tri_ids);
+    VH_HIP(hipGetLastError());
+    return VH_OK;
+}
+
+int launch_instance_bounds(hipStream_t s, const InstanceRecord* inst, const BlasRecord* blases, uint32_t n, float* lo,
+                           float* hi) {
+    hipLaunchKernelGGL(k_instance_bounds, dim3(blocks(n, 256)), dim3(256), 0, s, inst, blases, n, lo, hi);
+    VH_HIP(hipGetLastError());
+    return VH_OK;
+}
+
+}  // namespace vh

@@ -1,0 +1,164 @@
+// Acceleration-structure layouts and the traversal of the hot path.
+//
+// A BLAS is a compressed 8-wide BVH over one mesh: 80-byte nodes holding an
+// fp32 origin, per-axis power-of-two scales and 8-bit child boxes, plus a
+// triangle array (three fp32 vertices, 36 bytes) in leaf order.  A TLAS is
+// the same 8-wide layout whose leaves are instances; a leaf child of a TLAS
+// node names exactly one instance.  Traversal runs one ray per lane with a
+// per-lane stack in LDS, visiting children near-to-far.
+#pragma once
+
+#include "vh_math.h"
+
+namespace vh {
+
+// ------------------------------------------------------------ node layout
+// 20 dwords = 80 bytes; loaded as five 16-byte vectors.
+//  d0-2  origin xyz (fp32, the node's lower corner)
+//  d3    exponents ex, ey, ez (biased by 127: scale = 2^(e-127)), child count
+//  d4    first internal child node index (internal children are contiguous)
+//  d5    first item index of leaf children (items are contiguous)
+//  d6-7  meta[8]: 0x80|k = internal child k (node child_base + k);
+//        leaf: bits 0-4 item offset from item_base, bits 5-6 item count - 1
+//  d8-19 qlo x[8], qlo y[8], qlo z[8], qhi x[8], qhi y[8], qhi z[8]
+struct alignas(16) WideNode {
+    float origin[3];
+    uint8_t ex, ey, ez, count;
+    uint32_t child_base;
+    uint32_t item_base;
+    uint8_t meta[8];
+    uint8_t qlo[3][8];
+    uint8_t qhi[3][8];
+};
+static_assert(sizeof(WideNode) == 80, "wide node is 80 bytes");
+
+constexpr int kWidth = 8;
+constexpr int kLeafMax = 4;          // items per leaf child
+constexpr uint32_t kMetaInternal = 0x80u;
+
+VH_HD float exp_scale(uint8_t e) { return u2f(uint32_t(e) << 23); }
+
+// Decoded child bound on one axis: origin + q * 2^e, rounded as written.  The
+// builder verifies these decoded values enclose the exact child bounds.
+VH_HD float decode(float origin, uint8_t q, float scale) { VH_EXACT return origin + float(q) * scale; }
+
+// ------------------------------------------------------------ records
+// One BLAS on the device: its node and triangle arrays plus the shading data
+// of the mesh (original indexed vertices, normals and triangle materials).
+struct BlasRecord {
+    const WideNode* nodes;
+    const float* tris;        // 9 floats per triangle, leaf order
+    const uint32_t* tri_ids;  // original primitive index per leaf-order triangle
+    const float* positions;   // 3 per vertex
+    const float* normals;     // 3 per vertex or null
+    const uint32_t* indices;  // 3 per triangle
+    const uint32_t* materials;  // per triangle or null
+    float lo[3], hi[3];       // local bounds
+    uint32_t triangle_count, node_count;
+};
+
+// One TLAS instance.  w2o are the world-to-object rows computed on the host in
+// double precision and rounded to float (the oracle's vo_invert_transform).
+struct InstanceRecord {
+    float o2w[12];
+    float w2o[12];
+    uint32_t blas;
+    uint32_t custom_index;
+    uint32_t material;
+    uint32_t mask_flags;  // bits 0-7 mask, bit 8 cull back faces, bit 9 valid
+};
+
+struct TlasView {
+    const WideNode* nodes;
+    const uint32_t* inst_ids;   // leaf item -> instance index
+    const InstanceRecord* instances;
+    const BlasRecord* blases;
+    uint32_t node_count;
+    uint32_t instance_count;
+};
+
+// ------------------------------------------------------------ primitives
+// Watertight crossing of crates/vork-render/shaders/vork_medium_domain.slang:
+// 40-66 (Woop, Benthin and Wald 2013 with a symbolic tie at exact zeros).
+VH_HD float volume_edge(float px, float py, float qx, float qy) {
+    VH_EXACT
+    bool forward = px < qx || (px == qx && py < qy);
+    float lox = forward ? px : qx, loy = forward ? py : qy;
+    float hix = forward ? qx : px, hiy = forward ? qy : py;
+    float area = lox * hiy - loy * hix;
+    float tie = px != qx ? 1.f : (py != qy ? -1.f : 0.f);
+    float canonical = area != 0.f ? area : tie;
+    return forward ? canonical : -canonical;
+}
+
+// Per-ray shear of the watertight test: axis permutation and shear factors.
+struct Shear {
+    int kx, ky, kz;
+    float sx, sy;
+};
+VH_HD Shear make_shear(f3 d) {
+    VH_EXACT
+    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    int z = ax > ay ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+    int x = (z + 1) % 3, y = (x + 1) % 3;
+    if (comp(d, z) < 0.f) { int s = x; x = y; y = s; }
+    Shear s;
+    s.kx = x; s.ky = y; s.kz = z;
+    s.sx = comp(d, x) / comp(d, z);
+    s.sy = comp(d, y) / comp(d, z);
+    return s;
+}
+
+// Closest-hit triangle test: determinant (|det| > 1e-20, culled when det < 0
+// for culling instances), watertight crossing, then Moller-Trumbore t and
+// clamped barycentrics (crates/vork-render/shaders/camera_ray.slang:24-35).
+VH_HD bool intersect_triangle(f3 a, f3 b, f3 c, f3 o, f3 d, const Shear& sh, float tmin, float tmax, bool cull,
+                              float* t_out, float* u_out, float* v_out) {
+    VH_EXACT
+    f3 e1 = b - a, e2 = c - a;
+    f3 q = cross(d, e2);
+    float det = dot(e1, q);
+    if (!(fabsf(det) > 1e-20f)) return false;
+    if (cull && det < 0.f) return false;
+    f3 va = a - o, vb = b - o, vc = c - o;
+    float pax = comp(va, sh.kx) - sh.sx * comp(va, sh.kz), pay = comp(va, sh.ky) - sh.sy * comp(va, sh.kz);
+    float pbx = comp(vb, sh.kx) - sh.sx * comp(vb, sh.kz), pby = comp(vb, sh.ky) - sh.sy * comp(vb, sh.kz);
+    float pcx = comp(vc, sh.kx) - sh.sx * comp(vc, sh.kz), pcy = comp(vc, sh.ky) - sh.sy * comp(vc, sh.kz);
+    float eu = volume_edge(pbx, pby, pcx, pcy);
+    float ev = volume_edge(pcx, pcy, pax, pay);
+    float ew = volume_edge(pax, pay, pbx, pby);
+    bool crosses = (eu > 0.f && ev > 0.f && ew > 0.f) || (eu < 0.f && ev < 0.f && ew < 0.f);
+    if (!crosses) return false;
+    f3 s = o - a;
+    f3 r = cross(s, e1);
+    float t = dot(e2, r) / det;
+    if (!(t > tmin && t < tmax)) return false;
+    float u = fmaxf(dot(s, q) / det, 0.f), v = fmaxf(dot(d, r) / det, 0.f);
+    if (u + v > 1.f) { float k = u + v; u = u / k; v = v / k; }
+    *t_out = t; *u_out = u; *v_out = v;
+    return true;
+}
+
+// Ray/box slab test, conservative by (1 + 2 gamma(3)) on the far distance.
+// Returns the entry distance, or +inf when the box is missed.
+VH_HD float box_enter(float lox, float loy, float loz, float hix, float hiy, float hiz, f3 o, f3 inv, float tmin,
+                      float tmax) {
+    VH_EXACT
+    float t0x = (lox - o.x) * inv.x, t1x = (hix - o.x) * inv.x;
+    float t0y = (loy - o.y) * inv.y, t1y = (hiy - o.y) * inv.y;
+    float t0z = (loz - o.z) * inv.z, t1z = (hiz - o.z) * inv.z;
+    float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
+    float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tmax));
+    return tn <= tf * (1.f + 2.f * kGamma3) ? tn : __builtin_huge_valf();
+}
+
+VH_HD f3 safe_inv(f3 d) { VH_EXACT return mk(1.f / d.x, 1.f / d.y, 1.f / d.z); }
+
+// (t, instance, primitive) lexicographic order: the rule at equal t.
+VH_HD bool better_hit(float t, uint32_t inst, uint32_t prim, float bt, uint32_t binst, uint32_t bprim) {
+    if (t < bt) return true;
+    if (t > bt) return false;
+    return inst < binst || (inst == binst && prim < bprim);
+}
+
+}  // namespace vh

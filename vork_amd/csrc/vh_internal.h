@@ -1,0 +1,146 @@
+// Library-internal objects behind the opaque handles of include/vork_hip.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/vork_hip.h"
+#include "vh_bvh.h"
+
+namespace vh {
+
+// Status + thread-local message; every C entry point funnels failures here.
+int fail(int code, const std::string& message);
+int hip_fail(hipError_t e, const char* what);
+#define VH_HIP(call)                                   \
+    do {                                               \
+        hipError_t _e = (call);                        \
+        if (_e != hipSuccess) return hip_fail(_e, #call); \
+    } while (0)
+
+// A device allocation that frees itself.
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    ~DeviceBuffer() { release(); }
+    hipError_t allocate(size_t n) {
+        release();
+        if (n == 0) n = 16;
+        hipError_t e = hipMalloc(&ptr, n);
+        if (e == hipSuccess) bytes = n;
+        return e;
+    }
+    // Grows to at least n bytes; contents are not preserved.
+    hipError_t reserve(size_t n) { return n <= bytes ? hipSuccess : allocate(n); }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    template <class T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+}  // namespace vh
+
+struct vh_device {
+    int ordinal = 0;
+    hipStream_t stream = nullptr;
+    int cu_count = 0;
+    // GGX energy tables (moments 128x128 float4, dielectric 65x64x64 float)
+    vh::DeviceBuffer lut_moments, lut_dielectric;
+    // shared scratch of the build and trace kernels
+    vh::DeviceBuffer stack_overflow;
+    vh::DeviceBuffer scratch_u32;
+};
+
+struct vh_blas {
+    vh_device* device = nullptr;
+    vh::DeviceBuffer nodes, tris, tri_ids, positions, normals, indices, materials;
+    vh::BlasRecord record{};
+    vh_blas_info info{};
+};
+
+struct vh_tlas {
+    vh_device* device = nullptr;
+    vh::DeviceBuffer nodes, inst_ids, instances, blas_records;
+    std::vector<vh_blas*> blases;
+    uint32_t instance_count = 0;
+    uint32_t node_count = 0;
+    vh::TlasView view() const {
+        vh::TlasView v;
+        v.nodes = nodes.as<vh::WideNode>();
+        v.inst_ids = inst_ids.as<uint32_t>();
+        v.instances = instances.as<vh::InstanceRecord>();
+        v.blases = blas_records.as<vh::BlasRecord>();
+        v.node_count = node_count;
+        v.instance_count = instance_count;
+        return v;
+    }
+};
+
+// Light-tree node (crates/vork-render/shaders/light_tree.slang:3-8).
+struct vh_light_node {
+    float lower_power[4];
+    float upper_infinite[4];
+    uint32_t links[4];
+    uint32_t source[4];
+};
+
+struct vh_scene {
+    vh_device* device = nullptr;
+    const vh_tlas* tlas = nullptr;
+    vh::DeviceBuffer materials, lights, light_tree, filter;
+    uint32_t material_count = 0, light_count = 0, light_nodes = 0;
+    vh_camera camera{};
+    vh_sky sky{};
+    // wavefront state, grown on demand
+    vh::DeviceBuffer paths, queue_a, queue_b, hits, shadow, samples, counters;
+    size_t capacity = 0;
+    vh_render_stats stats{};
+};
+
+namespace vh {
+
+// ---------------------------------------------------------------- build
+// Generic LBVH over n boxes, collapsed to 8-wide nodes.  `lo`/`hi` are device
+// arrays of 3 floats per primitive.  Writes the nodes and the leaf-order
+// permutation `order` (leaf item -> primitive) and returns the node count.
+struct WideBuild {
+    DeviceBuffer nodes;   // WideNode[]
+    DeviceBuffer order;   // uint32 per item
+    uint32_t node_count = 0;
+    uint32_t binary_nodes = 0;
+    uint32_t max_depth = 0;
+    double sah_cost = 0;
+};
+int build_wide_bvh(vh_device* dev, hipStream_t stream, const float* d_lo, const float* d_hi, uint32_t n,
+                   uint32_t leaf_max, WideBuild* out);
+
+// Kernels of the BLAS build that depend on the mesh layout.
+int launch_triangle_bounds(hipStream_t s, const float* positions, uint32_t stride_floats, const uint32_t* indices,
+                           uint32_t n, float* lo, float* hi, float* compact_positions, uint32_t vertex_count);
+int launch_gather_triangles(hipStream_t s, const float* positions, const uint32_t* indices, const uint32_t* order,
+                            uint32_t n, float* tris, uint32_t* tri_ids);
+int launch_instance_bounds(hipStream_t s, const InstanceRecord* inst, const BlasRecord* blases, uint32_t n, float* lo,
+                           float* hi);
+int launch_compact3(hipStream_t s, const float* src, uint32_t stride_floats, uint32_t count, float* dst);
+int launch_bounds_reduce(hipStream_t s, const float* lo, const float* hi, uint32_t n, float out[6]);
+int launch_instance_records(hipStream_t s, const vh_instance* inst, uint32_t n, uint32_t blas_count,
+                            InstanceRecord* out);
+
+// ---------------------------------------------------------------- trace
+int launch_trace(vh_device* dev, hipStream_t s, const TlasView& tv, const void* rays, uint32_t count, uint32_t mask,
+                 void* out, int mode);  // mode 0 closest -> vh_hit, 1 any -> uint8, 2 counts -> uint32 pairs
+
+// ------------------------------------------------------------------ LUT
+int bake_luts(vh_device* dev);
+
+// --------------------------------------------------------------- render
+int render_scene(vh_scene* scene, const vh_render_params* p, float* film, hipStream_t s);
+
+}  // namespace vh

@@ -1,0 +1,689 @@
+// Wavefront path integrator: path_pixel + integrate of
+// crates/vork-render/shaders/path.slang:1313-1969 split into stages that each
+// run over a queue of live paths.
+//
+//   camera  -> one path per (pixel, exposure sample) of the batch
+//   extend  -> closest hit of every live path's ray
+//   shade   -> sky on a miss, else the closure, a BSDF sample, both direct
+//              estimators of next_event (their visibility deferred as shadow
+//              rays) and the continuation with caps and roulette
+//   shadow  -> any-hit of the pending shadow rays; adds the clamped sum
+//   film    -> per pixel, the batch's samples summed in sample order into the
+//              RGBA32F film (one thread per pixel, no atomics)
+//
+// Every path owns its state slot, so results do not depend on queue order.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "vh_internal.h"
+#include "vh_shade.h"
+#include "vh_traverse.h"
+
+namespace vh {
+namespace {
+
+constexpr int kBlock = 256;
+
+// flags word of a path
+constexpr uint32_t kEventShift = 0, kDiffuseShift = 8, kGlossyShift = 12, kBounceShift = 16;
+constexpr uint32_t kNonFinite = 1u << 30;
+
+struct Paths {
+    float4* ray_o;    // origin, tmax
+    float4* ray_d;    // direction
+    float4* thr;      // throughput
+    float4* rad;      // radiance
+    uint4* info;      // pixel, sample index, rng, flags
+    float4* hit;      // t, u, v, primitive bits
+    uint32_t* hit_inst;
+    float4* s0o;      // shadow ray 0: origin, distance (distance 0: none)
+    float4* s0d;
+    float4* s1o;      // shadow ray 1
+    float4* s1d;
+    float4* pend;     // throughput x estimator 0 (rgb), clamp limit
+    float4* pend1;    // throughput x estimator 1 (rgb)
+};
+
+struct Consts {
+    uint32_t width, height, samples, seed;
+    uint32_t max_bounces, diffuse_bounces, glossy_bounces, caustics;
+    float clamp_direct, clamp_indirect, light_threshold;
+    uint32_t light_count, light_nodes, material_count;
+    uint32_t batch_first, batch_samples, pixels;
+    vh_camera camera;
+    vh_sky sky;
+};
+
+struct Scene {
+    TlasView tv;
+    const vh_material* materials;
+    const vh_rect_light* lights;
+    const LightNode* tree;
+    const float* filter;
+    Luts luts;
+};
+
+// counters: [0] live count in, [1] live count out, [2] shadow count,
+// [3] dropped non-finite contributions, [4..5] extension rays (u64),
+// [6..7] shadow rays (u64), [8] errors, [9] non-finite samples zeroed,
+// [16..19] extension node / triangle visits, [20..23] shadow node / triangle visits (u64)
+__global__ void k_camera(Paths P, Consts c, Scene S, uint32_t n, uint32_t* queue, uint32_t* counters) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p == 0) {
+        counters[0] = n;
+        counters[1] = 0;
+        counters[2] = 0;
+    }
+    if (p >= n) return;
+    uint32_t s_local = p / c.pixels, pixel = p % c.pixels;
+    uint32_t px = pixel % c.width, py = pixel / c.width;
+    uint32_t index = c.batch_first + s_local;
+    Sampler smp = make_sampler(px, py, index, c.samples, c.seed);
+    float cs[4];
+    group_values(smp, 0u, cs);
+    // path.slang:1727-1775 with camera_ray_line (vork_prelude.slang:95-110): pinhole
+    const vh_camera& cam = c.camera;
+    float ndcx = ((float)px + .5f + filter_offset(S.filter, cs[0])) / (float)c.width * 2.f + -1.f;
+    float ndcy = ((float)py + .5f + filter_offset(S.filter, cs[1])) / (float)c.height * -2.f + 1.f;
+    f3 fwd = load3(cam.forward), right = load3(cam.right), up = load3(cam.up), eye = load3(cam.eye);
+    f3 d = normalize(fwd + right * (ndcx * cam.tan_half_x) + up * (ndcy * cam.tan_half_y));
+    float fd = dot(d, fwd);
+    float near_distance = cam.clip_start / fd, far_distance = cam.clip_end / fd;
+    f3 o = eye + d * near_distance;
+    float tmax = fmaxf(0.f, far_distance - length(o - eye));
+    P.ray_o[p] = make_float4(o.x, o.y, o.z, tmax);
+    P.ray_d[p] = make_float4(d.x, d.y, d.z, 0.f);
+    P.thr[p] = make_float4(1.f, 1.f, 1.f, 0.f);
+    P.rad[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+    P.info[p] = make_uint4(pixel, index, screen_ray_hash(px, py, index, c.seed), VH_RAY_CAMERA << kEventShift);
+    queue[p] = p;
+}
+
+template <bool kAny, bool kCount>
+__device__ __forceinline__ bool trace_ray(const TlasView& tv, f3 o, f3 d, float tmax, uint32_t mask, Stack& st,
+                                          TraceHit& h, TraceCounts& tc) {
+    return traverse<kAny, kCount>(tv, o, d, 0.f, tmax, mask, st, h, tc);
+}
+
+// Sums per-thread visit counts of a block into two u64 counters.
+template <bool kCount>
+__device__ __forceinline__ void flush_counts(uint64_t nodes, uint64_t tris, unsigned long long* out) {
+    if (!kCount) return;
+    __shared__ unsigned long long sn, st_;
+    if (threadIdx.x == 0) { sn = 0; st_ = 0; }
+    __syncthreads();
+    atomicAdd(&sn, (unsigned long long)nodes);
+    atomicAdd(&st_, (unsigned long long)tris);
+    __syncthreads();
+    if (threadIdx.x == 0) { atomicAdd(&out[0], sn); atomicAdd(&out[1], st_); }
+}
+
+template <bool kCount>
+__global__ __launch_bounds__(kBlock) void k_extend(Paths P, Scene S, const uint32_t* queue, uint32_t* counters,
+                                                   uint32_t* overflow) {
+    __shared__ uint32_t lds[kLdsStack * kBlock];
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, threads = gridDim.x * blockDim.x;
+    Stack st{lds + threadIdx.x, blockDim.x, overflow + gid, threads, 0};
+    const uint32_t n = counters[0];
+    uint64_t nodes = 0, tris = 0;
+    for (uint32_t i = gid; i < n; i += threads) {
+        uint32_t p = queue[i];
+        float4 ro = P.ray_o[p], rd = P.ray_d[p];
+        uint32_t mask = (P.info[p].w >> kEventShift) & 0xffu;
+        TraceHit h;
+        TraceCounts tc{0, 0};
+        if (!trace_ray<false, kCount>(S.tv, mk(ro.x, ro.y, ro.z), mk(rd.x, rd.y, rd.z), ro.w, mask, st, h, tc))
+            atomicOr(&counters[8], kStackOverflow);
+        nodes += tc.nodes;
+        tris += tc.tris;
+        P.hit[p] = make_float4(h.found ? h.t : -1.f, h.u, h.v, u2f(h.prim));
+        P.hit_inst[p] = h.inst;
+    }
+    if (gid == 0) atomicAdd((unsigned long long*)&counters[4], (unsigned long long)n);
+    flush_counts<kCount>(nodes, tris, (unsigned long long*)&counters[16]);
+}
+
+// material.slang:69-79 with the instance's local bounds
+__device__ __forceinline__ f3 surface_position_error(const InstanceRecord& r, const BlasRecord& b, f3 world) {
+    const float* model = r.o2w;
+    const float* inv = r.w2o;
+    f3 magnitude = mk(fmaxf(fabsf(b.lo[0]), fabsf(b.hi[0])), fmaxf(fabsf(b.lo[1]), fabsf(b.hi[1])),
+                      fmaxf(fabsf(b.lo[2]), fabsf(b.hi[2])));
+    f3 span = mk(fabsf(b.hi[0] - b.lo[0]), fabsf(b.hi[1] - b.lo[1]), fabsf(b.hi[2] - b.lo[2]));
+    float extent = fmaxf(span.x, fmaxf(span.y, span.z));
+    f3 aw = vabs(world);
+    f3 li = mk(fabsf(inv[0]) * aw.x + fabsf(inv[1]) * aw.y + fabsf(inv[2]) * aw.z + fabsf(inv[3]),
+               fabsf(inv[4]) * aw.x + fabsf(inv[5]) * aw.y + fabsf(inv[6]) * aw.z + fabsf(inv[7]),
+               fabsf(inv[8]) * aw.x + fabsf(inv[9]) * aw.y + fabsf(inv[10]) * aw.z + fabsf(inv[11]));
+    f3 le = (magnitude + mk(extent, extent, extent) + li) * kGamma7;
+    f3 a = mk(fabsf(model[0]) * le.x + fabsf(model[1]) * le.y + fabsf(model[2]) * le.z,
+              fabsf(model[4]) * le.x + fabsf(model[5]) * le.y + fabsf(model[6]) * le.z,
+              fabsf(model[8]) * le.x + fabsf(model[9]) * le.y + fabsf(model[10]) * le.z);
+    f3 bb = mk(fabsf(model[0]) * magnitude.x + fabsf(model[1]) * magnitude.y + fabsf(model[2]) * magnitude.z + fabsf(model[3]),
+               fabsf(model[4]) * magnitude.x + fabsf(model[5]) * magnitude.y + fabsf(model[6]) * magnitude.z + fabsf(model[7]),
+               fabsf(model[8]) * magnitude.x + fabsf(model[9]) * magnitude.y + fabsf(model[10]) * magnitude.z + fabsf(model[11]));
+    return a + bb * kGamma7;
+}
+
+struct Surface {
+    f3 world, face, smooth;
+    uint32_t material;
+};
+// hit_geometry.slang:120-135 and the double-sided facing of the hit
+__device__ __forceinline__ Surface reconstruct(const Scene& S, uint32_t material_count, const InstanceRecord& r,
+                                               const BlasRecord& b, uint32_t prim, float u, float v, f3 direction) {
+    Surface sf;
+    const uint32_t i0 = b.indices[3 * prim], i1 = b.indices[3 * prim + 1], i2 = b.indices[3 * prim + 2];
+    f3 p0 = load3(b.positions + 3 * i0), p1 = load3(b.positions + 3 * i1), p2 = load3(b.positions + 3 * i2);
+    float b1 = u, b2 = v, b0 = 1.f - b1 - b2;
+    f3 p = p0 * b0 + p1 * b1 + p2 * b2;
+    sf.world = xform_point(r.o2w, p);
+    f3 face_o = cross(p1 - p0, p2 - p0);
+    f3 n0 = b.normals ? load3(b.normals + 3 * i0) : face_o;
+    f3 n1 = b.normals ? load3(b.normals + 3 * i1) : face_o;
+    f3 n2 = b.normals ? load3(b.normals + 3 * i2) : face_o;
+    face_o = finite_unit_normal(face_o, n0);
+    // normal matrix rows = columns of world-to-object
+    const float* w = r.w2o;
+    f3 face = finite_unit_normal(mk(w[0] * face_o.x + w[4] * face_o.y + w[8] * face_o.z,
+                                    w[1] * face_o.x + w[5] * face_o.y + w[9] * face_o.z,
+                                    w[2] * face_o.x + w[6] * face_o.y + w[10] * face_o.z),
+                                 face_o);
+    f3 ns = n0 * b0 + n1 * b1 + n2 * b2;
+    ns = mk(w[0] * ns.x + w[4] * ns.y + w[8] * ns.z, w[1] * ns.x + w[5] * ns.y + w[9] * ns.z,
+            w[2] * ns.x + w[6] * ns.y + w[10] * ns.z);
+    ns = finite_unit_normal(ns, face);
+    if (dot(ns, face) < 0.f) ns = -ns;
+    if (dot(face, direction) > 0.f) {
+        face = -face;
+        ns = -ns;
+    }
+    sf.face = face;
+    sf.smooth = ns;
+    uint32_t local = b.materials ? b.materials[prim] : 0u;
+    uint32_t m = r.material + local;
+    sf.material = m < material_count ? m : material_count - 1;
+    return sf;
+}
+// path.slang:772-781 (no terminator offset, no hair)
+__device__ __forceinline__ f3 path_origin(const Surface& sf, f3 error, f3 direction) {
+    bool reflection = dot(sf.smooth, direction) > 0.f;
+    return surface_ray_origin(sf.world, error, reflection ? sf.smooth : sf.face, direction);
+}
+
+__global__ __launch_bounds__(kBlock) void k_shade(Paths P, Consts c, Scene S, const uint32_t* queue, uint32_t* next,
+                                                  uint32_t* shadow_queue, uint32_t* counters) {
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, threads = gridDim.x * blockDim.x;
+    const uint32_t n = counters[0];
+    uint32_t dropped = 0;
+    for (uint32_t i = gid; i < n; i += threads) {
+        const uint32_t p = queue[i];
+        uint4 info = P.info[p];
+        uint32_t flags = info.w;
+        const uint32_t bounce = (flags >> kBounceShift) & 0xffu;
+        uint32_t diffuse = (flags >> kDiffuseShift) & 0xfu, glossy = (flags >> kGlossyShift) & 0xfu;
+        float4 t4 = P.thr[p], r4 = P.rad[p], rd4 = P.ray_d[p];
+        f3 throughput = mk(t4.x, t4.y, t4.z), radiance = mk(r4.x, r4.y, r4.z), direction = mk(rd4.x, rd4.y, rd4.z);
+        const float limit = bounce == 0 ? c.clamp_direct : c.clamp_indirect;
+        float4 h = P.hit[p];
+        if (h.x < 0.f) {
+            radiance = radiance + clamp_sample(throughput * sky_radiance(c.sky, direction), limit, dropped);
+            P.rad[p] = make_float4(radiance.x, radiance.y, radiance.z, r4.w);
+            continue;
+        }
+        const uint32_t inst = P.hit_inst[p], prim = f2u(h.w);
+        const InstanceRecord& ir = S.tv.instances[inst];
+        const BlasRecord& br = S.tv.blases[ir.blas];
+        Surface sf = reconstruct(S, c.material_count, ir, br, prim, h.y, h.z, direction);
+        const vh_material& mat = S.materials[sf.material];
+        Closure cl;
+        cl.base_color = load3(mat.base_color);
+        cl.metal = mat.base_metalness;
+        cl.spec_weight = mat.specular_weight;
+        cl.roughness = mat.specular_roughness;
+        cl.ior = mat.specular_ior;
+        cl.n = sf.smooth;
+        const f3 error = surface_position_error(ir, br, sf.world);
+        const uint32_t px = info.x % c.width, py = info.x / c.width;
+        Sampler smp = make_sampler(px, py, info.y, c.samples, c.seed);
+        const uint32_t vertex = bounce;
+        float scatter[4];
+        group_values(smp, 1u + vertex * 5u + 0u, scatter);
+        const f3 view = -direction;
+        const f3 interface_face = dot(sf.smooth, view) > 0.f ? sf.smooth : sf.face;
+        Prep pr = transport_prepare(S.luts, cl, view, sf.smooth, interface_face);
+        BsdfSample bs = transport_sample(cl, view, scatter[0], scatter[1], scatter[2], pr);
+        if (!finite3(bs.direction) || !finite3(bs.weight) || !isfinite(bs.pdf)) {
+            P.info[p].w = flags | kNonFinite;
+            continue;
+        }
+        // ---- next_event (path.slang:851-952), visibility deferred
+        uint32_t rng = info.z;
+        f3 a = mk(0.f, 0.f, 0.f), b = mk(0.f, 0.f, 0.f);
+        float4 s0o = make_float4(0.f, 0.f, 0.f, 0.f), s0d = s0o, s1o = s0o, s1d = s0o;
+        if (c.light_count) {
+            float u[4];
+            group_values(smp, 1u + vertex * 5u + 1u, u);
+            float probability;
+            uint32_t li = light_pick(S.tree, c.light_nodes, sf.world, u[2], u[3], &probability);
+            if (probability > 0.f) {
+                const vh_rect_light l = S.lights[li];
+                f3 ldir;
+                EmitEval src = rect_sample(l, sf.world, u[0], u[1], &ldir);
+                f3 power = src.pdf > 0.f ? src.radiance / src.pdf : mk(0.f, 0.f, 0.f);
+                f3 response = analytic_mis_response(cl, view, ldir, sf.face, src.pdf, pr);
+                a = (response * power) / probability;
+                float threshold = c.light_threshold;
+                float importance = maxc(a);
+                if (importance < threshold && threshold > 0.f) {
+                    float keep = importance / threshold;
+                    if (path_random(rng) >= keep) a = mk(0.f, 0.f, 0.f); else a = a / keep;
+                }
+                if (nonzero(a)) {
+                    f3 origin = path_origin(sf, error, ldir);
+                    f3 cd;
+                    float cdist;
+                    surface_ray_connection(origin, sf.world, ldir, src.distance, &cd, &cdist);
+                    s0o = make_float4(origin.x, origin.y, origin.z, cdist);
+                    s0d = make_float4(cd.x, cd.y, cd.z, 0.f);
+                }
+                bool continuation = bs.event != 0 && event_hemisphere(bs.event, bs.direction, sf.face, sf.smooth);
+                if (continuation && dot(bs.direction, bs.direction) > 0.f) {
+                    EmitEval e = rect_eval(l, sf.world, bs.direction);
+                    if (e.pdf > 0.f) {
+                        float weight = bs.pdf > 0.f ? mis_power(bs.pdf, e.pdf) : 1.f;
+                        b = (bs.weight * e.radiance) * (weight / probability);
+                        float imp2 = maxc(b);
+                        if (imp2 < threshold && threshold > 0.f) {
+                            float keep = imp2 / threshold;
+                            if (path_random(rng) >= keep) b = mk(0.f, 0.f, 0.f); else b = b / keep;
+                        }
+                        if (nonzero(b)) {
+                            f3 origin = path_origin(sf, error, bs.direction);
+                            f3 cd;
+                            float cdist;
+                            surface_ray_connection(origin, sf.world, bs.direction, e.distance, &cd, &cdist);
+                            s1o = make_float4(origin.x, origin.y, origin.z, cdist);
+                            s1d = make_float4(cd.x, cd.y, cd.z, 0.f);
+                        }
+                    }
+                }
+            }
+        }
+        bool pending = nonzero(a) || nonzero(b);
+        if (pending) {
+            f3 ta = throughput * a, tb = throughput * b;
+            P.pend[p] = make_float4(ta.x, ta.y, ta.z, limit);
+            P.pend1[p] = make_float4(tb.x, tb.y, tb.z, 0.f);
+            P.s0o[p] = s0o;
+            P.s0d[p] = s0d;
+            P.s1o[p] = s1o;
+            P.s1d[p] = s1d;
+            shadow_queue[atomicAdd(&counters[2], 1u)] = p;
+        }
+        // ---- continuation (path.slang:1680-1705)
+        bool alive = true;
+        if (bounce == c.max_bounces) alive = false;
+        uint32_t event = bs.event;
+        if (alive) {
+            if (event == VH_RAY_DIFFUSE) {
+                if (diffuse++ >= c.diffuse_bounces) alive = false;
+            } else if (event == VH_RAY_SPECULAR) {
+                if (glossy++ >= c.glossy_bounces || (diffuse > 0 && (c.caustics & 1u) == 0)) alive = false;
+            } else {
+                alive = false;
+            }
+        }
+        if (alive && !event_hemisphere(event, bs.direction, sf.face, sf.smooth)) alive = false;
+        if (alive) {
+            throughput = throughput * bs.weight;
+            if (!finite3(throughput)) {
+                flags |= kNonFinite;
+                alive = false;
+            } else if (maxc(throughput) <= 0.f) {
+                alive = false;
+            }
+        }
+        if (alive && bounce >= kRouletteStart) {
+            float survive = clampf(maxc(throughput), kRouletteMin, kRouletteMax);
+            float d[4];
+            group_values(smp, 1u + vertex * 5u + 4u, d);
+            if (d[2] > survive) alive = false;
+            else throughput = throughput / survive;
+        }
+        flags = (flags & kNonFinite) | (min(diffuse, 15u) << kDiffuseShift) | (min(glossy, 15u) << kGlossyShift) |
+                ((bounce + 1) << kBounceShift) | (event << kEventShift);
+        P.info[p] = make_uint4(info.x, info.y, rng, flags);
+        if (alive) {
+            f3 origin = path_origin(sf, error, bs.direction);
+            P.ray_o[p] = make_float4(origin.x, origin.y, origin.z, FLT_MAX);
+            P.ray_d[p] = make_float4(bs.direction.x, bs.direction.y, bs.direction.z, 0.f);
+            P.thr[p] = make_float4(throughput.x, throughput.y, throughput.z, 0.f);
+            next[atomicAdd(&counters[1], 1u)] = p;
+        }
+    }
+    if (dropped) atomicAdd(&counters[3], dropped);
+}
+
+template <bool kCount>
+__global__ __launch_bounds__(kBlock) void k_shadow(Paths P, Scene S, const uint32_t* queue, uint32_t* counters,
+                                                   uint32_t* overflow) {
+    __shared__ uint32_t lds[kLdsStack * kBlock];
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, threads = gridDim.x * blockDim.x;
+    Stack st{lds + threadIdx.x, blockDim.x, overflow + gid, threads, 0};
+    const uint32_t n = counters[2];
+    uint32_t dropped = 0, rays = 0;
+    TraceCounts tc{0, 0};
+    for (uint32_t i = gid; i < n; i += threads) {
+        const uint32_t p = queue[i];
+        float4 pa = P.pend[p], pb = P.pend1[p];
+        f3 va = mk(pa.x, pa.y, pa.z), vb = mk(pb.x, pb.y, pb.z);
+        float4 o0 = P.s0o[p], o1 = P.s1o[p];
+        TraceHit h;
+        if (nonzero(va)) {
+            float4 d0 = P.s0d[p];
+            rays++;
+            if (!trace_ray<true, kCount>(S.tv, mk(o0.x, o0.y, o0.z), mk(d0.x, d0.y, d0.z), o0.w, VH_RAY_SHADOW, st, h, tc))
+                atomicOr(&counters[8], kStackOverflow);
+            if (h.found) va = mk(0.f, 0.f, 0.f);
+        }
+        if (nonzero(vb)) {
+            float4 d1 = P.s1d[p];
+            rays++;
+            if (!trace_ray<true, kCount>(S.tv, mk(o1.x, o1.y, o1.z), mk(d1.x, d1.y, d1.z), o1.w, VH_RAY_SHADOW, st, h, tc))
+                atomicOr(&counters[8], kStackOverflow);
+            if (h.found) vb = mk(0.f, 0.f, 0.f);
+        }
+        f3 add = clamp_sample(va + vb, pa.w, dropped);
+        float4 r = P.rad[p];
+        P.rad[p] = make_float4(r.x + add.x, r.y + add.y, r.z + add.z, r.w);
+    }
+    if (dropped) atomicAdd(&counters[3], dropped);
+    if (rays) atomicAdd((unsigned long long*)&counters[6], (unsigned long long)rays);
+    flush_counts<kCount>(tc.nodes, tc.tris, (unsigned long long*)&counters[20]);
+}
+
+// One thread per pixel: adds the batch's samples, in sample order, to the film.
+__global__ void k_film(Paths P, Consts c, float4* film) {
+    uint32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pixel >= c.pixels) return;
+    float4 acc = film[pixel];
+    for (uint32_t s = 0; s < c.batch_samples; s++) {
+        uint32_t p = s * c.pixels + pixel;
+        float4 r = P.rad[p];
+        bool bad = (P.info[p].w & kNonFinite) || !isfinite(r.x) || !isfinite(r.y) || !isfinite(r.z);
+        if (!bad) {
+            acc.x += r.x;
+            acc.y += r.y;
+            acc.z += r.z;
+        }
+        acc.w += 1.f;
+    }
+    film[pixel] = acc;
+}
+
+__global__ void k_swap_counts(uint32_t* counters) {
+    if (threadIdx.x == 0) {
+        counters[0] = counters[1];
+        counters[1] = 0;
+        counters[2] = 0;
+    }
+}
+
+__global__ void k_count_nonfinite(Paths P, uint32_t n, uint32_t* counters) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    float4 r = P.rad[p];
+    if ((P.info[p].w & kNonFinite) || !isfinite(r.x) || !isfinite(r.y) || !isfinite(r.z)) atomicAdd(&counters[9], 1u);
+}
+
+// ------------------------------------------------------------------ LUTs
+// crates/vork-render/shaders/environment.slang:19-27,78-114
+__device__ float brdf_cosine(uint32_t column) {
+    float t = (float)column / (float)(kBrdfSize - 1);
+    float t2 = t * t;
+    return t2 * t2;
+}
+__device__ float brdf_alpha(uint32_t row) {
+    if (row == 0) return kGgxAlphaMin;
+    if (row == kBrdfSize - 1) return 1.f;
+    return exp2f(log2f(kGgxAlphaMin) * (1.f - (float)row / (float)(kBrdfSize - 1)));
+}
+__global__ void k_bake_moments(float4* out) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= kBrdfSize * kBrdfSize) return;
+    uint32_t column = k % kBrdfSize, row = k / kBrdfSize;
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+    float nv = brdf_cosine(column), alpha = brdf_alpha(row);
+    for (uint32_t i = 0; i < kEnvironmentSamples; i++) {
+        float u0 = ((float)i + .5f) / (float)kEnvironmentSamples;
+        float u1 = (float)__builtin_bitreverse32(i) * kRadicalInverseScale;
+        float vh;
+        float g = ggx_visible_reflection(nv, alpha, u0, u1, &vh);
+        float f = powf(1.f - vh, 5.f);
+        s0 += (1.f - f) * g;
+        s1 += f * g;
+        s2 += vh * powf(1.f - vh, 6.f) * g;
+    }
+    float average = 0.f;
+    if (column == 0) {
+        for (uint32_t j = 0; j < kBrdfSize; j++) {
+            float nvj = brdf_cosine(j);
+            float interval = .5f * (brdf_cosine(min(j + 1, kBrdfSize - 1)) - brdf_cosine(j > 0 ? j - 1 : 0));
+            for (uint32_t i = 0; i < kEnvironmentSamples; i++) {
+                float u0 = ((float)i + .5f) / (float)kEnvironmentSamples;
+                float u1 = (float)__builtin_bitreverse32(i) * kRadicalInverseScale;
+                float vh;
+                average += interval * 2.f * nvj * ggx_visible_reflection(nvj, alpha, u0, u1, &vh);
+            }
+        }
+        average /= (float)kEnvironmentSamples;
+    }
+    out[k] = make_float4(s0 / (float)kEnvironmentSamples, s1 / (float)kEnvironmentSamples,
+                         s2 / (float)kEnvironmentSamples, average);
+}
+// openpbr_core.slang:154-166 over the (nv, roughness, eta) grid of environment.slang:40-50
+__global__ void k_bake_dielectric(float* out) {
+    const uint32_t n = kBrdfAlbedoSize;
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (kBrdfIorSteps + 1) * n * n) return;
+    uint32_t slice = k / (n * n), cell = k % (n * n);
+    uint32_t cx = cell % n, cy = cell / n;
+    float q = (float)slice / (float)kBrdfIorSteps;
+    float eta = slice == kBrdfIorSteps ? 0.f : q / fmaxf(1.f - q, 1e-20f);
+    float no = fmaxf((float)cx / (float)(n - 1), kGgxAlphaMin);
+    float r = (float)cy / (float)(n - 1);
+    float result = 0.f;
+    if (eta == 1.f) {
+        result = 0.f;
+    } else if (r == 0.f) {
+        result = eta <= 0.f ? 1.f : fresnel(no, eta);
+    } else {
+        float a = specular_alpha(r);
+        for (uint32_t i = 0; i < kDielectricAlbedoSamples; i++) {
+            float u0 = ((float)i + .5f) / (float)kDielectricAlbedoSamples;
+            float u1 = (float)(__builtin_bitreverse32(i) >> 8) * kUintScale;
+            float vh;
+            float w = ggx_visible_reflection(no, a, u0, u1, &vh);
+            result += (eta <= 0.f ? 1.f : fresnel(vh, eta)) * w;
+        }
+        result /= (float)kDielectricAlbedoSamples;
+    }
+    out[k] = result;
+}
+
+inline uint32_t blocks(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
+
+}  // namespace
+
+int bake_luts(vh_device* dev) {
+    hipStream_t s = dev->stream;
+    VH_HIP(dev->lut_moments.allocate(sizeof(float4) * kBrdfSize * kBrdfSize));
+    VH_HIP(dev->lut_dielectric.allocate(sizeof(float) * (kBrdfIorSteps + 1) * kBrdfAlbedoSize * kBrdfAlbedoSize));
+    hipLaunchKernelGGL(k_bake_moments, dim3(blocks(kBrdfSize * kBrdfSize, 64)), dim3(64), 0, s,
+                       dev->lut_moments.as<float4>());
+    hipLaunchKernelGGL(k_bake_dielectric, dim3(blocks((kBrdfIorSteps + 1) * kBrdfAlbedoSize * kBrdfAlbedoSize, 256)),
+                       dim3(256), 0, s, dev->lut_dielectric.as<float>());
+    VH_HIP(hipGetLastError());
+    VH_HIP(hipStreamSynchronize(s));
+    return VH_OK;
+}
+
+int ensure_overflow(vh_device* dev, uint32_t threads);
+
+int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, hipStream_t s) {
+    vh_device* dev = sc->device;
+    const uint32_t pixels = rp->width * rp->height;
+    const uint32_t total = rp->sample_end - rp->sample_begin;
+    sc->stats = vh_render_stats{};
+    if (total == 0) return VH_OK;
+    uint32_t batch = rp->batch_samples;
+    if (batch == 0) {
+        // about 4M paths in flight, whole exposure samples per batch
+        batch = std::max<uint32_t>(1u, (4u << 20) / std::max<uint32_t>(pixels, 1u));
+    }
+    batch = std::min(batch, total);
+    const size_t cap = (size_t)pixels * batch;
+    if (cap > 0x7fffffffull) return fail(VH_ERR_USAGE, "vh_render: batch too large");
+    if (sc->capacity < cap) {
+        VH_HIP(sc->paths.allocate(cap * (sizeof(float4) * 12 + sizeof(uint32_t))));
+        VH_HIP(sc->queue_a.allocate(cap * sizeof(uint32_t)));
+        VH_HIP(sc->queue_b.allocate(cap * sizeof(uint32_t)));
+        VH_HIP(sc->shadow.allocate(cap * sizeof(uint32_t)));
+        sc->capacity = cap;
+    }
+    VH_HIP(sc->counters.reserve(64 * sizeof(uint32_t)));
+    Paths P;
+    {
+        char* base = sc->paths.as<char>();
+        float4** f4[] = {&P.ray_o, &P.ray_d, &P.thr, &P.rad, (float4**)&P.info, &P.hit,
+                         &P.s0o, &P.s0d, &P.s1o, &P.s1d, &P.pend, &P.pend1};
+        size_t off = 0;
+        for (auto* q : f4) {
+            *q = reinterpret_cast<float4*>(base + off);
+            off += cap * sizeof(float4);
+        }
+        P.hit_inst = reinterpret_cast<uint32_t*>(base + off);
+    }
+    Scene S;
+    S.tv = sc->tlas->view();
+    S.materials = sc->materials.as<vh_material>();
+    S.lights = sc->lights.as<vh_rect_light>();
+    S.tree = sc->light_tree.as<LightNode>();
+    S.filter = sc->filter.as<float>();
+    S.luts.moments = dev->lut_moments.as<float4>();
+    S.luts.dielectric = dev->lut_dielectric.as<float>();
+    Consts c;
+    std::memset(&c, 0, sizeof(c));
+    c.width = rp->width; c.height = rp->height; c.samples = rp->samples; c.seed = rp->seed;
+    c.max_bounces = std::min(rp->max_bounces, 254u);
+    c.diffuse_bounces = std::min(rp->diffuse_bounces, 14u);
+    c.glossy_bounces = std::min(rp->glossy_bounces, 14u);
+    c.caustics = rp->caustics;
+    c.clamp_direct = rp->clamp_direct; c.clamp_indirect = rp->clamp_indirect; c.light_threshold = rp->light_threshold;
+    c.light_count = sc->light_count; c.light_nodes = sc->light_nodes; c.material_count = sc->material_count;
+    c.pixels = pixels;
+    c.camera = sc->camera;
+    c.sky = sc->sky;
+    const uint32_t grid = std::min<uint32_t>(blocks(cap, kBlock), (uint32_t)dev->cu_count * 8u);
+    int rc = ensure_overflow(dev, grid * kBlock);
+    if (rc) return rc;
+    uint32_t* counters = sc->counters.as<uint32_t>();
+    uint32_t* overflow = dev->stack_overflow.as<uint32_t>();
+    VH_HIP(hipMemsetAsync(counters, 0, 64 * sizeof(uint32_t), s));
+    const bool profile = (rp->flags & VH_RENDER_PROFILE) != 0;
+    const bool count = (rp->flags & VH_RENDER_COUNT_VISITS) != 0;
+    // stage events: (stage, start, stop) triples recorded around each launch
+    struct Mark { int stage; hipEvent_t a, b; };
+    std::vector<Mark> marks;
+    auto timed = [&](int stage, auto&& launch) -> int {
+        if (!profile) { launch(); return VH_OK; }
+        Mark m{stage, nullptr, nullptr};
+        VH_HIP(hipEventCreate(&m.a));
+        VH_HIP(hipEventCreate(&m.b));
+        VH_HIP(hipEventRecord(m.a, s));
+        launch();
+        VH_HIP(hipEventRecord(m.b, s));
+        marks.push_back(m);
+        return VH_OK;
+    };
+    hipEvent_t e0, e1;
+    VH_HIP(hipEventCreate(&e0));
+    VH_HIP(hipEventCreate(&e1));
+    VH_HIP(hipEventRecord(e0, s));
+    for (uint32_t first = rp->sample_begin; first < rp->sample_end; first += batch) {
+        const uint32_t nb = std::min(batch, rp->sample_end - first);
+        c.batch_first = first;
+        c.batch_samples = nb;
+        const uint32_t n = pixels * nb;
+        uint32_t* qa = sc->queue_a.as<uint32_t>();
+        uint32_t* qb = sc->queue_b.as<uint32_t>();
+        hipLaunchKernelGGL(k_camera, dim3(blocks(n, kBlock)), dim3(kBlock), 0, s, P, c, S, n, qa, counters);
+        for (uint32_t bounce = 0; bounce <= c.max_bounces; bounce++) {
+            int rc2 = timed(0, [&] {
+                if (count) hipLaunchKernelGGL(k_extend<true>, dim3(grid), dim3(kBlock), 0, s, P, S, qa, counters, overflow);
+                else hipLaunchKernelGGL(k_extend<false>, dim3(grid), dim3(kBlock), 0, s, P, S, qa, counters, overflow);
+            });
+            if (!rc2) rc2 = timed(1, [&] {
+                hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kBlock), 0, s, P, c, S, qa, qb, sc->shadow.as<uint32_t>(),
+                                   counters);
+            });
+            if (!rc2) rc2 = timed(2, [&] {
+                if (count) hipLaunchKernelGGL(k_shadow<true>, dim3(grid), dim3(kBlock), 0, s, P, S,
+                                              sc->shadow.as<uint32_t>(), counters, overflow);
+                else hipLaunchKernelGGL(k_shadow<false>, dim3(grid), dim3(kBlock), 0, s, P, S,
+                                        sc->shadow.as<uint32_t>(), counters, overflow);
+            });
+            if (rc2) return rc2;
+            hipLaunchKernelGGL(k_swap_counts, dim3(1), dim3(64), 0, s, counters);
+            std::swap(qa, qb);
+        }
+        hipLaunchKernelGGL(k_count_nonfinite, dim3(blocks(n, kBlock)), dim3(kBlock), 0, s, P, n, counters);
+        hipLaunchKernelGGL(k_film, dim3(blocks(pixels, kBlock)), dim3(kBlock), 0, s, P, c,
+                           reinterpret_cast<float4*>(film));
+        VH_HIP(hipGetLastError());
+    }
+    VH_HIP(hipEventRecord(e1, s));
+    uint32_t h[24];
+    VH_HIP(hipMemcpyAsync(h, counters, sizeof(h), hipMemcpyDeviceToHost, s));
+    VH_HIP(hipStreamSynchronize(s));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    double stage_ms[3] = {0, 0, 0};
+    uint64_t stage_n[3] = {0, 0, 0};
+    for (const Mark& m : marks) {
+        float t = 0;
+        (void)hipEventElapsedTime(&t, m.a, m.b);
+        stage_ms[m.stage] += t;
+        stage_n[m.stage]++;
+        (void)hipEventDestroy(m.a);
+        (void)hipEventDestroy(m.b);
+    }
+    uint64_t ext, shd, v[4];
+    std::memcpy(&ext, &h[4], 8);
+    std::memcpy(&shd, &h[6], 8);
+    std::memcpy(v, &h[16], 32);
+    sc->stats.extend_ms = stage_ms[0];
+    sc->stats.shade_ms = stage_ms[1];
+    sc->stats.shadow_ms = stage_ms[2];
+    sc->stats.other_ms = profile ? ms - stage_ms[0] - stage_ms[1] - stage_ms[2] : 0.0;
+    sc->stats.extend_launches = stage_n[0];
+    sc->stats.shade_launches = stage_n[1];
+    sc->stats.shadow_launches = stage_n[2];
+    sc->stats.extend_node_visits = v[0];
+    sc->stats.extend_triangle_tests = v[1];
+    sc->stats.shadow_node_visits = v[2];
+    sc->stats.shadow_triangle_tests = v[3];
+    sc->stats.paths = (uint64_t)pixels * total;
+    sc->stats.extension_rays = ext;
+    sc->stats.shadow_rays = shd;
+    sc->stats.dropped_samples = h[9];
+    sc->stats.device_ms = ms;
+    if (h[8] & kStackOverflow) return fail(VH_ERR_DEVICE, "traversal stack overflow during render");
+    return VH_OK;
+}
+
+}  // namespace vh

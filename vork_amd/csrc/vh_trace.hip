@@ -1,0 +1,86 @@
+// Ray-query kernels behind vh_trace_closest / vh_trace_any: persistent
+// workgroups stride over the ray array, one ray per lane.
+#include "vh_internal.h"
+#include "vh_traverse.h"
+
+namespace vh {
+namespace {
+
+constexpr int kTraceBlock = 256;
+
+template <int kMode>
+__global__ __launch_bounds__(kTraceBlock) void k_trace(TlasView tv, const vh_ray* __restrict__ rays, uint32_t n,
+                                                        uint32_t mask, void* __restrict__ out, uint32_t* overflow,
+                                                        uint32_t* errors) {
+    __shared__ uint32_t lds[kLdsStack * kTraceBlock];
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t threads = gridDim.x * blockDim.x;
+    Stack st{lds + threadIdx.x, blockDim.x, overflow + gid, threads, 0};
+    for (uint32_t i = gid; i < n; i += threads) {
+        const vh_ray r = rays[i];
+        f3 o = mk(r.origin[0], r.origin[1], r.origin[2]);
+        f3 d = mk(r.direction[0], r.direction[1], r.direction[2]);
+        TraceHit h;
+        TraceCounts c{0, 0};
+        bool ok;
+        if (kMode == 1) ok = traverse<true, false>(tv, o, d, r.tmin, r.tmax, mask, st, h, c);
+        else if (kMode == 2) ok = traverse<false, true>(tv, o, d, r.tmin, r.tmax, mask, st, h, c);
+        else ok = traverse<false, false>(tv, o, d, r.tmin, r.tmax, mask, st, h, c);
+        if (!ok) atomicOr(errors, kStackOverflow);
+        if (kMode == 0) {
+            vh_hit hit;
+            if (h.found) {
+                hit.t = h.t; hit.instance = h.inst; hit.primitive = h.prim;
+                hit.custom_index = tv.instances[h.inst].custom_index & 0xffffffu;
+                hit.u = h.u; hit.v = h.v;
+            } else {
+                hit.t = -1.f; hit.instance = hit.primitive = hit.custom_index = 0xffffffffu; hit.u = hit.v = 0.f;
+            }
+            static_cast<vh_hit*>(out)[i] = hit;
+        } else if (kMode == 1) {
+            static_cast<uint8_t*>(out)[i] = h.found ? 1 : 0;
+        } else {
+            static_cast<uint32_t*>(out)[2 * i] = c.nodes;
+            static_cast<uint32_t*>(out)[2 * i + 1] = c.tris;
+        }
+    }
+}
+
+}  // namespace
+
+// Workgroups resident at once for the persistent trace kernels.
+uint32_t trace_grid(vh_device* dev, uint32_t n) {
+    uint32_t want = (n + kTraceBlock - 1) / kTraceBlock;
+    uint32_t cap = (uint32_t)dev->cu_count * 8u;
+    return want < cap ? (want ? want : 1u) : cap;
+}
+
+int ensure_overflow(vh_device* dev, uint32_t threads) {
+    size_t need = (size_t)threads * kGlobalStack * sizeof(uint32_t);
+    VH_HIP(dev->stack_overflow.reserve(need));
+    return VH_OK;
+}
+
+int launch_trace(vh_device* dev, hipStream_t s, const TlasView& tv, const void* rays, uint32_t count, uint32_t mask,
+                 void* out, int mode) {
+    if (count == 0) return VH_OK;
+    uint32_t grid = trace_grid(dev, count);
+    int rc = ensure_overflow(dev, grid * kTraceBlock);
+    if (rc) return rc;
+    VH_HIP(dev->scratch_u32.reserve(64));
+    uint32_t* errors = dev->scratch_u32.as<uint32_t>();
+    VH_HIP(hipMemsetAsync(errors, 0, sizeof(uint32_t), s));
+    const vh_ray* r = static_cast<const vh_ray*>(rays);
+    uint32_t* of = dev->stack_overflow.as<uint32_t>();
+    if (mode == 0) hipLaunchKernelGGL(k_trace<0>, dim3(grid), dim3(kTraceBlock), 0, s, tv, r, count, mask, out, of, errors);
+    else if (mode == 1) hipLaunchKernelGGL(k_trace<1>, dim3(grid), dim3(kTraceBlock), 0, s, tv, r, count, mask, out, of, errors);
+    else hipLaunchKernelGGL(k_trace<2>, dim3(grid), dim3(kTraceBlock), 0, s, tv, r, count, mask, out, of, errors);
+    VH_HIP(hipGetLastError());
+    uint32_t h = 0;
+    VH_HIP(hipMemcpyAsync(&h, errors, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    VH_HIP(hipStreamSynchronize(s));
+    if (h & kStackOverflow) return fail(VH_ERR_DEVICE, "traversal stack overflow");
+    return VH_OK;
+}
+
+}  // namespace vh
