@@ -142,3 +142,39 @@ def test_invalid_inputs_raise(renderer):
     with pytest.raises(native.VorkError) as e:
         native.Blas(renderer.device, np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32))
     assert e.value.status == abi.VH_ERR_SCENE
+
+
+def _secondary_rays(sc, ora, n, seed):
+    """Rays that start on surfaces, as bounce and shadow rays do: at the hit
+    point itself, nudged along the face normal, and grazing along it."""
+    rng = np.random.default_rng(seed)
+    prim = random_rays(sc, n, seed=seed, edge_fraction=0.0)
+    h = ora.trace_closest(prim)
+    ok = h["t"] > 0
+    prim, h = prim[ok], h[ok]
+    p = prim[:, :3].astype(np.float64) + prim[:, 4:7].astype(np.float64) * h["t"][:, None]
+    d = rng.normal(size=p.shape)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    # a third graze: nearly perpendicular to the incoming ray
+    g = np.cross(prim[:, 4:7].astype(np.float64), rng.normal(size=p.shape))
+    g /= np.linalg.norm(g, axis=1, keepdims=True)
+    k = np.arange(p.shape[0]) % 3
+    d[k == 2] = g[k == 2]
+    nudge = (rng.uniform(size=(p.shape[0], 1)) < 0.5) * rng.choice([1e-6, 1e-4, 1e-3], size=(p.shape[0], 1))
+    o = p - prim[:, 4:7].astype(np.float64) * nudge
+    rays = np.zeros((p.shape[0], 8), np.float32)
+    rays[:, :3], rays[:, 4:7], rays[:, 7] = o, d, 1e30
+    bounded = rng.uniform(size=p.shape[0]) < 0.3
+    rays[bounded, 7] = rng.uniform(0.01, 5.0, size=bounded.sum())
+    return rays
+
+
+def test_secondary_rays_match_oracle_on_ggx_scene(renderer):
+    sc = S.config1(width=64, height=36, spp=4, sphere_level=6, terrain=128, materials=8)
+    inst = renderer.install(sc)
+    ora = oracle.OracleScene(sc)
+    rays = _secondary_rays(sc, ora, 60000, seed=21)
+    _compare(inst.tlas.trace_closest(rays), ora.trace_closest(rays))
+    assert np.array_equal(inst.tlas.trace_any(rays), ora.trace_any(rays))
+    sub = rays[:3000]
+    _compare(inst.tlas.trace_closest(sub), ora.trace_closest(sub, brute_force=True))

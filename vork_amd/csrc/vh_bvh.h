@@ -38,9 +38,10 @@ constexpr uint32_t kMetaInternal = 0x80u;
 
 VH_HD float exp_scale(uint8_t e) { return u2f(uint32_t(e) << 23); }
 
-// Decoded child bound on one axis: origin + q * 2^e, rounded as written.  The
-// builder verifies these decoded values enclose the exact child bounds.
-VH_HD float decode(float origin, uint8_t q, float scale) { VH_EXACT return origin + float(q) * scale; }
+// Decoded child bound on one axis: origin + q * 2^e with one rounding (fused
+// multiply-add).  The builder verifies these decoded values enclose the exact
+// child bounds, and traversal decodes with the same expression.
+VH_HD float decode(float origin, uint8_t q, float scale) { return __builtin_fmaf(float(q), scale, origin); }
 
 // ------------------------------------------------------------ records
 // One BLAS on the device: its node and triangle arrays plus the shading data

@@ -43,6 +43,7 @@ struct Paths {
     float4* s1d;
     float4* pend;     // throughput x estimator 0 (rgb), clamp limit
     float4* pend1;    // throughput x estimator 1 (rgb)
+    uint8_t* vis;     // per shadow ray (2 per path): 1 = occluded
 };
 
 struct Consts {
@@ -55,6 +56,21 @@ struct Consts {
     vh_sky sky;
 };
 
+// Path slot t of a sample <-> pixel: column-major inside bands of 8 rows, so
+// that 64 consecutive slots (one wave) cover an 8x8 tile.  A bijection on
+// [0, width*height) for any image size.
+__device__ __forceinline__ uint32_t slot_to_pixel(uint32_t t, uint32_t w, uint32_t h) {
+    const uint32_t band = t / (8u * w), r = t % (8u * w);
+    const uint32_t rows = min(8u, h - band * 8u);
+    const uint32_t x = r / rows, y = band * 8u + r % rows;
+    return y * w + x;
+}
+__device__ __forceinline__ uint32_t pixel_to_slot(uint32_t pixel, uint32_t w, uint32_t h) {
+    const uint32_t x = pixel % w, y = pixel / w, band = y / 8u;
+    const uint32_t rows = min(8u, h - band * 8u);
+    return band * 8u * w + x * rows + (y - band * 8u);
+}
+
 struct Scene {
     TlasView tv;
     const vh_material* materials;
@@ -64,19 +80,24 @@ struct Scene {
     Luts luts;
 };
 
-// counters: [0] live count in, [1] live count out, [2] shadow count,
+// counters: [0] live count in, [1] live count out, [2] shadow rays queued,
 // [3] dropped non-finite contributions, [4..5] extension rays (u64),
 // [6..7] shadow rays (u64), [8] errors, [9] non-finite samples zeroed,
-// [16..19] extension node / triangle visits, [20..23] shadow node / triangle visits (u64)
+// [10] extension ray hand-out, [11] shadow ray hand-out, [12] paths with a
+// pending next event, [16..19] extension node / triangle visits,
+// [20..23] shadow node / triangle visits (u64)
 __global__ void k_camera(Paths P, Consts c, Scene S, uint32_t n, uint32_t* queue, uint32_t* counters) {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p == 0) {
         counters[0] = n;
         counters[1] = 0;
         counters[2] = 0;
+        counters[10] = 0;
+        counters[11] = 0;
+        counters[12] = 0;
     }
     if (p >= n) return;
-    uint32_t s_local = p / c.pixels, pixel = p % c.pixels;
+    uint32_t s_local = p / c.pixels, pixel = slot_to_pixel(p % c.pixels, c.width, c.height);
     uint32_t px = pixel % c.width, py = pixel / c.width;
     uint32_t index = c.batch_first + s_local;
     Sampler smp = make_sampler(px, py, index, c.samples, c.seed);
@@ -100,12 +121,6 @@ __global__ void k_camera(Paths P, Consts c, Scene S, uint32_t n, uint32_t* queue
     queue[p] = p;
 }
 
-template <bool kAny, bool kCount>
-__device__ __forceinline__ bool trace_ray(const TlasView& tv, f3 o, f3 d, float tmax, uint32_t mask, Stack& st,
-                                          TraceHit& h, TraceCounts& tc) {
-    return traverse<kAny, kCount>(tv, o, d, 0.f, tmax, mask, st, h, tc);
-}
-
 // Sums per-thread visit counts of a block into two u64 counters.
 template <bool kCount>
 __device__ __forceinline__ void flush_counts(uint64_t nodes, uint64_t tris, unsigned long long* out) {
@@ -119,27 +134,44 @@ __device__ __forceinline__ void flush_counts(uint64_t nodes, uint64_t tris, unsi
     if (threadIdx.x == 0) { atomicAdd(&out[0], sn); atomicAdd(&out[1], st_); }
 }
 
+// Extension rays of the live paths; hits into the path state.
+template <bool kCount>
+struct ExtendSource {
+    Paths P;
+    const uint32_t* queue;
+    uint32_t n;
+    uint64_t* nodes;
+    uint64_t* tris;
+    __device__ uint32_t count() const { return n; }
+    __device__ void load(uint32_t i, f3& o, f3& d, float& t0, float& t1, uint32_t& m) const {
+        const uint32_t p = queue[i];
+        const float4 ro = P.ray_o[p], rd = P.ray_d[p];
+        o = mk(ro.x, ro.y, ro.z);
+        d = mk(rd.x, rd.y, rd.z);
+        t0 = 0.f;
+        t1 = ro.w;
+        m = (P.info[p].w >> kEventShift) & 0xffu;
+    }
+    __device__ void store(uint32_t i, const TraceHit& h, const TraceCounts& c) const {
+        const uint32_t p = queue[i];
+        P.hit[p] = make_float4(h.found ? h.t : -1.f, h.u, h.v, u2f(h.prim));
+        P.hit_inst[p] = h.inst;
+        if (kCount) { *nodes += c.nodes; *tris += c.tris; }
+    }
+};
+
 template <bool kCount>
 __global__ __launch_bounds__(kBlock) void k_extend(Paths P, Scene S, const uint32_t* queue, uint32_t* counters,
                                                    uint32_t* overflow) {
-    __shared__ uint32_t lds[kLdsStack * kBlock];
+    __shared__ uint32_t lds_e[kLdsStack * kBlock];
+    __shared__ float lds_t[kLdsStack * kBlock];
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, threads = gridDim.x * blockDim.x;
-    Stack st{lds + threadIdx.x, blockDim.x, overflow + gid, threads, 0};
+    Stack st{lds_e + threadIdx.x, lds_t + threadIdx.x, blockDim.x, overflow + gid,
+             reinterpret_cast<float*>(overflow) + (size_t)threads * kGlobalStack + gid, threads, 0};
     const uint32_t n = counters[0];
     uint64_t nodes = 0, tris = 0;
-    for (uint32_t i = gid; i < n; i += threads) {
-        uint32_t p = queue[i];
-        float4 ro = P.ray_o[p], rd = P.ray_d[p];
-        uint32_t mask = (P.info[p].w >> kEventShift) & 0xffu;
-        TraceHit h;
-        TraceCounts tc{0, 0};
-        if (!trace_ray<false, kCount>(S.tv, mk(ro.x, ro.y, ro.z), mk(rd.x, rd.y, rd.z), ro.w, mask, st, h, tc))
-            atomicOr(&counters[8], kStackOverflow);
-        nodes += tc.nodes;
-        tris += tc.tris;
-        P.hit[p] = make_float4(h.found ? h.t : -1.f, h.u, h.v, u2f(h.prim));
-        P.hit_inst[p] = h.inst;
-    }
+    ExtendSource<kCount> src{P, queue, n, &nodes, &tris};
+    if (!trace_persistent<false, kCount>(S.tv, src, &counters[10], st)) atomicOr(&counters[8], kStackOverflow);
     if (gid == 0) atomicAdd((unsigned long long*)&counters[4], (unsigned long long)n);
     flush_counts<kCount>(nodes, tris, (unsigned long long*)&counters[16]);
 }
@@ -213,7 +245,7 @@ __device__ __forceinline__ f3 path_origin(const Surface& sf, f3 error, f3 direct
 }
 
 __global__ __launch_bounds__(kBlock) void k_shade(Paths P, Consts c, Scene S, const uint32_t* queue, uint32_t* next,
-                                                  uint32_t* shadow_queue, uint32_t* counters) {
+                                                  uint32_t* shadow_queue, uint32_t* pending_queue, uint32_t* counters) {
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, threads = gridDim.x * blockDim.x;
     const uint32_t n = counters[0];
     uint32_t dropped = 0;
@@ -311,16 +343,17 @@ __global__ __launch_bounds__(kBlock) void k_shade(Paths P, Consts c, Scene S, co
                 }
             }
         }
-        bool pending = nonzero(a) || nonzero(b);
-        if (pending) {
+        const bool has_a = nonzero(a), has_b = nonzero(b);
+        if (has_a || has_b) {
             f3 ta = throughput * a, tb = throughput * b;
             P.pend[p] = make_float4(ta.x, ta.y, ta.z, limit);
             P.pend1[p] = make_float4(tb.x, tb.y, tb.z, 0.f);
-            P.s0o[p] = s0o;
-            P.s0d[p] = s0d;
-            P.s1o[p] = s1o;
-            P.s1d[p] = s1d;
-            shadow_queue[atomicAdd(&counters[2], 1u)] = p;
+            if (has_a) { P.s0o[p] = s0o; P.s0d[p] = s0d; }
+            if (has_b) { P.s1o[p] = s1o; P.s1d[p] = s1d; }
+            pending_queue[atomicAdd(&counters[12], 1u)] = p;
+            uint32_t slot = atomicAdd(&counters[2], (has_a ? 1u : 0u) + (has_b ? 1u : 0u));
+            if (has_a) shadow_queue[slot++] = 2u * p;
+            if (has_b) shadow_queue[slot] = 2u * p + 1u;
         }
         // ---- continuation (path.slang:1680-1705)
         bool alive = true;
@@ -366,42 +399,63 @@ __global__ __launch_bounds__(kBlock) void k_shade(Paths P, Consts c, Scene S, co
     if (dropped) atomicAdd(&counters[3], dropped);
 }
 
+// Shadow rays of the pending next events: entry 2p+k is estimator k of path p.
+template <bool kCount>
+struct ShadowSource {
+    Paths P;
+    const uint32_t* queue;
+    uint32_t n;
+    uint64_t* nodes;
+    uint64_t* tris;
+    __device__ uint32_t count() const { return n; }
+    __device__ void load(uint32_t i, f3& o, f3& d, float& t0, float& t1, uint32_t& m) const {
+        const uint32_t r = queue[i], p = r >> 1;
+        const float4 ro = (r & 1u) ? P.s1o[p] : P.s0o[p];
+        const float4 rd = (r & 1u) ? P.s1d[p] : P.s0d[p];
+        o = mk(ro.x, ro.y, ro.z);
+        d = mk(rd.x, rd.y, rd.z);
+        t0 = 0.f;
+        t1 = ro.w;
+        m = VH_RAY_SHADOW;
+    }
+    __device__ void store(uint32_t i, const TraceHit& h, const TraceCounts& c) const {
+        P.vis[queue[i]] = h.found ? 1 : 0;
+        if (kCount) { *nodes += c.nodes; *tris += c.tris; }
+    }
+};
+
 template <bool kCount>
 __global__ __launch_bounds__(kBlock) void k_shadow(Paths P, Scene S, const uint32_t* queue, uint32_t* counters,
                                                    uint32_t* overflow) {
-    __shared__ uint32_t lds[kLdsStack * kBlock];
+    __shared__ uint32_t lds_e[kLdsStack * kBlock];
+    __shared__ float lds_t[kLdsStack * kBlock];
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, threads = gridDim.x * blockDim.x;
-    Stack st{lds + threadIdx.x, blockDim.x, overflow + gid, threads, 0};
+    Stack st{lds_e + threadIdx.x, lds_t + threadIdx.x, blockDim.x, overflow + gid,
+             reinterpret_cast<float*>(overflow) + (size_t)threads * kGlobalStack + gid, threads, 0};
     const uint32_t n = counters[2];
-    uint32_t dropped = 0, rays = 0;
-    TraceCounts tc{0, 0};
-    for (uint32_t i = gid; i < n; i += threads) {
+    uint64_t nodes = 0, tris = 0;
+    ShadowSource<kCount> src{P, queue, n, &nodes, &tris};
+    if (!trace_persistent<true, kCount>(S.tv, src, &counters[11], st)) atomicOr(&counters[8], kStackOverflow);
+    if (gid == 0) atomicAdd((unsigned long long*)&counters[6], (unsigned long long)n);
+    flush_counts<kCount>(nodes, tris, (unsigned long long*)&counters[20]);
+}
+
+// Adds each pending next event: the clamped sum of its unoccluded estimators
+// (sample_clamp.slang:7-16 over path.slang:925-950's value).
+__global__ void k_resolve(Paths P, const uint32_t* queue, uint32_t* counters) {
+    const uint32_t n = counters[12];
+    uint32_t dropped = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t p = queue[i];
-        float4 pa = P.pend[p], pb = P.pend1[p];
+        const float4 pa = P.pend[p], pb = P.pend1[p];
         f3 va = mk(pa.x, pa.y, pa.z), vb = mk(pb.x, pb.y, pb.z);
-        float4 o0 = P.s0o[p], o1 = P.s1o[p];
-        TraceHit h;
-        if (nonzero(va)) {
-            float4 d0 = P.s0d[p];
-            rays++;
-            if (!trace_ray<true, kCount>(S.tv, mk(o0.x, o0.y, o0.z), mk(d0.x, d0.y, d0.z), o0.w, VH_RAY_SHADOW, st, h, tc))
-                atomicOr(&counters[8], kStackOverflow);
-            if (h.found) va = mk(0.f, 0.f, 0.f);
-        }
-        if (nonzero(vb)) {
-            float4 d1 = P.s1d[p];
-            rays++;
-            if (!trace_ray<true, kCount>(S.tv, mk(o1.x, o1.y, o1.z), mk(d1.x, d1.y, d1.z), o1.w, VH_RAY_SHADOW, st, h, tc))
-                atomicOr(&counters[8], kStackOverflow);
-            if (h.found) vb = mk(0.f, 0.f, 0.f);
-        }
-        f3 add = clamp_sample(va + vb, pa.w, dropped);
-        float4 r = P.rad[p];
+        if (nonzero(va) && P.vis[2 * p]) va = mk(0.f, 0.f, 0.f);
+        if (nonzero(vb) && P.vis[2 * p + 1]) vb = mk(0.f, 0.f, 0.f);
+        const f3 add = clamp_sample(va + vb, pa.w, dropped);
+        const float4 r = P.rad[p];
         P.rad[p] = make_float4(r.x + add.x, r.y + add.y, r.z + add.z, r.w);
     }
     if (dropped) atomicAdd(&counters[3], dropped);
-    if (rays) atomicAdd((unsigned long long*)&counters[6], (unsigned long long)rays);
-    flush_counts<kCount>(tc.nodes, tc.tris, (unsigned long long*)&counters[20]);
 }
 
 // One thread per pixel: adds the batch's samples, in sample order, to the film.
@@ -410,7 +464,7 @@ __global__ void k_film(Paths P, Consts c, float4* film) {
     if (pixel >= c.pixels) return;
     float4 acc = film[pixel];
     for (uint32_t s = 0; s < c.batch_samples; s++) {
-        uint32_t p = s * c.pixels + pixel;
+        uint32_t p = s * c.pixels + pixel_to_slot(pixel, c.width, c.height);
         float4 r = P.rad[p];
         bool bad = (P.info[p].w & kNonFinite) || !isfinite(r.x) || !isfinite(r.y) || !isfinite(r.z);
         if (!bad) {
@@ -428,6 +482,9 @@ __global__ void k_swap_counts(uint32_t* counters) {
         counters[0] = counters[1];
         counters[1] = 0;
         counters[2] = 0;
+        counters[10] = 0;
+        counters[11] = 0;
+        counters[12] = 0;
     }
 }
 
@@ -547,10 +604,11 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, hipStrea
     const size_t cap = (size_t)pixels * batch;
     if (cap > 0x7fffffffull) return fail(VH_ERR_USAGE, "vh_render: batch too large");
     if (sc->capacity < cap) {
-        VH_HIP(sc->paths.allocate(cap * (sizeof(float4) * 12 + sizeof(uint32_t))));
+        VH_HIP(sc->paths.allocate(cap * (sizeof(float4) * 12 + sizeof(uint32_t) + 2)));
         VH_HIP(sc->queue_a.allocate(cap * sizeof(uint32_t)));
         VH_HIP(sc->queue_b.allocate(cap * sizeof(uint32_t)));
-        VH_HIP(sc->shadow.allocate(cap * sizeof(uint32_t)));
+        VH_HIP(sc->shadow.allocate(2 * cap * sizeof(uint32_t)));
+        VH_HIP(sc->hits.allocate(cap * sizeof(uint32_t)));
         sc->capacity = cap;
     }
     VH_HIP(sc->counters.reserve(64 * sizeof(uint32_t)));
@@ -565,6 +623,8 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, hipStrea
             off += cap * sizeof(float4);
         }
         P.hit_inst = reinterpret_cast<uint32_t*>(base + off);
+        off += cap * sizeof(uint32_t);
+        P.vis = reinterpret_cast<uint8_t*>(base + off);
     }
     Scene S;
     S.tv = sc->tlas->view();
@@ -627,7 +687,7 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, hipStrea
             });
             if (!rc2) rc2 = timed(1, [&] {
                 hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kBlock), 0, s, P, c, S, qa, qb, sc->shadow.as<uint32_t>(),
-                                   counters);
+                                   sc->hits.as<uint32_t>(), counters);
             });
             if (!rc2) rc2 = timed(2, [&] {
                 if (count) hipLaunchKernelGGL(k_shadow<true>, dim3(grid), dim3(kBlock), 0, s, P, S,
@@ -636,6 +696,7 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, hipStrea
                                         sc->shadow.as<uint32_t>(), counters, overflow);
             });
             if (rc2) return rc2;
+            hipLaunchKernelGGL(k_resolve, dim3(grid), dim3(kBlock), 0, s, P, sc->hits.as<uint32_t>(), counters);
             hipLaunchKernelGGL(k_swap_counts, dim3(1), dim3(64), 0, s, counters);
             std::swap(qa, qb);
         }

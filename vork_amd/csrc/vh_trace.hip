@@ -8,30 +8,28 @@ namespace {
 
 constexpr int kTraceBlock = 256;
 
+// Rays from a vh_ray array; results to vh_hit / occlusion bytes / visit counts.
 template <int kMode>
-__global__ __launch_bounds__(kTraceBlock) void k_trace(TlasView tv, const vh_ray* __restrict__ rays, uint32_t n,
-                                                        uint32_t mask, void* __restrict__ out, uint32_t* overflow,
-                                                        uint32_t* errors) {
-    __shared__ uint32_t lds[kLdsStack * kTraceBlock];
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t threads = gridDim.x * blockDim.x;
-    Stack st{lds + threadIdx.x, blockDim.x, overflow + gid, threads, 0};
-    for (uint32_t i = gid; i < n; i += threads) {
+struct ArraySource {
+    const vh_ray* rays;
+    uint32_t n, mask;
+    void* out;
+    const InstanceRecord* instances;
+    __device__ uint32_t count() const { return n; }
+    __device__ void load(uint32_t i, f3& o, f3& d, float& t0, float& t1, uint32_t& m) const {
         const vh_ray r = rays[i];
-        f3 o = mk(r.origin[0], r.origin[1], r.origin[2]);
-        f3 d = mk(r.direction[0], r.direction[1], r.direction[2]);
-        TraceHit h;
-        TraceCounts c{0, 0};
-        bool ok;
-        if (kMode == 1) ok = traverse<true, false>(tv, o, d, r.tmin, r.tmax, mask, st, h, c);
-        else if (kMode == 2) ok = traverse<false, true>(tv, o, d, r.tmin, r.tmax, mask, st, h, c);
-        else ok = traverse<false, false>(tv, o, d, r.tmin, r.tmax, mask, st, h, c);
-        if (!ok) atomicOr(errors, kStackOverflow);
+        o = mk(r.origin[0], r.origin[1], r.origin[2]);
+        d = mk(r.direction[0], r.direction[1], r.direction[2]);
+        t0 = r.tmin;
+        t1 = r.tmax;
+        m = mask;
+    }
+    __device__ void store(uint32_t i, const TraceHit& h, const TraceCounts& c) const {
         if (kMode == 0) {
             vh_hit hit;
             if (h.found) {
                 hit.t = h.t; hit.instance = h.inst; hit.primitive = h.prim;
-                hit.custom_index = tv.instances[h.inst].custom_index & 0xffffffu;
+                hit.custom_index = instances[h.inst].custom_index & 0xffffffu;
                 hit.u = h.u; hit.v = h.v;
             } else {
                 hit.t = -1.f; hit.instance = hit.primitive = hit.custom_index = 0xffffffffu; hit.u = hit.v = 0.f;
@@ -44,6 +42,18 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TlasView tv, const vh_ray
             static_cast<uint32_t*>(out)[2 * i + 1] = c.tris;
         }
     }
+};
+
+template <int kMode>
+__global__ __launch_bounds__(kTraceBlock) void k_trace(TlasView tv, ArraySource<kMode> src, uint32_t* overflow,
+                                                        uint32_t* counters) {
+    __shared__ uint32_t lds_e[kLdsStack * kTraceBlock];
+    __shared__ float lds_t[kLdsStack * kTraceBlock];
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, threads = gridDim.x * blockDim.x;
+    Stack st{lds_e + threadIdx.x, lds_t + threadIdx.x, blockDim.x, overflow + gid,
+             reinterpret_cast<float*>(overflow) + (size_t)threads * kGlobalStack + gid, threads, 0};
+    bool ok = trace_persistent<kMode == 1, kMode == 2>(tv, src, &counters[1], st);
+    if (!ok) atomicOr(&counters[0], kStackOverflow);
 }
 
 }  // namespace
@@ -55,8 +65,9 @@ uint32_t trace_grid(vh_device* dev, uint32_t n) {
     return want < cap ? (want ? want : 1u) : cap;
 }
 
+// Overflow stacks: entries then distances, kGlobalStack of each per thread.
 int ensure_overflow(vh_device* dev, uint32_t threads) {
-    size_t need = (size_t)threads * kGlobalStack * sizeof(uint32_t);
+    size_t need = (size_t)threads * kGlobalStack * 2 * sizeof(uint32_t);
     VH_HIP(dev->stack_overflow.reserve(need));
     return VH_OK;
 }
@@ -68,13 +79,19 @@ int launch_trace(vh_device* dev, hipStream_t s, const TlasView& tv, const void* 
     int rc = ensure_overflow(dev, grid * kTraceBlock);
     if (rc) return rc;
     VH_HIP(dev->scratch_u32.reserve(64));
-    uint32_t* errors = dev->scratch_u32.as<uint32_t>();
-    VH_HIP(hipMemsetAsync(errors, 0, sizeof(uint32_t), s));
+    uint32_t* errors = dev->scratch_u32.as<uint32_t>();  // [0] errors, [1] ray counter
+    VH_HIP(hipMemsetAsync(errors, 0, 2 * sizeof(uint32_t), s));
     const vh_ray* r = static_cast<const vh_ray*>(rays);
     uint32_t* of = dev->stack_overflow.as<uint32_t>();
-    if (mode == 0) hipLaunchKernelGGL(k_trace<0>, dim3(grid), dim3(kTraceBlock), 0, s, tv, r, count, mask, out, of, errors);
-    else if (mode == 1) hipLaunchKernelGGL(k_trace<1>, dim3(grid), dim3(kTraceBlock), 0, s, tv, r, count, mask, out, of, errors);
-    else hipLaunchKernelGGL(k_trace<2>, dim3(grid), dim3(kTraceBlock), 0, s, tv, r, count, mask, out, of, errors);
+    if (mode == 0)
+        hipLaunchKernelGGL(k_trace<0>, dim3(grid), dim3(kTraceBlock), 0, s, tv,
+                           ArraySource<0>{r, count, mask, out, tv.instances}, of, errors);
+    else if (mode == 1)
+        hipLaunchKernelGGL(k_trace<1>, dim3(grid), dim3(kTraceBlock), 0, s, tv,
+                           ArraySource<1>{r, count, mask, out, tv.instances}, of, errors);
+    else
+        hipLaunchKernelGGL(k_trace<2>, dim3(grid), dim3(kTraceBlock), 0, s, tv,
+                           ArraySource<2>{r, count, mask, out, tv.instances}, of, errors);
     VH_HIP(hipGetLastError());
     uint32_t h = 0;
     VH_HIP(hipMemcpyAsync(&h, errors, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
