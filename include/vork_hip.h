@@ -165,7 +165,8 @@ typedef struct vh_hit {
 
 /* Closest hit per ray; replaces the RayQuery in trace
  * (crates/vork-render/shaders/path.slang:140-200).  Instances whose mask
- * shares no bit with `mask` are skipped.  `flags` takes VH_DEVICE_POINTERS. */
+ * shares no bit with `mask` are skipped.  `flags` takes VH_DEVICE_POINTERS
+ * (then `rays` must be 16-byte aligned). */
 int vh_trace_closest(const vh_tlas* tlas, const vh_ray* rays, uint32_t count, uint32_t mask,
                      uint32_t flags, void* stream, vh_hit* hits);
 /* Any hit per ray (occluded[i] = 1 or 0); replaces the first-hit query of

@@ -287,7 +287,11 @@ static int trace_common(const vh_tlas* t, const vh_ray* rays, uint32_t count, ui
     VH_HIP(hipSetDevice(dev->ordinal));
     hipStream_t s = pick(dev, stream);
     TlasView tv = t->view();
-    if (flags & VH_DEVICE_POINTERS) return launch_trace(dev, s, tv, rays, count, mask, out, mode);
+    if (flags & VH_DEVICE_POINTERS) {
+        if (reinterpret_cast<uintptr_t>(rays) % 16u)
+            return fail(VH_ERR_USAGE, "vh_trace: device ray array must be 16-byte aligned");
+        return launch_trace(dev, s, tv, rays, count, mask, out, mode);
+    }
     DeviceBuffer dr, dout;
     VH_HIP(dr.allocate(sizeof(vh_ray) * (size_t)count));
     VH_HIP(dout.allocate(out_elem * (size_t)count));

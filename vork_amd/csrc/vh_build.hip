@@ -276,19 +276,47 @@ __global__ void k_collapse(Tree t, const QueueItem* in, const uint32_t* in_count
             nlo[a] = fminf(nlo[a], lo[a]); nhi[a] = fmaxf(nhi[a], hi[a]);
         }
     }
+    // Slot assignment: slot s should come first for rays whose direction has
+    // sign octant s, i.e. hold the child that lies farthest towards -d_s with
+    // d_s = (+-1, +-1, +-1) (minus where bit a of s is set).  Greedy on the
+    // smallest centroid projection over all free (child, slot) pairs.
+    int slot_child[kWidth];
+    for (int q = 0; q < kWidth; q++) slot_child[q] = -1;
+    {
+        float proj[kWidth][3];
+        for (int k = 0; k < m; k++)
+            for (int a = 0; a < 3; a++) proj[k][a] = (clo[a][k] + chi[a][k]) - (nlo[a] + nhi[a]);
+        uint32_t child_free = (1u << m) - 1u, slot_free = 0xffu;
+        for (int round = 0; round < m; round++) {
+            float best = INFINITY;
+            int bk = -1, bq = -1;
+            for (int k = 0; k < m; k++) {
+                if (!((child_free >> k) & 1u)) continue;
+                for (int q = 0; q < kWidth; q++) {
+                    if (!((slot_free >> q) & 1u)) continue;
+                    float cost = ((q & 1) ? -proj[k][0] : proj[k][0]) + ((q & 2) ? -proj[k][1] : proj[k][1]) +
+                                 ((q & 4) ? -proj[k][2] : proj[k][2]);
+                    if (bk < 0 || cost < best) { best = cost; bk = k; bq = q; }
+                }
+            }
+            slot_child[bq] = bk;
+            child_free &= ~(1u << bk);
+            slot_free &= ~(1u << bq);
+        }
+    }
     WideNode node;
     for (int a = 0; a < 3; a++) node.origin[a] = nlo[a];
     uint8_t e[3];
     for (int a = 0; a < 3; a++) {
         uint8_t ql[kWidth], qh[kWidth];
         quantise_axis(nlo[a], nhi[a], m, clo[a], chi[a], &e[a], ql, qh);
-        for (int k = 0; k < kWidth; k++) {
-            node.qlo[a][k] = k < m ? ql[k] : 255;
-            node.qhi[a][k] = k < m ? qh[k] : 0;
+        for (int q = 0; q < kWidth; q++) {
+            const int k = slot_child[q];
+            node.qlo[a][q] = k >= 0 ? ql[k] : 255;
+            node.qhi[a][q] = k >= 0 ? qh[k] : 0;
         }
     }
     node.ex = e[0]; node.ey = e[1]; node.ez = e[2];
-    node.count = (uint8_t)m;
     uint32_t n_internal = 0, n_items = 0;
     for (int k = 0; k < m; k++) {
         bool internal = !(c[k] & kLeafBit) && ref_size(t, c[k]) > leaf_max;
@@ -298,23 +326,26 @@ __global__ void k_collapse(Tree t, const QueueItem* in, const uint32_t* in_count
     uint32_t item_base = n_items ? atomicAdd(item_counter, n_items) : 0u;
     node.child_base = child_base;
     node.item_base = item_base;
-    uint32_t ki = 0, off = 0;
-    for (int k = 0; k < kWidth; k++) node.meta[k] = 0;
-    for (int k = 0; k < m; k++) {
+    uint32_t ki = 0, off = 0, imask = 0;
+    for (int q = 0; q < kWidth; q++) {
+        node.meta[q] = (uint8_t)kMetaNoItems;
+        const int k = slot_child[q];
+        if (k < 0) continue;
         bool internal = !(c[k] & kLeafBit) && ref_size(t, c[k]) > leaf_max;
         if (internal) {
-            node.meta[k] = (uint8_t)(kMetaInternal | ki);
+            imask |= 1u << q;
             uint32_t slot = atomicAdd(out_count, 1u);
             out[slot].binary = c[k];
             out[slot].wide = child_base + ki;
             ki++;
         } else {
             uint32_t cnt = ref_size(t, c[k]), f0 = ref_first(t, c[k]);
-            node.meta[k] = (uint8_t)(off | ((cnt - 1) << 5));
+            node.meta[q] = (uint8_t)(off | ((cnt - 1) << 5));
             for (uint32_t j = 0; j < cnt; j++) order[item_base + off + j] = t.sorted_ids[f0 + j];
             off += cnt;
         }
     }
+    node.imask = (uint8_t)imask;
     nodes[it.wide] = node;
     if (ki) atomicMax(max_level, level + 1);
 }

@@ -4,8 +4,7 @@
 // fp32 origin, per-axis power-of-two scales and 8-bit child boxes, plus a
 // triangle array (three fp32 vertices, 36 bytes) in leaf order.  A TLAS is
 // the same 8-wide layout whose leaves are instances; a leaf child of a TLAS
-// node names exactly one instance.  Traversal runs one ray per lane with a
-// per-lane stack in LDS, visiting children near-to-far.
+// node names exactly one instance.
 #pragma once
 
 #include "vh_math.h"
@@ -15,15 +14,19 @@ namespace vh {
 // ------------------------------------------------------------ node layout
 // 20 dwords = 80 bytes; loaded as five 16-byte vectors.
 //  d0-2  origin xyz (fp32, the node's lower corner)
-//  d3    exponents ex, ey, ez (biased by 127: scale = 2^(e-127)), child count
-//  d4    first internal child node index (internal children are contiguous)
-//  d5    first item index of leaf children (items are contiguous)
-//  d6-7  meta[8]: 0x80|k = internal child k (node child_base + k);
-//        leaf: bits 0-4 item offset from item_base, bits 5-6 item count - 1
+//  d3    exponents ex, ey, ez (biased by 127: scale = 2^(e-127)), imask
+//        (bit s set when slot s holds an internal child)
+//  d4    first internal child node index; the internal children are
+//        contiguous, in slot order
+//  d5    first item index of the leaf children (items are contiguous)
+//  d6-7  meta[8]: leaf slot: bits 0-4 item offset from item_base, bits 5-6
+//        item count - 1; internal or empty slot: kMetaNoItems
 //  d8-19 qlo x[8], qlo y[8], qlo z[8], qhi x[8], qhi y[8], qhi z[8]
+// Slots are assigned per node so that visiting slot s in increasing order of
+// s ^ octant(ray direction) is roughly near-to-far.
 struct alignas(16) WideNode {
     float origin[3];
-    uint8_t ex, ey, ez, count;
+    uint8_t ex, ey, ez, imask;
     uint32_t child_base;
     uint32_t item_base;
     uint8_t meta[8];
@@ -34,7 +37,7 @@ static_assert(sizeof(WideNode) == 80, "wide node is 80 bytes");
 
 constexpr int kWidth = 8;
 constexpr int kLeafMax = 4;          // items per leaf child
-constexpr uint32_t kMetaInternal = 0x80u;
+constexpr uint32_t kMetaNoItems = 0x80u;
 
 VH_HD float exp_scale(uint8_t e) { return u2f(uint32_t(e) << 23); }
 
@@ -138,19 +141,6 @@ VH_HD bool intersect_triangle(f3 a, f3 b, f3 c, f3 o, f3 d, const Shear& sh, flo
     if (u + v > 1.f) { float k = u + v; u = u / k; v = v / k; }
     *t_out = t; *u_out = u; *v_out = v;
     return true;
-}
-
-// Ray/box slab test, conservative by (1 + 2 gamma(3)) on the far distance.
-// Returns the entry distance, or +inf when the box is missed.
-VH_HD float box_enter(float lox, float loy, float loz, float hix, float hiy, float hiz, f3 o, f3 inv, float tmin,
-                      float tmax) {
-    VH_EXACT
-    float t0x = (lox - o.x) * inv.x, t1x = (hix - o.x) * inv.x;
-    float t0y = (loy - o.y) * inv.y, t1y = (hiy - o.y) * inv.y;
-    float t0z = (loz - o.z) * inv.z, t1z = (hiz - o.z) * inv.z;
-    float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
-    float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tmax));
-    return tn <= tf * (1.f + 2.f * kGamma3) ? tn : __builtin_huge_valf();
 }
 
 VH_HD f3 safe_inv(f3 d) { VH_EXACT return mk(1.f / d.x, 1.f / d.y, 1.f / d.z); }

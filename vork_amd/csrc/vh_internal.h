@@ -99,7 +99,7 @@ struct vh_scene {
     vh_camera camera{};
     vh_sky sky{};
     // wavefront state, grown on demand
-    vh::DeviceBuffer paths, queue_a, queue_b, hits, shadow, samples, counters;
+    vh::DeviceBuffer paths, rays_a, rays_b, shadow_rays, pending, counters;
     size_t capacity = 0;
     vh_render_stats stats{};
 };

@@ -12,6 +12,12 @@
 // Expressions that must round as written (no FMA contraction): the oracle is
 // compiled with -ffp-contract=off and this keeps the two bit-identical.
 #define VH_EXACT _Pragma("clang fp contract(off)")
+// Address-space qualifiers for device pointers whose space the compiler
+// cannot infer (pointers loaded from memory, per-lane stacks that may live in
+// LDS or global memory): they make it emit global_* / ds_* instead of flat_*
+// accesses.
+#define VH_GLOBAL __attribute__((address_space(1)))
+#define VH_LDS __attribute__((address_space(3)))
 
 namespace vh {
 
@@ -41,6 +47,15 @@ VH_HD float lerpf(float a, float b, float s) { VH_EXACT return a * (1.f - s) + b
 VH_HD f3 lerp3(f3 a, f3 b, float s) { return a * (1.f - s) + b * s; }
 VH_HD f3 reflect3(f3 i, f3 n) { return i - n * (2.f * dot(n, i)); }
 VH_HD f3 load3(const float* p) { return mk(p[0], p[1], p[2]); }
+
+// Plain 4 x u32 vector (HIP's uint4 is a class whose members only take
+// generic-address-space objects).
+typedef uint32_t vu4 __attribute__((ext_vector_type(4)));
+
+template <class T>
+__device__ __forceinline__ const VH_GLOBAL T* as_global(const T* p) {
+    return (const VH_GLOBAL T*)p;
+}
 
 VH_HD uint32_t f2u(float f) { return __builtin_bit_cast(uint32_t, f); }
 VH_HD float u2f(uint32_t u) { return __builtin_bit_cast(float, u); }

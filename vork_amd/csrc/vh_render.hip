@@ -30,21 +30,26 @@ constexpr uint32_t kEventShift = 0, kDiffuseShift = 8, kGlossyShift = 12, kBounc
 constexpr uint32_t kNonFinite = 1u << 30;
 
 struct Paths {
-    float4* ray_o;    // origin, tmax
-    float4* ray_d;    // direction
     float4* thr;      // throughput
     float4* rad;      // radiance
     uint4* info;      // pixel, sample index, rng, flags
     float4* hit;      // t, u, v, primitive bits
     uint32_t* hit_inst;
-    float4* s0o;      // shadow ray 0: origin, distance (distance 0: none)
-    float4* s0d;
-    float4* s1o;      // shadow ray 1
-    float4* s1d;
     float4* pend;     // throughput x estimator 0 (rgb), clamp limit
     float4* pend1;    // throughput x estimator 1 (rgb)
     uint8_t* vis;     // per shadow ray (2 per path): 1 = occluded
 };
+
+// Ray queues hold 32-byte records, float4 pairs: [2i] origin and tmax,
+// [2i+1] direction and an id word.  Extension rays carry path | event << 24
+// (the event of the previous vertex is the ray's visibility mask), shadow
+// rays 2 * path + estimator.  Paths of one batch are therefore limited to
+// 2^24.
+constexpr uint32_t kPathBits = 24, kPathMask = (1u << kPathBits) - 1u;
+__device__ __forceinline__ void put_ray(float4* q, uint32_t i, f3 o, float tmax, f3 d, uint32_t id) {
+    q[2 * i] = make_float4(o.x, o.y, o.z, tmax);
+    q[2 * i + 1] = make_float4(d.x, d.y, d.z, u2f(id));
+}
 
 struct Consts {
     uint32_t width, height, samples, seed;
@@ -83,18 +88,23 @@ struct Scene {
 // counters: [0] live count in, [1] live count out, [2] shadow rays queued,
 // [3] dropped non-finite contributions, [4..5] extension rays (u64),
 // [6..7] shadow rays (u64), [8] errors, [9] non-finite samples zeroed,
-// [10] extension ray hand-out, [11] shadow ray hand-out, [12] paths with a
-// pending next event, [16..19] extension node / triangle visits,
-// [20..23] shadow node / triangle visits (u64)
-__global__ void k_camera(Paths P, Consts c, Scene S, uint32_t n, uint32_t* queue, uint32_t* counters) {
+// [12] paths with a pending next event, [16..19] extension node / triangle
+// visits, [20..23] shadow node / triangle visits (u64), [24] extension ray
+// claims, [25] shadow ray claims
+constexpr int kExtendClaims = 24, kShadowClaims = 25;
+
+__device__ __forceinline__ void reset_claims(uint32_t* counters) {
+    counters[kExtendClaims] = 0;
+    counters[kShadowClaims] = 0;
+}
+__global__ void k_camera(Paths P, Consts c, Scene S, uint32_t n, float4* rays, uint32_t* counters) {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p == 0) {
         counters[0] = n;
         counters[1] = 0;
         counters[2] = 0;
-        counters[10] = 0;
-        counters[11] = 0;
         counters[12] = 0;
+        reset_claims(counters);
     }
     if (p >= n) return;
     uint32_t s_local = p / c.pixels, pixel = slot_to_pixel(p % c.pixels, c.width, c.height);
@@ -113,12 +123,10 @@ __global__ void k_camera(Paths P, Consts c, Scene S, uint32_t n, uint32_t* queue
     float near_distance = cam.clip_start / fd, far_distance = cam.clip_end / fd;
     f3 o = eye + d * near_distance;
     float tmax = fmaxf(0.f, far_distance - length(o - eye));
-    P.ray_o[p] = make_float4(o.x, o.y, o.z, tmax);
-    P.ray_d[p] = make_float4(d.x, d.y, d.z, 0.f);
+    put_ray(rays, p, o, tmax, d, p | (VH_RAY_CAMERA << kPathBits));
     P.thr[p] = make_float4(1.f, 1.f, 1.f, 0.f);
     P.rad[p] = make_float4(0.f, 0.f, 0.f, 0.f);
     P.info[p] = make_uint4(pixel, index, screen_ray_hash(px, py, index, c.seed), VH_RAY_CAMERA << kEventShift);
-    queue[p] = p;
 }
 
 // Sums per-thread visit counts of a block into two u64 counters.
@@ -138,22 +146,26 @@ __device__ __forceinline__ void flush_counts(uint64_t nodes, uint64_t tris, unsi
 template <bool kCount>
 struct ExtendSource {
     Paths P;
-    const uint32_t* queue;
+    const float4* rays;
     uint32_t n;
     uint64_t* nodes;
     uint64_t* tris;
     __device__ uint32_t count() const { return n; }
-    __device__ void load(uint32_t i, f3& o, f3& d, float& t0, float& t1, uint32_t& m) const {
-        const uint32_t p = queue[i];
-        const float4 ro = P.ray_o[p], rd = P.ray_d[p];
-        o = mk(ro.x, ro.y, ro.z);
-        d = mk(rd.x, rd.y, rd.z);
-        t0 = 0.f;
-        t1 = ro.w;
-        m = (P.info[p].w >> kEventShift) & 0xffu;
+    __device__ void fetch(uint32_t i, RayData& r) const {
+        r.a = rays[2 * i];
+        r.b = rays[2 * i + 1];
     }
-    __device__ void store(uint32_t i, const TraceHit& h, const TraceCounts& c) const {
-        const uint32_t p = queue[i];
+    __device__ void unpack(uint32_t, const RayData& r, f3& o, f3& d, float& t0, float& t1, uint32_t& m,
+                           uint32_t& key) const {
+        o = mk(r.a.x, r.a.y, r.a.z);
+        d = mk(r.b.x, r.b.y, r.b.z);
+        t0 = 0.f;
+        t1 = r.a.w;
+        const uint32_t id = f2u(r.b.w);
+        m = id >> kPathBits;
+        key = id & kPathMask;
+    }
+    __device__ void store(uint32_t p, const TraceHit& h, const TraceCounts& c) const {
         P.hit[p] = make_float4(h.found ? h.t : -1.f, h.u, h.v, u2f(h.prim));
         P.hit_inst[p] = h.inst;
         if (kCount) { *nodes += c.nodes; *tris += c.tris; }
@@ -161,17 +173,16 @@ struct ExtendSource {
 };
 
 template <bool kCount>
-__global__ __launch_bounds__(kBlock) void k_extend(Paths P, Scene S, const uint32_t* queue, uint32_t* counters,
+VH_TRACE_KERNEL void k_extend(Paths P, Scene S, const float4* rays, uint32_t* counters,
                                                    uint32_t* overflow) {
-    __shared__ uint32_t lds_e[kLdsStack * kBlock];
-    __shared__ float lds_t[kLdsStack * kBlock];
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, threads = gridDim.x * blockDim.x;
-    Stack st{lds_e + threadIdx.x, lds_t + threadIdx.x, blockDim.x, overflow + gid,
-             reinterpret_cast<float*>(overflow) + (size_t)threads * kGlobalStack + gid, threads, 0};
+    __shared__ uint64_t lds_stack[kLdsStack * kBlock];
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    Stack st = make_stack(lds_stack, reinterpret_cast<uint64_t*>(overflow));
     const uint32_t n = counters[0];
     uint64_t nodes = 0, tris = 0;
-    ExtendSource<kCount> src{P, queue, n, &nodes, &tris};
-    if (!trace_persistent<false, kCount>(S.tv, src, &counters[10], st)) atomicOr(&counters[8], kStackOverflow);
+    ExtendSource<kCount> src{P, rays, n, &nodes, &tris};
+    if (!trace_persistent<false, kCount>(S.tv, src, &counters[kExtendClaims], st))
+        atomicOr(&counters[8], kStackOverflow);
     if (gid == 0) atomicAdd((unsigned long long*)&counters[4], (unsigned long long)n);
     flush_counts<kCount>(nodes, tris, (unsigned long long*)&counters[16]);
 }
@@ -244,18 +255,19 @@ __device__ __forceinline__ f3 path_origin(const Surface& sf, f3 error, f3 direct
     return surface_ray_origin(sf.world, error, reflection ? sf.smooth : sf.face, direction);
 }
 
-__global__ __launch_bounds__(kBlock) void k_shade(Paths P, Consts c, Scene S, const uint32_t* queue, uint32_t* next,
-                                                  uint32_t* shadow_queue, uint32_t* pending_queue, uint32_t* counters) {
+__global__ __launch_bounds__(kBlock) void k_shade(Paths P, Consts c, Scene S, const float4* rays, float4* next,
+                                                  float4* shadow_rays, uint32_t* pending_queue, uint32_t* counters) {
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, threads = gridDim.x * blockDim.x;
     const uint32_t n = counters[0];
     uint32_t dropped = 0;
     for (uint32_t i = gid; i < n; i += threads) {
-        const uint32_t p = queue[i];
+        const float4 rd4 = rays[2 * i + 1];
+        const uint32_t p = f2u(rd4.w) & kPathMask;
         uint4 info = P.info[p];
         uint32_t flags = info.w;
         const uint32_t bounce = (flags >> kBounceShift) & 0xffu;
         uint32_t diffuse = (flags >> kDiffuseShift) & 0xfu, glossy = (flags >> kGlossyShift) & 0xfu;
-        float4 t4 = P.thr[p], r4 = P.rad[p], rd4 = P.ray_d[p];
+        float4 t4 = P.thr[p], r4 = P.rad[p];
         f3 throughput = mk(t4.x, t4.y, t4.z), radiance = mk(r4.x, r4.y, r4.z), direction = mk(rd4.x, rd4.y, rd4.z);
         const float limit = bounce == 0 ? c.clamp_direct : c.clamp_indirect;
         float4 h = P.hit[p];
@@ -293,7 +305,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(Paths P, Consts c, Scene S, co
         // ---- next_event (path.slang:851-952), visibility deferred
         uint32_t rng = info.z;
         f3 a = mk(0.f, 0.f, 0.f), b = mk(0.f, 0.f, 0.f);
-        float4 s0o = make_float4(0.f, 0.f, 0.f, 0.f), s0d = s0o, s1o = s0o, s1d = s0o;
+        f3 s0o = a, s0d = a, s1o = a, s1d = a;
+        float s0t = 0.f, s1t = 0.f;
         if (c.light_count) {
             float u[4];
             group_values(smp, 1u + vertex * 5u + 1u, u);
@@ -317,8 +330,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(Paths P, Consts c, Scene S, co
                     f3 cd;
                     float cdist;
                     surface_ray_connection(origin, sf.world, ldir, src.distance, &cd, &cdist);
-                    s0o = make_float4(origin.x, origin.y, origin.z, cdist);
-                    s0d = make_float4(cd.x, cd.y, cd.z, 0.f);
+                    s0o = origin;
+                    s0d = cd;
+                    s0t = cdist;
                 }
                 bool continuation = bs.event != 0 && event_hemisphere(bs.event, bs.direction, sf.face, sf.smooth);
                 if (continuation && dot(bs.direction, bs.direction) > 0.f) {
@@ -336,8 +350,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(Paths P, Consts c, Scene S, co
                             f3 cd;
                             float cdist;
                             surface_ray_connection(origin, sf.world, bs.direction, e.distance, &cd, &cdist);
-                            s1o = make_float4(origin.x, origin.y, origin.z, cdist);
-                            s1d = make_float4(cd.x, cd.y, cd.z, 0.f);
+                            s1o = origin;
+                            s1d = cd;
+                            s1t = cdist;
                         }
                     }
                 }
@@ -348,12 +363,10 @@ __global__ __launch_bounds__(kBlock) void k_shade(Paths P, Consts c, Scene S, co
             f3 ta = throughput * a, tb = throughput * b;
             P.pend[p] = make_float4(ta.x, ta.y, ta.z, limit);
             P.pend1[p] = make_float4(tb.x, tb.y, tb.z, 0.f);
-            if (has_a) { P.s0o[p] = s0o; P.s0d[p] = s0d; }
-            if (has_b) { P.s1o[p] = s1o; P.s1d[p] = s1d; }
             pending_queue[atomicAdd(&counters[12], 1u)] = p;
             uint32_t slot = atomicAdd(&counters[2], (has_a ? 1u : 0u) + (has_b ? 1u : 0u));
-            if (has_a) shadow_queue[slot++] = 2u * p;
-            if (has_b) shadow_queue[slot] = 2u * p + 1u;
+            if (has_a) put_ray(shadow_rays, slot++, s0o, s0t, s0d, 2u * p);
+            if (has_b) put_ray(shadow_rays, slot, s1o, s1t, s1d, 2u * p + 1u);
         }
         // ---- continuation (path.slang:1680-1705)
         bool alive = true;
@@ -390,10 +403,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(Paths P, Consts c, Scene S, co
         P.info[p] = make_uint4(info.x, info.y, rng, flags);
         if (alive) {
             f3 origin = path_origin(sf, error, bs.direction);
-            P.ray_o[p] = make_float4(origin.x, origin.y, origin.z, FLT_MAX);
-            P.ray_d[p] = make_float4(bs.direction.x, bs.direction.y, bs.direction.z, 0.f);
             P.thr[p] = make_float4(throughput.x, throughput.y, throughput.z, 0.f);
-            next[atomicAdd(&counters[1], 1u)] = p;
+            put_ray(next, atomicAdd(&counters[1], 1u), origin, FLT_MAX, bs.direction, p | (event << kPathBits));
         }
     }
     if (dropped) atomicAdd(&counters[3], dropped);
@@ -403,39 +414,41 @@ __global__ __launch_bounds__(kBlock) void k_shade(Paths P, Consts c, Scene S, co
 template <bool kCount>
 struct ShadowSource {
     Paths P;
-    const uint32_t* queue;
+    const float4* rays;
     uint32_t n;
     uint64_t* nodes;
     uint64_t* tris;
     __device__ uint32_t count() const { return n; }
-    __device__ void load(uint32_t i, f3& o, f3& d, float& t0, float& t1, uint32_t& m) const {
-        const uint32_t r = queue[i], p = r >> 1;
-        const float4 ro = (r & 1u) ? P.s1o[p] : P.s0o[p];
-        const float4 rd = (r & 1u) ? P.s1d[p] : P.s0d[p];
-        o = mk(ro.x, ro.y, ro.z);
-        d = mk(rd.x, rd.y, rd.z);
-        t0 = 0.f;
-        t1 = ro.w;
-        m = VH_RAY_SHADOW;
+    __device__ void fetch(uint32_t i, RayData& r) const {
+        r.a = rays[2 * i];
+        r.b = rays[2 * i + 1];
     }
-    __device__ void store(uint32_t i, const TraceHit& h, const TraceCounts& c) const {
-        P.vis[queue[i]] = h.found ? 1 : 0;
+    __device__ void unpack(uint32_t, const RayData& r, f3& o, f3& d, float& t0, float& t1, uint32_t& m,
+                           uint32_t& key) const {
+        o = mk(r.a.x, r.a.y, r.a.z);
+        d = mk(r.b.x, r.b.y, r.b.z);
+        t0 = 0.f;
+        t1 = r.a.w;
+        m = VH_RAY_SHADOW;
+        key = f2u(r.b.w);
+    }
+    __device__ void store(uint32_t r, const TraceHit& h, const TraceCounts& c) const {
+        P.vis[r] = h.found ? 1 : 0;
         if (kCount) { *nodes += c.nodes; *tris += c.tris; }
     }
 };
 
 template <bool kCount>
-__global__ __launch_bounds__(kBlock) void k_shadow(Paths P, Scene S, const uint32_t* queue, uint32_t* counters,
+VH_TRACE_KERNEL void k_shadow(Paths P, Scene S, const float4* rays, uint32_t* counters,
                                                    uint32_t* overflow) {
-    __shared__ uint32_t lds_e[kLdsStack * kBlock];
-    __shared__ float lds_t[kLdsStack * kBlock];
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, threads = gridDim.x * blockDim.x;
-    Stack st{lds_e + threadIdx.x, lds_t + threadIdx.x, blockDim.x, overflow + gid,
-             reinterpret_cast<float*>(overflow) + (size_t)threads * kGlobalStack + gid, threads, 0};
+    __shared__ uint64_t lds_stack[kLdsStack * kBlock];
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    Stack st = make_stack(lds_stack, reinterpret_cast<uint64_t*>(overflow));
     const uint32_t n = counters[2];
     uint64_t nodes = 0, tris = 0;
-    ShadowSource<kCount> src{P, queue, n, &nodes, &tris};
-    if (!trace_persistent<true, kCount>(S.tv, src, &counters[11], st)) atomicOr(&counters[8], kStackOverflow);
+    ShadowSource<kCount> src{P, rays, n, &nodes, &tris};
+    if (!trace_persistent<true, kCount>(S.tv, src, &counters[kShadowClaims], st))
+        atomicOr(&counters[8], kStackOverflow);
     if (gid == 0) atomicAdd((unsigned long long*)&counters[6], (unsigned long long)n);
     flush_counts<kCount>(nodes, tris, (unsigned long long*)&counters[20]);
 }
@@ -482,9 +495,8 @@ __global__ void k_swap_counts(uint32_t* counters) {
         counters[0] = counters[1];
         counters[1] = 0;
         counters[2] = 0;
-        counters[10] = 0;
-        counters[11] = 0;
         counters[12] = 0;
+        reset_claims(counters);
     }
 }
 
@@ -601,22 +613,22 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, hipStrea
         batch = std::max<uint32_t>(1u, (4u << 20) / std::max<uint32_t>(pixels, 1u));
     }
     batch = std::min(batch, total);
+    if (pixels > kPathMask) return fail(VH_ERR_USAGE, "vh_render: more than 2^24 - 1 pixels");
+    batch = std::min<uint32_t>(batch, kPathMask / pixels);
     const size_t cap = (size_t)pixels * batch;
-    if (cap > 0x7fffffffull) return fail(VH_ERR_USAGE, "vh_render: batch too large");
     if (sc->capacity < cap) {
-        VH_HIP(sc->paths.allocate(cap * (sizeof(float4) * 12 + sizeof(uint32_t) + 2)));
-        VH_HIP(sc->queue_a.allocate(cap * sizeof(uint32_t)));
-        VH_HIP(sc->queue_b.allocate(cap * sizeof(uint32_t)));
-        VH_HIP(sc->shadow.allocate(2 * cap * sizeof(uint32_t)));
-        VH_HIP(sc->hits.allocate(cap * sizeof(uint32_t)));
+        VH_HIP(sc->paths.allocate(cap * (sizeof(float4) * 6 + sizeof(uint32_t) + 2)));
+        VH_HIP(sc->rays_a.allocate(cap * 2 * sizeof(float4)));
+        VH_HIP(sc->rays_b.allocate(cap * 2 * sizeof(float4)));
+        VH_HIP(sc->shadow_rays.allocate(2 * cap * 2 * sizeof(float4)));
+        VH_HIP(sc->pending.allocate(cap * sizeof(uint32_t)));
         sc->capacity = cap;
     }
     VH_HIP(sc->counters.reserve(64 * sizeof(uint32_t)));
     Paths P;
     {
         char* base = sc->paths.as<char>();
-        float4** f4[] = {&P.ray_o, &P.ray_d, &P.thr, &P.rad, (float4**)&P.info, &P.hit,
-                         &P.s0o, &P.s0d, &P.s1o, &P.s1d, &P.pend, &P.pend1};
+        float4** f4[] = {&P.thr, &P.rad, (float4**)&P.info, &P.hit, &P.pend, &P.pend1};
         size_t off = 0;
         for (auto* q : f4) {
             *q = reinterpret_cast<float4*>(base + off);
@@ -677,8 +689,10 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, hipStrea
         c.batch_first = first;
         c.batch_samples = nb;
         const uint32_t n = pixels * nb;
-        uint32_t* qa = sc->queue_a.as<uint32_t>();
-        uint32_t* qb = sc->queue_b.as<uint32_t>();
+        float4* qa = sc->rays_a.as<float4>();
+        float4* qb = sc->rays_b.as<float4>();
+        float4* sq = sc->shadow_rays.as<float4>();
+        uint32_t* pending = sc->pending.as<uint32_t>();
         hipLaunchKernelGGL(k_camera, dim3(blocks(n, kBlock)), dim3(kBlock), 0, s, P, c, S, n, qa, counters);
         for (uint32_t bounce = 0; bounce <= c.max_bounces; bounce++) {
             int rc2 = timed(0, [&] {
@@ -686,17 +700,14 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, hipStrea
                 else hipLaunchKernelGGL(k_extend<false>, dim3(grid), dim3(kBlock), 0, s, P, S, qa, counters, overflow);
             });
             if (!rc2) rc2 = timed(1, [&] {
-                hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kBlock), 0, s, P, c, S, qa, qb, sc->shadow.as<uint32_t>(),
-                                   sc->hits.as<uint32_t>(), counters);
+                hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kBlock), 0, s, P, c, S, qa, qb, sq, pending, counters);
             });
             if (!rc2) rc2 = timed(2, [&] {
-                if (count) hipLaunchKernelGGL(k_shadow<true>, dim3(grid), dim3(kBlock), 0, s, P, S,
-                                              sc->shadow.as<uint32_t>(), counters, overflow);
-                else hipLaunchKernelGGL(k_shadow<false>, dim3(grid), dim3(kBlock), 0, s, P, S,
-                                        sc->shadow.as<uint32_t>(), counters, overflow);
+                if (count) hipLaunchKernelGGL(k_shadow<true>, dim3(grid), dim3(kBlock), 0, s, P, S, sq, counters, overflow);
+                else hipLaunchKernelGGL(k_shadow<false>, dim3(grid), dim3(kBlock), 0, s, P, S, sq, counters, overflow);
             });
             if (rc2) return rc2;
-            hipLaunchKernelGGL(k_resolve, dim3(grid), dim3(kBlock), 0, s, P, sc->hits.as<uint32_t>(), counters);
+            hipLaunchKernelGGL(k_resolve, dim3(grid), dim3(kBlock), 0, s, P, pending, counters);
             hipLaunchKernelGGL(k_swap_counts, dim3(1), dim3(64), 0, s, counters);
             std::swap(qa, qb);
         }

@@ -16,13 +16,20 @@ struct ArraySource {
     void* out;
     const InstanceRecord* instances;
     __device__ uint32_t count() const { return n; }
-    __device__ void load(uint32_t i, f3& o, f3& d, float& t0, float& t1, uint32_t& m) const {
-        const vh_ray r = rays[i];
-        o = mk(r.origin[0], r.origin[1], r.origin[2]);
-        d = mk(r.direction[0], r.direction[1], r.direction[2]);
-        t0 = r.tmin;
-        t1 = r.tmax;
+    __device__ void fetch(uint32_t i, RayData& r) const {
+        const float4* p = reinterpret_cast<const float4*>(rays + i);
+        r.a = p[0];
+        r.b = p[1];
+    }
+    // vh_ray: origin, tmin, direction, tmax
+    __device__ void unpack(uint32_t i, const RayData& r, f3& o, f3& d, float& t0, float& t1, uint32_t& m,
+                           uint32_t& key) const {
+        o = mk(r.a.x, r.a.y, r.a.z);
+        t0 = r.a.w;
+        d = mk(r.b.x, r.b.y, r.b.z);
+        t1 = r.b.w;
         m = mask;
+        key = i;
     }
     __device__ void store(uint32_t i, const TraceHit& h, const TraceCounts& c) const {
         if (kMode == 0) {
@@ -45,13 +52,10 @@ struct ArraySource {
 };
 
 template <int kMode>
-__global__ __launch_bounds__(kTraceBlock) void k_trace(TlasView tv, ArraySource<kMode> src, uint32_t* overflow,
+VH_TRACE_KERNEL void k_trace(TlasView tv, ArraySource<kMode> src, uint32_t* overflow,
                                                         uint32_t* counters) {
-    __shared__ uint32_t lds_e[kLdsStack * kTraceBlock];
-    __shared__ float lds_t[kLdsStack * kTraceBlock];
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, threads = gridDim.x * blockDim.x;
-    Stack st{lds_e + threadIdx.x, lds_t + threadIdx.x, blockDim.x, overflow + gid,
-             reinterpret_cast<float*>(overflow) + (size_t)threads * kGlobalStack + gid, threads, 0};
+    __shared__ uint64_t lds_stack[kLdsStack * kTraceBlock];
+    Stack st = make_stack(lds_stack, reinterpret_cast<uint64_t*>(overflow));
     bool ok = trace_persistent<kMode == 1, kMode == 2>(tv, src, &counters[1], st);
     if (!ok) atomicOr(&counters[0], kStackOverflow);
 }
@@ -65,7 +69,7 @@ uint32_t trace_grid(vh_device* dev, uint32_t n) {
     return want < cap ? (want ? want : 1u) : cap;
 }
 
-// Overflow stacks: entries then distances, kGlobalStack of each per thread.
+// Overflow stacks: kGlobalStack 8-byte entries per thread.
 int ensure_overflow(vh_device* dev, uint32_t threads) {
     size_t need = (size_t)threads * kGlobalStack * 2 * sizeof(uint32_t);
     VH_HIP(dev->stack_overflow.reserve(need));
@@ -78,8 +82,8 @@ int launch_trace(vh_device* dev, hipStream_t s, const TlasView& tv, const void* 
     uint32_t grid = trace_grid(dev, count);
     int rc = ensure_overflow(dev, grid * kTraceBlock);
     if (rc) return rc;
-    VH_HIP(dev->scratch_u32.reserve(64));
-    uint32_t* errors = dev->scratch_u32.as<uint32_t>();  // [0] errors, [1] ray counter
+    VH_HIP(dev->scratch_u32.reserve(2 * sizeof(uint32_t)));
+    uint32_t* errors = dev->scratch_u32.as<uint32_t>();  // [0] errors, [1] ray claims
     VH_HIP(hipMemsetAsync(errors, 0, 2 * sizeof(uint32_t), s));
     const vh_ray* r = static_cast<const vh_ray*>(rays);
     uint32_t* of = dev->stack_overflow.as<uint32_t>();

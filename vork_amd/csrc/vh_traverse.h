@@ -1,19 +1,31 @@
 // Two-level traversal of the 8-wide BVH, one ray per lane, in persistent
-// workgroups that keep every lane busy:
+// workgroups that keep every lane busy.
 //
-//  * a lane whose ray finished takes the next ray from a queue (one atomic per
-//    wave, shared out by ballot), so long rays do not idle the rest of the
-//    wave;
-//  * node steps and triangle steps alternate in phases: every lane visits
-//    nodes until it holds a leaf (or finishes), then the wave tests leaves
-//    together, so the two code paths do not split one wave;
-//  * a stack entry carries the distance at which the ray enters it, and
-//    entries farther than the current closest hit are dropped without a load.
+// Child order without sorting.  The builder places the children of a node in
+// slots so that, for a ray whose direction has sign octant `oct` (bit a set
+// when the direction is negative on axis a), visiting slot s in increasing
+// order of s ^ oct is near-to-far in the common case (the ordering of
+// compressed wide BVHs, Ylitie, Karras and Laine 2017).  A node visit then
+// reduces its hit children to two masks:
 //
-// Stack entries (top two bits): 00 TLAS node, 01 instance, 10 BLAS node,
-// 11 BLAS leaf (bits 28-29 item count - 1, bits 0-27 first item).  The top
-// kLdsStack entries of a lane live in LDS (interleaved across the workgroup),
-// deeper ones in a global overflow array.
+//  * the node group: hit internal children, one bit per child at position
+//    slot ^ oct, plus the node's internal-slot mask to turn a slot into the
+//    child's index (child_base + number of internal slots below it);
+//  * the item group: hit leaf children as a mask over the node's contiguous
+//    items (triangles in a BLAS, instances in the TLAS).
+//
+// The lane keeps one group of each kind in registers; the stack holds the
+// groups still to visit, one 8-byte entry per pushed group (word 0: base
+// index, bit 31 set for an item group; word 1: mask).  Items are tested
+// before the next node is visited, so a near hit shrinks the box tests of
+// the rest of the traversal.
+//
+// Entering an instance pushes the lane's TLAS groups, moves the ray into
+// object space and records the stack depth; once the BLAS groups are used up
+// and the stack is back at that depth, the world ray is restored.
+//
+// The top kLdsStack entries of a lane live in LDS (interleaved across the
+// workgroup), deeper ones in a global overflow array.
 #pragma once
 
 #include "vh_bvh.h"
@@ -23,45 +35,69 @@ namespace vh {
 constexpr int kLdsStack = 12;         // entries per lane in LDS
 constexpr int kGlobalStack = 244;     // overflow entries per lane
 constexpr uint32_t kStackOverflow = 1u;
-constexpr uint32_t kTypeMask = 0xc0000000u;
-constexpr uint32_t kTypeTlas = 0x00000000u, kTypeInst = 0x40000000u, kTypeBlas = 0x80000000u, kTypeLeaf = 0xc0000000u;
-constexpr uint32_t kMaxLeafItems = 0x0fffffffu;
+constexpr uint32_t kItemGroup = 0x80000000u;
 
+// Idle lanes of a wave before it leaves its stepping loop to start new rays
+// (starting costs the ray loads; lanes that start together also stay in step
+// longer).
+#ifndef VH_TRACE_REFILL_MIN
+#define VH_TRACE_REFILL_MIN 48
+#endif
+// Items (triangles or instances) a lane tests per traversal step.
+#ifndef VH_TRACE_ITEMS_PER_STEP
+#define VH_TRACE_ITEMS_PER_STEP 2
+#endif
+// Waves per SIMD the trace kernels are compiled for (bounds their VGPRs:
+// 512 / waves).
+#ifndef VH_TRACE_WAVES_PER_EU
+#define VH_TRACE_WAVES_PER_EU 4
+#endif
+#define VH_TRACE_KERNEL __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VH_TRACE_WAVES_PER_EU)))
+constexpr uint32_t kRefillMin = VH_TRACE_REFILL_MIN;
+constexpr int kItemsPerStep = VH_TRACE_ITEMS_PER_STEP;
+
+// Raw 32-byte ray record as a ray source stores it; the source decodes it.
+struct RayData {
+    float4 a, b;
+};
+
+// Entries are (word 0) | (word 1) << 32.
 struct Stack {
-    uint32_t* lds;        // &lds_entries[threadIdx.x]
-    float* ldst;          // &lds_distances[threadIdx.x]
-    uint32_t lds_stride;  // blockDim.x
-    uint32_t* glob;       // &overflow_entries[global thread id]
-    float* globt;         // &overflow_distances[global thread id]
-    uint32_t glob_stride; // threads in the grid
+    VH_LDS uint64_t* lds;       // &lds_entries[threadIdx.x]
+    uint32_t lds_stride;        // blockDim.x
+    VH_GLOBAL uint64_t* glob;   // &overflow_entries[global thread id]
+    uint32_t glob_stride;       // threads in the grid
     int sp;
-    __device__ __forceinline__ bool push(uint32_t v, float t) {
+    __device__ __forceinline__ bool push(uint32_t a, uint32_t b) {
+        const uint64_t e = (uint64_t)a | ((uint64_t)b << 32);
         if (sp < kLdsStack) {
-            lds[sp * lds_stride] = v;
-            ldst[sp * lds_stride] = t;
+            lds[sp * lds_stride] = e;
         } else if (sp < kLdsStack + kGlobalStack) {
-            glob[(sp - kLdsStack) * glob_stride] = v;
-            globt[(sp - kLdsStack) * glob_stride] = t;
+            glob[(sp - kLdsStack) * glob_stride] = e;
         } else {
             return false;
         }
         sp++;
         return true;
     }
-    __device__ __forceinline__ uint32_t peek() const {
-        const int i = sp - 1;
-        return i < kLdsStack ? lds[i * lds_stride] : glob[(i - kLdsStack) * glob_stride];
-    }
-    __device__ __forceinline__ uint32_t pop(float& t) {
+    __device__ __forceinline__ uint64_t pop() {
         sp--;
-        if (sp < kLdsStack) {
-            t = ldst[sp * lds_stride];
-            return lds[sp * lds_stride];
-        }
-        t = globt[(sp - kLdsStack) * glob_stride];
+        if (sp < kLdsStack) return lds[sp * lds_stride];
         return glob[(sp - kLdsStack) * glob_stride];
     }
 };
+
+// Stack of the calling lane: `lds` holds kLdsStack * blockDim.x entries,
+// `overflow` kGlobalStack entries per thread of the grid.
+__device__ __forceinline__ Stack make_stack(uint64_t* lds, uint64_t* overflow) {
+    Stack st;
+    st.lds = (VH_LDS uint64_t*)lds + threadIdx.x;
+    st.lds_stride = blockDim.x;
+    st.glob = (VH_GLOBAL uint64_t*)overflow + blockIdx.x * blockDim.x + threadIdx.x;
+    st.glob_stride = gridDim.x * blockDim.x;
+    st.sp = 0;
+    return st;
+}
 
 struct TraceHit {
     float t, u, v;
@@ -73,263 +109,319 @@ struct TraceCounts {
     uint32_t nodes, tris;
 };
 
-// Compare-exchange of the 8-key sorting network (descending).
-#define VH_CX(a, b)                     \
-    {                                   \
-        uint32_t hi_ = max(k[a], k[b]); \
-        uint32_t lo_ = min(k[a], k[b]); \
-        k[a] = hi_;                     \
-        k[b] = lo_;                     \
-    }
-
-__device__ __forceinline__ void sort8_desc(uint32_t k[8]) {
-    // Batcher's odd-even merge sort network, 19 compare-exchanges.
-    VH_CX(0, 1) VH_CX(2, 3) VH_CX(4, 5) VH_CX(6, 7)
-    VH_CX(0, 2) VH_CX(1, 3) VH_CX(4, 6) VH_CX(5, 7)
-    VH_CX(1, 2) VH_CX(5, 6) VH_CX(0, 4) VH_CX(3, 7)
-    VH_CX(1, 5) VH_CX(2, 6)
-    VH_CX(1, 4) VH_CX(3, 6)
-    VH_CX(2, 4) VH_CX(3, 5)
-    VH_CX(3, 4)
-}
-#undef VH_CX
-
 union NodeBits {
-    uint4 v[5];
+    vu4 v[5];
     WideNode n;
 };
 
-// meta byte of child `slot` without indexing a register array dynamically
-__device__ __forceinline__ uint32_t node_meta(const NodeBits& nb, uint32_t slot) {
-    uint32_t w = slot < 4u ? nb.v[1].z : nb.v[1].w;
-    return (w >> ((slot & 3u) * 8u)) & 0xffu;
-}
-
 __device__ __forceinline__ void load_node(const WideNode* base, uint32_t i, NodeBits& nb) {
-    const uint4* p = reinterpret_cast<const uint4*>(base + i);
+    const VH_GLOBAL vu4* p = (const VH_GLOBAL vu4*)(base + i);
 #pragma unroll
     for (int k = 0; k < 5; k++) nb.v[k] = p[k];
 }
 
-constexpr uint32_t kKeyLive = 0x80000000u;
+__device__ __forceinline__ uint32_t octant(f3 inv) {
+    return (inv.x < 0.f ? 1u : 0u) | (inv.y < 0.f ? 2u : 0u) | (inv.z < 0.f ? 4u : 0u);
+}
 
-// Child entry keys: a live child's entry distance bits (non-negative, so they
-// order like the values) with the low 3 bits replaced by its slot and the top
-// bit set; 0 for a child that is missed or absent.
+// Relative size of the slab-distance error margin; see child_groups.
+constexpr float kBoxMargin = 1.0f / 524288.0f;  // 2^-19
+// Scale on the closest hit before it bounds a box test, so a triangle whose
+// rounded t does not exceed the current hit is never culled by its box.
+constexpr float kCullScale = 1.0f + 1.0f / 262144.0f;  // 1 + 2^-18
+
+// Box tests of the eight children of one node.  Returns the hit internal
+// children as bits at slot ^ oct and the hit leaf children's items as a mask
+// over the node's items.
 //
-// The near plane of each axis is chosen once per node from the ray's
-// direction signs, so every child costs six plane decodes, six slab
-// distances and two three-way min/max.  Each plane is decoded to the float
-// the builder verified, then (plane - o) * inv with two roundings: the far
-// distance is scaled by (1 + 2 gamma(3)) so that the test stays conservative.
-__device__ __forceinline__ void child_keys(const NodeBits& nb, f3 o, f3 inv, float tmin, float tmax, uint32_t k[8]) {
+// Plane t of child c on axis a is q * (scale_a * inv_a) + (origin_a - o_a) *
+// inv_a, one fused multiply-add per plane.  That rounds differently from the
+// exact slab distance (decoded plane - o) / d, by at most a few units of
+// 2^-24 * M_a with M_a = (|origin_a| + 255 scale_a + |o_a|) |inv_a|, the
+// largest magnitude any term on the axis takes.  Near planes are shifted
+// down and far planes up by 2^-19 M_a, which is larger than that error, so
+// a child whose exact box meets [tmin, tmax] is always reported.  A zero
+// direction component gives infinite inv_a; the planes of that axis then come
+// out as -inf / +inf or NaN, and NaN drops out of fmin / fmax, so the axis
+// does not constrain the test.
+__device__ __forceinline__ void child_groups(const NodeBits& nb, f3 o, f3 inv, uint32_t oct, float tmin, float tmax,
+                                             uint32_t& hits, uint32_t& items) {
     const WideNode& n = nb.n;
     const float sx = exp_scale(n.ex), sy = exp_scale(n.ey), sz = exp_scale(n.ez);
-    // q planes: qlo x = v[2].xy, y = v[2].zw, z = v[3].xy; qhi x = v[3].zw, y = v[4].xy, z = v[4].zw
-    const bool px = inv.x >= 0.f, py = inv.y >= 0.f, pz = inv.z >= 0.f;
+    const float ax = sx * inv.x, ay = sy * inv.y, az = sz * inv.z;
+    const float bx = (n.origin[0] - o.x) * inv.x, by = (n.origin[1] - o.y) * inv.y, bz = (n.origin[2] - o.z) * inv.z;
+    const float ex = ((255.f * sx + fabsf(n.origin[0])) + fabsf(o.x)) * fabsf(inv.x) * kBoxMargin;
+    const float ey = ((255.f * sy + fabsf(n.origin[1])) + fabsf(o.y)) * fabsf(inv.y) * kBoxMargin;
+    const float ez = ((255.f * sz + fabsf(n.origin[2])) + fabsf(o.z)) * fabsf(inv.z) * kBoxMargin;
+    const float bnx = bx - ex, bfx = bx + ex, bny = by - ey, bfy = by + ey, bnz = bz - ez, bfz = bz + ez;
+    // q planes: qlo x = v[2].xy, y = v[2].zw, z = v[3].xy; qhi x = v[3].zw, y = v[4].xy, z = v[4].zw.
+    // A negative direction enters through the high plane.
+    const bool px = !(oct & 1u), py = !(oct & 2u), pz = !(oct & 4u);
     const uint32_t nx0 = px ? nb.v[2].x : nb.v[3].z, nx1 = px ? nb.v[2].y : nb.v[3].w;
     const uint32_t fx0 = px ? nb.v[3].z : nb.v[2].x, fx1 = px ? nb.v[3].w : nb.v[2].y;
     const uint32_t ny0 = py ? nb.v[2].z : nb.v[4].x, ny1 = py ? nb.v[2].w : nb.v[4].y;
     const uint32_t fy0 = py ? nb.v[4].x : nb.v[2].z, fy1 = py ? nb.v[4].y : nb.v[2].w;
     const uint32_t nz0 = pz ? nb.v[3].x : nb.v[4].z, nz1 = pz ? nb.v[3].y : nb.v[4].w;
     const uint32_t fz0 = pz ? nb.v[4].z : nb.v[3].x, fz1 = pz ? nb.v[4].w : nb.v[3].y;
-    const uint32_t count = n.count;
+    const uint32_t imask = n.imask;
+    uint32_t h = 0, it = 0;
 #pragma unroll
     for (int c = 0; c < 8; c++) {
         const uint32_t sh = (c & 3) * 8;
         const uint32_t nxw = c < 4 ? nx0 : nx1, fxw = c < 4 ? fx0 : fx1;
         const uint32_t nyw = c < 4 ? ny0 : ny1, fyw = c < 4 ? fy0 : fy1;
         const uint32_t nzw = c < 4 ? nz0 : nz1, fzw = c < 4 ? fz0 : fz1;
-        float tnx, tny, tnz, tfx, tfy, tfz;
-        {
-            VH_EXACT
-            tnx = (decode(n.origin[0], (uint8_t)(nxw >> sh), sx) - o.x) * inv.x;
-            tfx = (decode(n.origin[0], (uint8_t)(fxw >> sh), sx) - o.x) * inv.x;
-            tny = (decode(n.origin[1], (uint8_t)(nyw >> sh), sy) - o.y) * inv.y;
-            tfy = (decode(n.origin[1], (uint8_t)(fyw >> sh), sy) - o.y) * inv.y;
-            tnz = (decode(n.origin[2], (uint8_t)(nzw >> sh), sz) - o.z) * inv.z;
-            tfz = (decode(n.origin[2], (uint8_t)(fzw >> sh), sz) - o.z) * inv.z;
-        }
+        const float tnx = __builtin_fmaf((float)((nxw >> sh) & 0xffu), ax, bnx);
+        const float tfx = __builtin_fmaf((float)((fxw >> sh) & 0xffu), ax, bfx);
+        const float tny = __builtin_fmaf((float)((nyw >> sh) & 0xffu), ay, bny);
+        const float tfy = __builtin_fmaf((float)((fyw >> sh) & 0xffu), ay, bfy);
+        const float tnz = __builtin_fmaf((float)((nzw >> sh) & 0xffu), az, bnz);
+        const float tfz = __builtin_fmaf((float)((fzw >> sh) & 0xffu), az, bfz);
         const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tmin));
         const float tf = fminf(fminf(tfx, tfy), fminf(tfz, tmax));
-        const bool live = (uint32_t)c < count && tn <= tf * (1.f + 2.f * kGamma3);
-        k[c] = live ? ((f2u(tn) & 0x7ffffff8u) | (uint32_t)c | kKeyLive) : 0u;  // sign cleared: tn >= +-0
+        if (tn <= tf) {
+            const uint32_t meta = ((c < 4 ? nb.v[1].z : nb.v[1].w) >> sh) & 0xffu;
+            if ((imask >> c) & 1u) {
+                h |= 1u << ((uint32_t)c ^ oct);
+            } else if (!(meta & kMetaNoItems)) {
+                it |= ((1u << ((meta >> 5) + 1u)) - 1u) << (meta & 31u);
+            }
+        }
     }
+    hits = h;
+    items = it;
 }
 
 // One lane's traversal state.
 template <bool kAny, bool kCount>
 struct Tracer {
-    f3 o, d, inv;           // world ray
-    f3 lo, ld, linv;        // ray in the current instance's object space
+    f3 wo, wd;              // world ray
+    f3 o, inv;              // ray of the current space (world or instance object space)
+    uint32_t oct;
+    f3 ld;                  // object-space direction of the triangle tests
     Shear lsh;
     float tmin, tmax;
     uint32_t mask;
     uint32_t cur_inst;
-    bool cull;
-    const BlasRecord* blas;
-    uint32_t leaf;          // pending leaf entry, 0 = none
+    bool cull, in_blas;
+    int blas_sp;            // stack depth at the current instance's entry
+    const WideNode* bnodes; // current BLAS
+    const float* btris;
+    const uint32_t* bids;
+    uint32_t nbase, ngroup; // node group: child base, hits (bits 0-7) | internal-slot mask << 8
+    uint32_t ibase, items;  // item group
     bool done;
     TraceHit hit;
     TraceCounts cnt;
 
+    __device__ __forceinline__ void set_root() {
+        nbase = 0;
+        ngroup = (1u << oct) | (1u << 8);  // the root as slot 0 of a one-child group
+        ibase = 0;
+        items = 0;
+    }
+
     __device__ __forceinline__ void begin(const TlasView& tv, f3 o_, f3 d_, float tmin_, float tmax_, uint32_t mask_,
                                           Stack& st) {
-        o = o_; d = d_; inv = safe_inv(d_);
-        lo = o_; ld = d_; linv = inv;
+        wo = o_; wd = d_;
+        o = o_; inv = safe_inv(d_); oct = octant(inv);
+        ld = d_;
         lsh = make_shear(d_);
         tmin = tmin_; tmax = tmax_; mask = mask_;
-        cur_inst = 0; cull = false; blas = nullptr;
-        leaf = 0;
+        cur_inst = 0; cull = false; in_blas = false; blas_sp = 0;
+        bnodes = nullptr; btris = nullptr; bids = nullptr;
+        set_root();
         hit.found = false; hit.t = tmax_; hit.u = hit.v = 0.f; hit.inst = hit.prim = 0xffffffffu;
         cnt.nodes = cnt.tris = 0;
         st.sp = 0;
         done = tv.node_count == 0;
-        if (!done) st.push(kTypeTlas, tmin_);
     }
 
-    // Pops one entry: visits a node, enters an instance, or takes a leaf.
-    // Returns false when the stack overflowed.
-    __device__ __forceinline__ bool node_step(const TlasView& tv, Stack& st) {
-        if (st.sp == 0) { done = true; return true; }
-        float te;
-        uint32_t e = st.pop(te);
-        // entered beyond the closest hit so far: te is a rounded entry distance, so
-        // compare with the same (1 + 2 gamma(3)) margin as the box test
-        if (!kAny && te > hit.t * (1.f + 2.f * kGamma3)) return true;
-        const uint32_t type = e & kTypeMask;
-        if (type == kTypeLeaf) { leaf = e; return true; }
-        if (type == kTypeInst) {
-            const uint32_t inst = e & ~kTypeMask;
-            const InstanceRecord& r = tv.instances[inst];
-            const uint32_t mf = r.mask_flags;
-            if (!(mf & (1u << 9)) || !(mf & mask & 0xffu)) return true;
-            lo = xform_point(r.w2o, o);
-            ld = xform_dir(r.w2o, d);
-            linv = safe_inv(ld);
-            lsh = make_shear(ld);
-            cull = (mf >> 8) & 1u;
-            blas = &tv.blases[r.blas];
-            cur_inst = inst;
-            return st.push(kTypeBlas, te);
-        }
-        const bool in_blas = type == kTypeBlas;
-        NodeBits nb;
-        load_node(in_blas ? blas->nodes : tv.nodes, e & ~kTypeMask, nb);
-        if (kCount) cnt.nodes++;
-        uint32_t k[8];
-        child_keys(nb, in_blas ? lo : o, in_blas ? linv : inv, tmin, kAny ? tmax : hit.t, k);
-        sort8_desc(k);
-        // far to near, so the nearest entry is popped first; missed children sort last
+    __device__ __forceinline__ void leave_instance() {
+        o = wo;
+        inv = safe_inv(wd);
+        oct = octant(inv);
+        in_blas = false;
+    }
+
+    // Enters instance `inst` unless it is masked out or invalid.
+    __device__ __forceinline__ bool enter_instance(const TlasView& tv, uint32_t inst, Stack& st) {
+        const VH_GLOBAL InstanceRecord& r = as_global(tv.instances)[inst];
+        const uint32_t mf = r.mask_flags;
+        if (!(mf & (1u << 9)) || !(mf & mask & 0xffu)) return true;
+        if ((ngroup & 0xffu) && !st.push(nbase, ngroup)) return false;
+        if (items && !st.push(ibase | kItemGroup, items)) return false;
+        blas_sp = st.sp;
+        float w2o[12];
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-            if (k[j] == 0u) break;
-            const uint32_t meta = node_meta(nb, k[j] & 7u);
-            const float t = u2f(k[j] & ~(7u | kKeyLive));
-            uint32_t entry;
-            if (meta & kMetaInternal) {
-                entry = (nb.n.child_base + (meta & 7u)) | type;
-            } else if (in_blas) {
-                entry = kTypeLeaf | (((meta >> 5) & 3u) << 28) | (nb.n.item_base + (meta & 31u));
-            } else {
-                entry = kTypeInst | tv.inst_ids[nb.n.item_base + (meta & 31u)];
-            }
-            if (!st.push(entry, t)) return false;
-        }
+        for (int k = 0; k < 12; k++) w2o[k] = r.w2o[k];
+        o = xform_point(w2o, wo);
+        ld = xform_dir(w2o, wd);
+        inv = safe_inv(ld);
+        oct = octant(inv);
+        lsh = make_shear(ld);
+        cull = (mf >> 8) & 1u;
+        const VH_GLOBAL BlasRecord& b = as_global(tv.blases)[r.blas];
+        bnodes = b.nodes;
+        btris = b.tris;
+        bids = b.tri_ids;
+        cur_inst = inst;
+        in_blas = true;
+        set_root();
         return true;
     }
 
-    // Tests the pending leaf's triangles.
-    __device__ __forceinline__ void leaf_step() {
-        const uint32_t first = leaf & kMaxLeafItems, count = ((leaf >> 28) & 3u) + 1u;
-        leaf = 0;
-        for (uint32_t q = 0; q < count; q++) {
-            const uint32_t item = first + q;
-            const float* tp = blas->tris + 9 * (size_t)item;
-            const f3 a = mk(tp[0], tp[1], tp[2]), b = mk(tp[3], tp[4], tp[5]), c = mk(tp[6], tp[7], tp[8]);
-            float t, u, v;
-            if (kCount) cnt.tris++;
-            // admit equal t so that the tie rule can pick the lower id
-            const float lim = (kAny || !hit.found) ? tmax : next_up(hit.t);
-            if (intersect_triangle(a, b, c, lo, ld, lsh, tmin, lim, cull, &t, &u, &v)) {
-                if (!(t < tmax)) continue;
-                if (kAny) {
-                    hit.found = true;
-                    hit.t = t;
-                    done = true;
-                    return;
-                }
-                const uint32_t prim = blas->tri_ids[item];
-                if (!hit.found || better_hit(t, cur_inst, prim, hit.t, hit.inst, hit.prim)) {
-                    hit.found = true; hit.t = t; hit.u = u; hit.v = v; hit.inst = cur_inst; hit.prim = prim;
-                }
+    // Tests one triangle of the current BLAS.
+    __device__ __forceinline__ void test_triangle(uint32_t item) {
+        const VH_GLOBAL float* tp = as_global(btris) + 9 * (size_t)item;
+        const f3 a = mk(tp[0], tp[1], tp[2]), b = mk(tp[3], tp[4], tp[5]), c = mk(tp[6], tp[7], tp[8]);
+        if (kCount) cnt.tris++;
+        // admit equal t so that the tie rule can pick the lower id
+        const float lim = (kAny || !hit.found) ? tmax : next_up(hit.t);
+        float t, u, v;
+        if (intersect_triangle(a, b, c, o, ld, lsh, tmin, lim, cull, &t, &u, &v)) {
+            if (!(t < tmax)) return;
+            if (kAny) {
+                hit.found = true;
+                hit.t = t;
+                done = true;
+                return;
+            }
+            const uint32_t prim = as_global(bids)[item];
+            if (!hit.found || better_hit(t, cur_inst, prim, hit.t, hit.inst, hit.prim)) {
+                hit.found = true; hit.t = t; hit.u = u; hit.v = v; hit.inst = cur_inst; hit.prim = prim;
             }
         }
+    }
+
+    // Takes a group off the stack when the lane holds none; the ray is done
+    // when the stack is empty.
+    __device__ __forceinline__ void pop_phase(Stack& st) {
+        if ((ngroup & 0xffu) || items) return;
+        if (in_blas && st.sp == blas_sp) leave_instance();
+        if (st.sp == 0) {
+            done = true;
+            return;
+        }
+        const uint64_t e = st.pop();
+        const uint32_t e0 = (uint32_t)e, e1 = (uint32_t)(e >> 32);
+        if (e0 & kItemGroup) {
+            ibase = e0 & ~kItemGroup;
+            items = e1;
+        } else {
+            nbase = e0;
+            ngroup = e1;
+        }
+    }
+
+    __device__ __forceinline__ bool wants_node() const { return (ngroup & 0xffu) && !items; }
+
+    // Visits the next node of the node group.  Returns false when the stack
+    // overflowed.
+    __device__ __forceinline__ bool node_phase(const TlasView& tv, Stack& st) {
+        const uint32_t hits = ngroup & 0xffu, imask = ngroup >> 8;
+        const uint32_t slot = (uint32_t)__builtin_ctz(hits) ^ oct;
+        const uint32_t rest = hits & (hits - 1u);
+        const uint32_t node = nbase + (uint32_t)__popc(imask & ((1u << slot) - 1u));
+        if (rest && !st.push(nbase, rest | (imask << 8))) return false;
+        NodeBits nb;
+        load_node(in_blas ? bnodes : tv.nodes, node, nb);
+        if (kCount) cnt.nodes++;
+        uint32_t h, it;
+        child_groups(nb, o, inv, oct, tmin, kAny ? tmax : hit.t * kCullScale, h, it);
+        nbase = nb.n.child_base;
+        ngroup = h | ((uint32_t)nb.n.imask << 8);
+        ibase = nb.n.item_base;
+        items = it;
+        return true;
+    }
+
+    // Tests up to kItemsPerStep items of the item group (an instance entry
+    // ends the phase).  Returns false when the stack overflowed.
+    __device__ __forceinline__ bool item_phase(const TlasView& tv, Stack& st) {
+        for (int q = 0; q < kItemsPerStep && items && !done; q++) {
+            const uint32_t item = ibase + (uint32_t)__builtin_ctz(items);
+            items &= items - 1u;
+            if (in_blas) {
+                test_triangle(item);
+            } else {
+                if (!enter_instance(tv, as_global(tv.inst_ids)[item], st)) return false;
+                break;
+            }
+        }
+        return true;
     }
 };
 
 // Persistent trace loop.  `src` supplies rays and consumes results:
 //   uint32_t count() const;
-//   void load(uint32_t i, f3& o, f3& d, float& tmin, float& tmax, uint32_t& mask) const;
-//   void store(uint32_t i, const TraceHit& h, const TraceCounts& c) const;
-// `next` is a global counter of handed-out ray indices, zero at launch.
-// Every lane leaves the loop once the queue is empty and its ray is stored.
-// Returns false when a stack overflowed.
+//   void fetch(uint32_t i, RayData& r) const;      // loads ray i
+//   void unpack(uint32_t i, const RayData& r, f3& o, f3& d, float& tmin, float& tmax, uint32_t& mask,
+//               uint32_t& key) const;
+//   void store(uint32_t key, const TraceHit& h, const TraceCounts& c) const;
+// `next` is a global counter of claimed ray indices, zero at launch.
+//
+// Lanes that finish wait until kRefillMin lanes of the wave are idle, then
+// claim consecutive rays with one atomic per wave.  All waves claim from the
+// one counter, so the rays in flight on the whole device are a narrow window
+// of the queue (neighbouring pixels at the first bounce) and share the BVH
+// data they touch in the caches.  Each iteration of the inner loop every busy
+// lane takes one traversal step.  Every lane leaves the outer loop once the
+// queue is drained and its ray is stored.  Returns false when a stack
+// overflowed.
 template <bool kAny, bool kCount, class Src>
 __device__ bool trace_persistent(const TlasView& tv, const Src& src, uint32_t* next, Stack& st) {
     const uint32_t n = src.count();
     Tracer<kAny, kCount> tr;
     bool active = false, exhausted = false, ok = true;
-    uint32_t ray = 0;
+    uint32_t key = 0;
     const uint64_t lt_mask = (1ull << __lane_id()) - 1ull;
     while (true) {
-        // lanes without a ray take new ones: one atomic per wave
-        const uint64_t need = __ballot(!active && !exhausted);
-        if (need) {
-            const int leader = __ffsll((unsigned long long)need) - 1;
+        const uint64_t want = __ballot(!active && !exhausted);
+        if (want) {
+            const int leader = __ffsll((unsigned long long)want) - 1;
             uint32_t base = 0;
-            if ((int)__lane_id() == leader) base = atomicAdd(next, (uint32_t)__popcll(need));
+            if ((int)__lane_id() == leader) base = atomicAdd(next, (uint32_t)__popcll(want));
             base = __shfl(base, leader);
             if (!active && !exhausted) {
-                const uint32_t idx = base + (uint32_t)__popcll(need & lt_mask);
+                const uint32_t idx = base + (uint32_t)__popcll(want & lt_mask);
                 if (idx < n) {
+                    RayData rd;
+                    src.fetch(idx, rd);
                     f3 o, d;
                     float t0, t1;
                     uint32_t m;
-                    src.load(idx, o, d, t0, t1, m);
+                    src.unpack(idx, rd, o, d, t0, t1, m, key);
                     tr.begin(tv, o, d, t0, t1, m, st);
-                    ray = idx;
-                    active = true;
+                    active = !tr.done;
+                    if (!active) src.store(key, tr.hit, tr.cnt);
                 } else {
                     exhausted = true;
                 }
             }
         }
-        if (!__ballot(active)) break;
-        // node phase: visit nodes until every lane holds a leaf or is done.  A
-        // lane that already holds a leaf keeps visiting node entries on top of
-        // its stack meanwhile (speculative traversal, Aila and Laine 2009), so
-        // it does not idle while the others search.
-        while (true) {
-            const bool searching = active && !tr.done && tr.leaf == 0;
-            if (!__ballot(searching)) break;
-            bool step = searching;
-            if (active && !tr.done && tr.leaf != 0 && st.sp > 0) {
-                // node entries only: a leaf or an instance entry would change
-                // the object space the pending leaf is tested in
-                const uint32_t top = st.peek() & kTypeMask;
-                step = top == kTypeBlas || top == kTypeTlas;
-            }
-            if (step && !tr.node_step(tv, st)) {
-                ok = false;
-                tr.done = true;
-            }
+        if (!__ballot(active)) {
+            if (!__ballot(!exhausted)) break;
+            continue;
         }
-        // leaf phase
-        if (active && tr.leaf != 0 && !tr.done) tr.leaf_step();
-        if (active && tr.done) {
-            src.store(ray, tr.hit, tr.cnt);
-            active = false;
+        while (true) {
+            if (active) {
+                tr.pop_phase(st);
+                if (!tr.done && tr.wants_node() && !tr.node_phase(tv, st)) {
+                    ok = false;
+                    tr.done = true;
+                }
+                if (!tr.done && tr.items && !tr.item_phase(tv, st)) {
+                    ok = false;
+                    tr.done = true;
+                }
+                if (tr.done) {
+                    src.store(key, tr.hit, tr.cnt);
+                    active = false;
+                }
+            }
+            if (!__ballot(active)) break;
+            if ((uint32_t)__popcll(__ballot(!active && !exhausted)) >= kRefillMin) break;
         }
     }
     return ok;
