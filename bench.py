@@ -45,6 +45,9 @@ def main():
     ap.add_argument("--batch", type=int, default=0, help="exposure samples per wavefront batch (0 = auto)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (read + write) three times, for scaling "
+                         "rocprof FETCH_SIZE / WRITE_SIZE to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -55,6 +58,16 @@ def main():
     import numpy as np
     import torch
     from vork_amd import abi, native, scene as S
+
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.add(src, 0.0, out=dst)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 3}))
+        return
 
     t0 = time.time()
     sc = S.config1(width=args.width, height=args.height, spp=args.spp, sphere_level=args.sphere_level,

@@ -36,7 +36,11 @@ struct alignas(16) WideNode {
 static_assert(sizeof(WideNode) == 80, "wide node is 80 bytes");
 
 constexpr int kWidth = 8;
-constexpr int kLeafMax = 4;          // items per leaf child
+#ifndef VH_LEAF_MAX
+#define VH_LEAF_MAX 2
+#endif
+constexpr int kLeafMax = VH_LEAF_MAX;  // triangles per BLAS leaf child (at most 4)
+static_assert(kLeafMax >= 1 && kLeafMax <= 4, "leaf meta holds 1-4 items");
 constexpr uint32_t kMetaNoItems = 0x80u;
 
 VH_HD float exp_scale(uint8_t e) { return u2f(uint32_t(e) << 23); }

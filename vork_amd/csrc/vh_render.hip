@@ -61,19 +61,14 @@ struct Consts {
     vh_sky sky;
 };
 
-// Path slot t of a sample <-> pixel: column-major inside bands of 8 rows, so
+// Path slot t of a sample -> pixel: column-major inside bands of 8 rows, so
 // that 64 consecutive slots (one wave) cover an 8x8 tile.  A bijection on
-// [0, width*height) for any image size.
+// [0, width*height) for any image size (the last band may be shorter).
 __device__ __forceinline__ uint32_t slot_to_pixel(uint32_t t, uint32_t w, uint32_t h) {
     const uint32_t band = t / (8u * w), r = t % (8u * w);
     const uint32_t rows = min(8u, h - band * 8u);
     const uint32_t x = r / rows, y = band * 8u + r % rows;
     return y * w + x;
-}
-__device__ __forceinline__ uint32_t pixel_to_slot(uint32_t pixel, uint32_t w, uint32_t h) {
-    const uint32_t x = pixel % w, y = pixel / w, band = y / 8u;
-    const uint32_t rows = min(8u, h - band * 8u);
-    return band * 8u * w + x * rows + (y - band * 8u);
 }
 
 struct Scene {
@@ -255,7 +250,11 @@ __device__ __forceinline__ f3 path_origin(const Surface& sf, f3 error, f3 direct
     return surface_ray_origin(sf.world, error, reflection ? sf.smooth : sf.face, direction);
 }
 
-__global__ __launch_bounds__(kBlock) void k_shade(Paths P, Consts c, Scene S, const float4* rays, float4* next,
+// Waves per SIMD k_shade is compiled for (bounds its VGPRs to 512 / waves).
+#ifndef VH_SHADE_WAVES_PER_EU
+#define VH_SHADE_WAVES_PER_EU 3
+#endif
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VH_SHADE_WAVES_PER_EU))) void k_shade(Paths P, Consts c, Scene S, const float4* rays, float4* next,
                                                   float4* shadow_rays, uint32_t* pending_queue, uint32_t* counters) {
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, threads = gridDim.x * blockDim.x;
     const uint32_t n = counters[0];
@@ -471,13 +470,16 @@ __global__ void k_resolve(Paths P, const uint32_t* queue, uint32_t* counters) {
     if (dropped) atomicAdd(&counters[3], dropped);
 }
 
-// One thread per pixel: adds the batch's samples, in sample order, to the film.
+// One thread per pixel: adds the batch's samples, in sample order, to the
+// film.  Threads walk the path slots in order (consecutive slots hold an 8x8
+// pixel tile), so the path-state reads are contiguous.
 __global__ void k_film(Paths P, Consts c, float4* film) {
-    uint32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pixel >= c.pixels) return;
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= c.pixels) return;
+    const uint32_t pixel = slot_to_pixel(slot, c.width, c.height);
     float4 acc = film[pixel];
     for (uint32_t s = 0; s < c.batch_samples; s++) {
-        uint32_t p = s * c.pixels + pixel_to_slot(pixel, c.width, c.height);
+        uint32_t p = s * c.pixels + slot;
         float4 r = P.rad[p];
         bool bad = (P.info[p].w & kNonFinite) || !isfinite(r.x) || !isfinite(r.y) || !isfinite(r.z);
         if (!bad) {

@@ -45,7 +45,7 @@ constexpr uint32_t kItemGroup = 0x80000000u;
 #endif
 // Items (triangles or instances) a lane tests per traversal step.
 #ifndef VH_TRACE_ITEMS_PER_STEP
-#define VH_TRACE_ITEMS_PER_STEP 2
+#define VH_TRACE_ITEMS_PER_STEP 1
 #endif
 // Waves per SIMD the trace kernels are compiled for (bounds their VGPRs:
 // 512 / waves).
