@@ -172,6 +172,7 @@ def main():
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 1), "p99": round(float(np.percentile(lat, 99)), 1)},
         "blas_build_ms": [round(b["build_ms"], 2) for b in binfo],
         "blas_nodes": [b["node_count"] for b in binfo],
+        "blas_sah_cost": [round(b["sah_cost"], 1) for b in binfo],
         "rays": {"extension": stats["extension_rays"], "shadow": stats["shadow_rays"]},
         "film_finite": finite, "wall_s": round(total, 2),
         "roofline": roofline, "cpu_baseline": cpu,

@@ -101,13 +101,13 @@ typedef struct vh_blas_info {
     uint32_t node_count;      /* 8-wide nodes */
     uint32_t binary_node_count;
     uint32_t max_depth;       /* wide levels */
-    double sah_cost;          /* binary-tree SAH cost, C_node = 1, C_tri = 1 */
+    double sah_cost;          /* binary-tree SAH cost: sum over nodes and leaves of area / root area */
     double build_ms;          /* device time of the build kernels */
     uint64_t device_bytes;    /* nodes + triangles + shading attributes */
 } vh_blas_info;
 
-/* Builds one BLAS on the device: Morton-code LBVH, SAH-binned treelet
- * refinement, collapse to 8-wide nodes.  Replaces rt::Rays::new
+/* Builds one BLAS on the device: Morton order, binary tree by parallel
+ * locally-ordered clustering, collapse to 8-wide nodes.  Replaces rt::Rays::new
  * (crates/vork-render/src/rt.rs:1097-1289) and the driver build it requests
  * with BUILD_FLAGS=PREFER_FAST_TRACE (rt.rs:9-10).  `flags` takes
  * VH_DEVICE_POINTERS. */
@@ -133,8 +133,9 @@ typedef struct vh_instance {
 #define VH_TLAS_REBUILD 0u
 #define VH_TLAS_REFIT 1u
 
-/* Builds (or refits, keeping the topology) the instance hierarchy over
- * `blases`.  Replaces FrameRays::new / rt::encode / rebuild_tlas
+/* Builds the instance hierarchy over `blases`; VH_TLAS_REFIT keeps the
+ * handle and instance order and rebuilds the hierarchy from the moved
+ * instances' boxes.  Replaces FrameRays::new / rt::encode / rebuild_tlas
  * (rt.rs:1339-1385,1387-1463,1465) and the descriptor placement kernel
  * (gpu_tlas.rs:70-118).  On VH_TLAS_REFIT `*inout` must be a TLAS built over
  * the same BLAS list and instance count.  `flags` takes VH_DEVICE_POINTERS

@@ -252,7 +252,7 @@ int vh_tlas_build(vh_device* dev, vh_blas* const* blases, uint32_t blas_count, c
                                      instance_count, lo.as<float>(), hi.as<float>())))
         return bail(rc);
     // A refit keeps the handle and instance order; the 8-wide hierarchy is
-    // rebuilt from the new boxes (an LBVH rebuild costs about as much as a
+    // rebuilt from the new boxes (a clustering rebuild costs little more than a
     // quantised refit pass and keeps the tree quality of moving instances).
     WideBuild wb;
     if ((rc = build_wide_bvh(dev, s, lo.as<float>(), hi.as<float>(), instance_count, 1, &wb))) return bail(rc);

@@ -3,18 +3,22 @@
 //
 //  1. primitive boxes and centroid bounds
 //  2. 63-bit Morton codes of the centroids, radix-sorted with their ids
-//  3. binary radix tree over the sorted codes (Karras 2012), parents linked
-//  4. bottom-up box refit, one thread per leaf, the second child to arrive at
-//     a node continues upward
-//  5. top-down collapse into 8-wide nodes, one kernel per wide level: each
+//  3. binary tree by parallel locally-ordered clustering (PLOC, Meister and
+//     Bittner 2018): the clusters stay in Morton order; each round every
+//     cluster finds the neighbour within +-kPlocRadius positions whose merged
+//     box has the smallest surface area, mutual pairs merge into a new node
+//     (its box is the union, so no refit pass is needed), and a prefix sum
+//     compacts the survivors, until one cluster is left
+//  4. top-down collapse into 8-wide nodes, one kernel per wide level: each
 //     node opens its largest-area children until it holds 8; subtrees of at
 //     most `leaf_max` primitives become leaf children whose items are written
-//     contiguously per node
+//     contiguously per node; children are placed in octant-ordered slots
 //
-// The rocprim radix sort is the only library call; all other passes are
-// kernels written here.
+// rocprim's radix sort and prefix sum are the only library calls; all other
+// passes are kernels written here.
 #include <cstring>
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -25,8 +29,6 @@
 namespace vh {
 namespace {
 
-constexpr uint32_t kLeafBit = 0x80000000u;
-constexpr uint32_t kNone = 0xffffffffu;
 
 __device__ __forceinline__ uint32_t float_to_ordered(float f) {
     uint32_t u = f2u(f);
@@ -95,111 +97,136 @@ __global__ void k_morton(const float* lo, const float* hi, uint32_t n, const uin
     ids[i] = i;
 }
 
-// Karras 2012 prefix length with ties broken by index.
-__device__ __forceinline__ int delta(const uint64_t* keys, int n, int i, int j) {
-    if (j < 0 || j >= n) return -1;
-    uint64_t a = keys[i], b = keys[j];
-    if (a == b) return 64 + __clz((uint32_t)(i ^ j));
-    return __clzll(a ^ b);
+// ---------------------------------------------------------------- PLOC
+// Node ids: 0..n-1 are the leaves (sorted primitive i), n..2n-2 the merged
+// nodes in creation order; the root is 2n-2 (0 when n == 1).
+#ifndef VH_PLOC_RADIUS
+#define VH_PLOC_RADIUS 16
+#endif
+constexpr int kPlocRadius = VH_PLOC_RADIUS;
+constexpr int kPlocBlock = 256;
+
+__device__ __forceinline__ float half_area(const float lo[3], const float hi[3]) {
+    float dx = fmaxf(hi[0] - lo[0], 0.f), dy = fmaxf(hi[1] - lo[1], 0.f), dz = fmaxf(hi[2] - lo[2], 0.f);
+    return dx * dy + dy * dz + dz * dx;
 }
 
-// Internal node arrays (n-1 of them): left/right child refs (kLeafBit marks a
-// leaf = sorted position), parent, covered range [first, last].
-__global__ void k_karras(const uint64_t* keys, int n, uint32_t* left, uint32_t* right, uint32_t* parent,
-                         uint32_t* leaf_parent, uint32_t* first, uint32_t* last) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n - 1) return;
-    int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
-    int dmin = delta(keys, n, i, i - d);
-    int lmax = 2;
-    while (delta(keys, n, i, i + lmax * d) > dmin) lmax *= 2;
-    int l = 0;
-    for (int t = lmax / 2; t >= 1; t /= 2)
-        if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
-    int j = i + l * d;
-    int dnode = delta(keys, n, i, j);
-    int s = 0;
-    for (int div = 2;; div *= 2) {
-        int t = (l + div - 1) / div;
-        if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
-        if (t <= 1) break;
-    }
-    int gamma = i + s * d + min(d, 0);
-    int lo = min(i, j), hi = max(i, j);
-    uint32_t lc = (lo == gamma) ? (kLeafBit | (uint32_t)gamma) : (uint32_t)gamma;
-    uint32_t rc = (hi == gamma + 1) ? (kLeafBit | (uint32_t)(gamma + 1)) : (uint32_t)(gamma + 1);
-    left[i] = lc;
-    right[i] = rc;
-    first[i] = (uint32_t)lo;
-    last[i] = (uint32_t)hi;
-    if (lc & kLeafBit) leaf_parent[lc & ~kLeafBit] = (uint32_t)i; else parent[lc] = (uint32_t)i;
-    if (rc & kLeafBit) leaf_parent[rc & ~kLeafBit] = (uint32_t)i; else parent[rc] = (uint32_t)i;
-    if (i == 0) parent[0] = kNone;
-}
-
-__device__ __forceinline__ float aload(const float* p) {
-    return u2f(__hip_atomic_load((const uint32_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-}
-__device__ __forceinline__ void astore(float* p, float v) {
-    __hip_atomic_store((uint32_t*)p, f2u(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Bottom-up refit: node boxes are 6 floats (lo xyz, hi xyz).
-__global__ void k_refit(const float* plo, const float* phi, const uint32_t* sorted_ids, uint32_t n,
-                        const uint32_t* left, const uint32_t* right, const uint32_t* parent,
-                        const uint32_t* leaf_parent, uint32_t* flags, float* nb) {
+__global__ void k_ploc_init(const float* plo, const float* phi, const uint32_t* sorted_ids, uint32_t n, float* nb,
+                            uint32_t* size, uint32_t* clusters) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    uint32_t p = leaf_parent[i];
-    while (p != kNone) {
-        __threadfence();
-        uint32_t prev = atomicAdd(&flags[p], 1u);
-        if (prev == 0) return;  // the sibling subtree is not finished yet
-        __threadfence();
-        float lo[3], hi[3];
-        uint32_t c[2] = {left[p], right[p]};
-        for (int k = 0; k < 3; k++) { lo[k] = INFINITY; hi[k] = -INFINITY; }
-        for (int s = 0; s < 2; s++) {
-            if (c[s] & kLeafBit) {
-                uint32_t prim = sorted_ids[c[s] & ~kLeafBit];
-                for (int k = 0; k < 3; k++) {
-                    lo[k] = fminf(lo[k], plo[3 * prim + k]);
-                    hi[k] = fmaxf(hi[k], phi[3 * prim + k]);
-                }
-            } else {
-                for (int k = 0; k < 3; k++) {
-                    lo[k] = fminf(lo[k], aload(&nb[6 * c[s] + k]));
-                    hi[k] = fmaxf(hi[k], aload(&nb[6 * c[s] + 3 + k]));
-                }
-            }
+    uint32_t prim = sorted_ids[i];
+    for (int k = 0; k < 3; k++) {
+        nb[6 * i + k] = plo[3 * prim + k];
+        nb[6 * i + 3 + k] = phi[3 * prim + k];
+    }
+    size[i] = 1u;
+    clusters[i] = i;
+}
+
+// Nearest neighbour of each of the `count` clusters within +-kPlocRadius
+// positions: smallest surface area of the merged box, ties to the lower
+// position.  The block stages its clusters' boxes and the halo in LDS.
+__global__ __launch_bounds__(kPlocBlock) void k_ploc_nn(const uint32_t* clusters, uint32_t count, const float* nb,
+                                                        uint32_t* nn) {
+    constexpr int kSpan = kPlocBlock + 2 * kPlocRadius;
+    __shared__ float box[6][kSpan];
+    const int64_t base = (int64_t)blockIdx.x * kPlocBlock - kPlocRadius;
+    for (int k = threadIdx.x; k < kSpan; k += blockDim.x) {
+        const int64_t j = base + k;
+        if (j >= 0 && j < (int64_t)count) {
+            const uint32_t c = clusters[j];
+            for (int q = 0; q < 6; q++) box[q][k] = nb[6 * (size_t)c + q];
         }
-        for (int k = 0; k < 3; k++) { astore(&nb[6 * p + k], lo[k]); astore(&nb[6 * p + 3 + k], hi[k]); }
-        p = parent[p];
+    }
+    __syncthreads();
+    const uint32_t i = blockIdx.x * kPlocBlock + threadIdx.x;
+    if (i >= count) return;
+    const int li = threadIdx.x + kPlocRadius;
+    const float lo0 = box[0][li], lo1 = box[1][li], lo2 = box[2][li];
+    const float hi0 = box[3][li], hi1 = box[4][li], hi2 = box[5][li];
+    float best = INFINITY;
+    uint32_t bj = i;
+    for (int d = -kPlocRadius; d <= kPlocRadius; d++) {
+        const int64_t j = (int64_t)i + d;
+        if (d == 0 || j < 0 || j >= (int64_t)count) continue;
+        const int lj = li + d;
+        float lo[3] = {fminf(lo0, box[0][lj]), fminf(lo1, box[1][lj]), fminf(lo2, box[2][lj])};
+        float hi[3] = {fmaxf(hi0, box[3][lj]), fmaxf(hi1, box[4][lj]), fmaxf(hi2, box[5][lj])};
+        const float a = half_area(lo, hi);
+        if (a < best) { best = a; bj = (uint32_t)j; }
+    }
+    nn[i] = bj;
+}
+
+// Per cluster position: bit 0 = the cluster survives into the next round
+// (unpaired, or the lower position of a mutual pair, which becomes the
+// merged node), bit 32 = it creates a merged node.
+__global__ void k_ploc_flags(const uint32_t* nn, uint32_t count, uint64_t* flags) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const uint32_t j = nn[i];
+    const bool mutual = j != i && nn[j] == i;
+    const bool keep = !mutual || i < j;
+    const bool owner = mutual && i < j;
+    flags[i] = (uint64_t)(keep ? 1u : 0u) | ((uint64_t)(owner ? 1u : 0u) << 32);
+}
+
+// Writes the merged nodes and the next round's cluster array; the last
+// thread stores the round's totals (survivors, merged nodes).
+__global__ void k_ploc_merge(const uint32_t* clusters, const uint32_t* nn, uint32_t count, const uint64_t* flags,
+                             const uint64_t* offsets, uint32_t n, uint32_t node_base, float* nb, uint32_t* left,
+                             uint32_t* right, uint32_t* size, uint32_t* next, uint32_t* totals) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const uint64_t f = flags[i], o = offsets[i];
+    if (f & 1u) {
+        uint32_t id = clusters[i];
+        if (f >> 32) {
+            const uint32_t a = id, b = clusters[nn[i]];
+            id = node_base + (uint32_t)(o >> 32);
+            for (int k = 0; k < 3; k++) {
+                nb[6 * (size_t)id + k] = fminf(nb[6 * (size_t)a + k], nb[6 * (size_t)b + k]);
+                nb[6 * (size_t)id + 3 + k] = fmaxf(nb[6 * (size_t)a + 3 + k], nb[6 * (size_t)b + 3 + k]);
+            }
+            left[id - n] = a;
+            right[id - n] = b;
+            size[id] = size[a] + size[b];
+        }
+        next[(uint32_t)o] = id;
+    }
+    if (i == count - 1) {
+        totals[0] = (uint32_t)o + (uint32_t)(f & 1u);
+        totals[1] = (uint32_t)(o >> 32) + (uint32_t)(f >> 32);
     }
 }
 
 struct Tree {
-    const uint32_t *left, *right, *first, *last, *sorted_ids;
-    const float *nb, *plo, *phi;
+    const uint32_t *left, *right, *size, *sorted_ids;
+    const float* nb;
+    uint32_t n;
 };
 
-__device__ __forceinline__ uint32_t ref_size(const Tree& t, uint32_t r) {
-    return (r & kLeafBit) ? 1u : t.last[r] - t.first[r] + 1u;
-}
-__device__ __forceinline__ uint32_t ref_first(const Tree& t, uint32_t r) {
-    return (r & kLeafBit) ? (r & ~kLeafBit) : t.first[r];
-}
+__device__ __forceinline__ bool is_leaf(const Tree& t, uint32_t r) { return r < t.n; }
 __device__ __forceinline__ void ref_box(const Tree& t, uint32_t r, float lo[3], float hi[3]) {
-    if (r & kLeafBit) {
-        uint32_t prim = t.sorted_ids[r & ~kLeafBit];
-        for (int k = 0; k < 3; k++) { lo[k] = t.plo[3 * prim + k]; hi[k] = t.phi[3 * prim + k]; }
-    } else {
-        for (int k = 0; k < 3; k++) { lo[k] = t.nb[6 * r + k]; hi[k] = t.nb[6 * r + 3 + k]; }
-    }
+    for (int k = 0; k < 3; k++) { lo[k] = t.nb[6 * (size_t)r + k]; hi[k] = t.nb[6 * (size_t)r + 3 + k]; }
 }
-__device__ __forceinline__ float half_area(const float lo[3], const float hi[3]) {
-    float dx = fmaxf(hi[0] - lo[0], 0.f), dy = fmaxf(hi[1] - lo[1], 0.f), dz = fmaxf(hi[2] - lo[2], 0.f);
-    return dx * dy + dy * dz + dz * dx;
+// Primitives under a subtree of at most 4 of them, left to right.
+__device__ __forceinline__ uint32_t gather_items(const Tree& t, uint32_t r, uint32_t* out) {
+    uint32_t st[4];
+    int sp = 0;
+    uint32_t c = 0;
+    st[sp++] = r;
+    while (sp > 0) {
+        const uint32_t x = st[--sp];
+        if (is_leaf(t, x)) {
+            out[c++] = t.sorted_ids[x];
+        } else if (sp + 2 <= 4) {
+            st[sp++] = t.right[x - t.n];
+            st[sp++] = t.left[x - t.n];
+        }
+    }
+    return c;
 }
 
 // Quantise one axis of up to 8 child boxes against the node's lower corner,
@@ -244,17 +271,17 @@ __global__ void k_collapse(Tree t, const QueueItem* in, const uint32_t* in_count
     QueueItem it = in[idx];
     uint32_t c[kWidth];
     int m = 0;
-    if (it.binary == kNone) {
+    if (is_leaf(t, it.binary)) {
         // a single-primitive tree: one leaf child
-        c[m++] = kLeafBit | 0u;
+        c[m++] = it.binary;
     } else {
-        c[m++] = t.left[it.binary];
-        c[m++] = t.right[it.binary];
+        c[m++] = t.left[it.binary - t.n];
+        c[m++] = t.right[it.binary - t.n];
         while (m < kWidth) {
             int best = -1;
             float best_area = -1.f;
             for (int k = 0; k < m; k++) {
-                if ((c[k] & kLeafBit) || ref_size(t, c[k]) <= leaf_max) continue;
+                if (is_leaf(t, c[k]) || t.size[c[k]] <= leaf_max) continue;
                 float lo[3], hi[3];
                 ref_box(t, c[k], lo, hi);
                 float a = half_area(lo, hi);
@@ -262,8 +289,8 @@ __global__ void k_collapse(Tree t, const QueueItem* in, const uint32_t* in_count
             }
             if (best < 0) break;
             uint32_t r = c[best];
-            c[best] = t.left[r];
-            c[m++] = t.right[r];
+            c[best] = t.left[r - t.n];
+            c[m++] = t.right[r - t.n];
         }
     }
     float clo[3][kWidth], chi[3][kWidth];
@@ -319,8 +346,8 @@ __global__ void k_collapse(Tree t, const QueueItem* in, const uint32_t* in_count
     node.ex = e[0]; node.ey = e[1]; node.ez = e[2];
     uint32_t n_internal = 0, n_items = 0;
     for (int k = 0; k < m; k++) {
-        bool internal = !(c[k] & kLeafBit) && ref_size(t, c[k]) > leaf_max;
-        if (internal) n_internal++; else n_items += ref_size(t, c[k]);
+        bool internal = !is_leaf(t, c[k]) && t.size[c[k]] > leaf_max;
+        if (internal) n_internal++; else n_items += t.size[c[k]];
     }
     uint32_t child_base = n_internal ? atomicAdd(node_counter, n_internal) : 0u;
     uint32_t item_base = n_items ? atomicAdd(item_counter, n_items) : 0u;
@@ -331,7 +358,7 @@ __global__ void k_collapse(Tree t, const QueueItem* in, const uint32_t* in_count
         node.meta[q] = (uint8_t)kMetaNoItems;
         const int k = slot_child[q];
         if (k < 0) continue;
-        bool internal = !(c[k] & kLeafBit) && ref_size(t, c[k]) > leaf_max;
+        bool internal = !is_leaf(t, c[k]) && t.size[c[k]] > leaf_max;
         if (internal) {
             imask |= 1u << q;
             uint32_t slot = atomicAdd(out_count, 1u);
@@ -339,9 +366,10 @@ __global__ void k_collapse(Tree t, const QueueItem* in, const uint32_t* in_count
             out[slot].wide = child_base + ki;
             ki++;
         } else {
-            uint32_t cnt = ref_size(t, c[k]), f0 = ref_first(t, c[k]);
+            uint32_t prims[4];
+            const uint32_t cnt = gather_items(t, c[k], prims);
             node.meta[q] = (uint8_t)(off | ((cnt - 1) << 5));
-            for (uint32_t j = 0; j < cnt; j++) order[item_base + off + j] = t.sorted_ids[f0 + j];
+            for (uint32_t j = 0; j < cnt; j++) order[item_base + off + j] = prims[j];
             off += cnt;
         }
     }
@@ -350,13 +378,15 @@ __global__ void k_collapse(Tree t, const QueueItem* in, const uint32_t* in_count
     if (ki) atomicMax(max_level, level + 1);
 }
 
-// SAH cost of the binary tree (C_node = 1, C_prim = 1), relative to the root.
-__global__ void k_sah(const float* nb, uint32_t n_internal, double* out) {
+// SAH cost of the binary tree relative to its root, with a cost of 1 per
+// node and per primitive: the sum over all 2n-1 nodes (leaves included) of
+// area / root area.
+__global__ void k_sah(const float* nb, uint32_t nodes, const float* rootb, double* out) {
     __shared__ double s[256];
     double acc = 0;
-    float rlo[3] = {nb[0], nb[1], nb[2]}, rhi[3] = {nb[3], nb[4], nb[5]};
+    float rlo[3] = {rootb[0], rootb[1], rootb[2]}, rhi[3] = {rootb[3], rootb[4], rootb[5]};
     double root = half_area(rlo, rhi);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_internal; i += gridDim.x * blockDim.x) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nodes; i += gridDim.x * blockDim.x) {
         float lo[3] = {nb[6 * i], nb[6 * i + 1], nb[6 * i + 2]}, hi[3] = {nb[6 * i + 3], nb[6 * i + 4], nb[6 * i + 5]};
         acc += root > 0 ? half_area(lo, hi) / root : 0.0;
     }
@@ -377,8 +407,9 @@ int build_wide_bvh(vh_device* dev, hipStream_t s, const float* d_lo, const float
                    WideBuild* out) {
     if (n == 0) return fail(VH_ERR_USAGE, "cannot build a BVH over zero primitives");
     const uint32_t n_int = n > 1 ? n - 1 : 0;
-    DeviceBuffer bounds, keys, keys_sorted, ids, ids_sorted, left, right, parent, leaf_parent, first, last, flags, nb,
-        tmp, qa, qb, counters, sah;
+    const size_t n_nodes = (size_t)n + n_int;
+    DeviceBuffer bounds, keys, keys_sorted, ids, ids_sorted, left, right, size, nb, ca, cb, nn, flags, offsets, tmp,
+        scan_tmp, qa, qb, counters, sah;
     VH_HIP(bounds.allocate(6 * sizeof(uint32_t)));
     VH_HIP(keys.allocate(sizeof(uint64_t) * n));
     VH_HIP(keys_sorted.allocate(sizeof(uint64_t) * n));
@@ -386,12 +417,13 @@ int build_wide_bvh(vh_device* dev, hipStream_t s, const float* d_lo, const float
     VH_HIP(ids_sorted.allocate(sizeof(uint32_t) * n));
     VH_HIP(left.allocate(sizeof(uint32_t) * (n_int + 1)));
     VH_HIP(right.allocate(sizeof(uint32_t) * (n_int + 1)));
-    VH_HIP(parent.allocate(sizeof(uint32_t) * (n_int + 1)));
-    VH_HIP(leaf_parent.allocate(sizeof(uint32_t) * n));
-    VH_HIP(first.allocate(sizeof(uint32_t) * (n_int + 1)));
-    VH_HIP(last.allocate(sizeof(uint32_t) * (n_int + 1)));
-    VH_HIP(flags.allocate(sizeof(uint32_t) * (n_int + 1)));
-    VH_HIP(nb.allocate(sizeof(float) * 6 * (n_int + 1)));
+    VH_HIP(size.allocate(sizeof(uint32_t) * n_nodes));
+    VH_HIP(nb.allocate(sizeof(float) * 6 * n_nodes));
+    VH_HIP(ca.allocate(sizeof(uint32_t) * n));
+    VH_HIP(cb.allocate(sizeof(uint32_t) * n));
+    VH_HIP(nn.allocate(sizeof(uint32_t) * n));
+    VH_HIP(flags.allocate(sizeof(uint64_t) * n));
+    VH_HIP(offsets.allocate(sizeof(uint64_t) * n));
     VH_HIP(counters.allocate(sizeof(uint32_t) * 8));
     VH_HIP(sah.allocate(sizeof(double)));
 
@@ -406,15 +438,35 @@ int build_wide_bvh(vh_device* dev, hipStream_t s, const float* d_lo, const float
     VH_HIP(tmp.allocate(tmp_bytes));
     VH_HIP(rocprim::radix_sort_pairs(tmp.ptr, tmp_bytes, keys.as<uint64_t>(), keys_sorted.as<uint64_t>(),
                                      ids.as<uint32_t>(), ids_sorted.as<uint32_t>(), n, 0, 63, s));
-    if (n > 1) {
-        hipLaunchKernelGGL(k_karras, dim3(blocks(n_int, 256)), dim3(256), 0, s, keys_sorted.as<uint64_t>(), (int)n,
-                           left.as<uint32_t>(), right.as<uint32_t>(), parent.as<uint32_t>(),
-                           leaf_parent.as<uint32_t>(), first.as<uint32_t>(), last.as<uint32_t>());
-        VH_HIP(hipMemsetAsync(flags.ptr, 0, sizeof(uint32_t) * n_int, s));
-        hipLaunchKernelGGL(k_refit, dim3(blocks(n, 256)), dim3(256), 0, s, d_lo, d_hi, ids_sorted.as<uint32_t>(), n,
-                           left.as<uint32_t>(), right.as<uint32_t>(), parent.as<uint32_t>(),
-                           leaf_parent.as<uint32_t>(), flags.as<uint32_t>(), nb.as<float>());
+    hipLaunchKernelGGL(k_ploc_init, dim3(blocks(n, 256)), dim3(256), 0, s, d_lo, d_hi, ids_sorted.as<uint32_t>(), n,
+                       nb.as<float>(), size.as<uint32_t>(), ca.as<uint32_t>());
+    uint32_t* totals = counters.as<uint32_t>() + 6;
+    size_t scan_bytes = 0;
+    VH_HIP(rocprim::exclusive_scan(nullptr, scan_bytes, flags.as<uint64_t>(), offsets.as<uint64_t>(), (uint64_t)0, n,
+                                   rocprim::plus<uint64_t>(), s));
+    VH_HIP(scan_tmp.allocate(scan_bytes));
+    uint32_t clusters = n, node_base = n;
+    uint32_t *cl_in = ca.as<uint32_t>(), *cl_out = cb.as<uint32_t>();
+    while (clusters > 1) {
+        hipLaunchKernelGGL(k_ploc_nn, dim3(blocks(clusters, kPlocBlock)), dim3(kPlocBlock), 0, s, cl_in, clusters,
+                           nb.as<float>(), nn.as<uint32_t>());
+        hipLaunchKernelGGL(k_ploc_flags, dim3(blocks(clusters, 256)), dim3(256), 0, s, nn.as<uint32_t>(), clusters,
+                           flags.as<uint64_t>());
+        VH_HIP(rocprim::exclusive_scan(scan_tmp.ptr, scan_bytes, flags.as<uint64_t>(), offsets.as<uint64_t>(),
+                                       (uint64_t)0, clusters, rocprim::plus<uint64_t>(), s));
+        hipLaunchKernelGGL(k_ploc_merge, dim3(blocks(clusters, 256)), dim3(256), 0, s, cl_in, nn.as<uint32_t>(), clusters,
+                           flags.as<uint64_t>(), offsets.as<uint64_t>(), n, node_base, nb.as<float>(),
+                           left.as<uint32_t>(), right.as<uint32_t>(), size.as<uint32_t>(), cl_out, totals);
+        uint32_t tot[2];
+        VH_HIP(hipMemcpyAsync(tot, totals, sizeof(tot), hipMemcpyDeviceToHost, s));
+        VH_HIP(hipStreamSynchronize(s));
+        if (tot[0] >= clusters || tot[1] == 0 || node_base + tot[1] > n_nodes)
+            return fail(VH_ERR_DEVICE, "BVH clustering made no progress");
+        clusters = tot[0];
+        node_base += tot[1];
+        std::swap(cl_in, cl_out);
     }
+    const uint32_t root_id = (uint32_t)(n_nodes - 1);
     // Upper bound of wide nodes: every wide node but the root has > leaf_max
     // primitives under it except through the tree shape; n internal binary
     // nodes bound it.
@@ -426,10 +478,10 @@ int build_wide_bvh(vh_device* dev, hipStream_t s, const float* d_lo, const float
     // counters: 0 node_counter, 1 item_counter, 2 count_a, 3 count_b, 4 max_level
     uint32_t init[8] = {1, 0, 1, 0, 0, 0, 0, 0};
     VH_HIP(hipMemcpyAsync(counters.ptr, init, sizeof(init), hipMemcpyHostToDevice, s));
-    QueueItem root = {n > 1 ? 0u : kNone, 0u};
+    QueueItem root = {root_id, 0u};
     VH_HIP(hipMemcpyAsync(qa.ptr, &root, sizeof(root), hipMemcpyHostToDevice, s));
-    Tree t{left.as<uint32_t>(), right.as<uint32_t>(), first.as<uint32_t>(), last.as<uint32_t>(),
-           ids_sorted.as<uint32_t>(), nb.as<float>(), d_lo, d_hi};
+    Tree t{left.as<uint32_t>(), right.as<uint32_t>(), size.as<uint32_t>(), ids_sorted.as<uint32_t>(), nb.as<float>(),
+           n};
     uint32_t* cnt = counters.as<uint32_t>();
     uint32_t count = 1, level = 0;
     QueueItem *qin = qa.as<QueueItem>(), *qout = qb.as<QueueItem>();
@@ -447,18 +499,16 @@ int build_wide_bvh(vh_device* dev, hipStream_t s, const float* d_lo, const float
     uint32_t host_counters[8];
     VH_HIP(hipMemcpyAsync(host_counters, cnt, sizeof(host_counters), hipMemcpyDeviceToHost, s));
     double sah_sum = 0;
-    if (n_int) {
-        VH_HIP(hipMemsetAsync(sah.ptr, 0, sizeof(double), s));
-        hipLaunchKernelGGL(k_sah, dim3(std::min<uint32_t>(blocks(n_int, 256), 512)), dim3(256), 0, s, nb.as<float>(),
-                           n_int, sah.as<double>());
-        VH_HIP(hipMemcpyAsync(&sah_sum, sah.ptr, sizeof(double), hipMemcpyDeviceToHost, s));
-    }
+    VH_HIP(hipMemsetAsync(sah.ptr, 0, sizeof(double), s));
+    hipLaunchKernelGGL(k_sah, dim3(std::min<uint32_t>(blocks(n_nodes, 256), 512)), dim3(256), 0, s, nb.as<float>(),
+                       (uint32_t)n_nodes, nb.as<float>() + 6 * (size_t)root_id, sah.as<double>());
+    VH_HIP(hipMemcpyAsync(&sah_sum, sah.ptr, sizeof(double), hipMemcpyDeviceToHost, s));
     VH_HIP(hipStreamSynchronize(s));
     if (host_counters[1] != n) return fail(VH_ERR_DEVICE, "BVH collapse lost primitives");
     out->node_count = host_counters[0];
     out->binary_nodes = n_int;
     out->max_depth = level;
-    out->sah_cost = sah_sum + (double)n;
+    out->sah_cost = sah_sum;
     return VH_OK;
 }
 

@@ -107,7 +107,7 @@ struct vh_scene {
 namespace vh {
 
 // ---------------------------------------------------------------- build
-// Generic LBVH over n boxes, collapsed to 8-wide nodes.  `lo`/`hi` are device
+// Generic BVH over n boxes (Morton order + clustering), collapsed to 8-wide nodes.  `lo`/`hi` are device
 // arrays of 3 floats per primitive.  Writes the nodes and the leaf-order
 // permutation `order` (leaf item -> primitive) and returns the node count.
 struct WideBuild {
