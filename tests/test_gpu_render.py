@@ -96,3 +96,23 @@ def test_device_film_accumulates(renderer):
     inst.render(p, device_film=dev)
     torch.cuda.synchronize()
     np.testing.assert_array_equal(dev.cpu().numpy(), host)
+
+
+def test_instanced_flock_matches_oracle(renderer):
+    """Config 2 at test size: thousands of instances of one BLAS, TLAS rebuilt
+    for a later animation frame through the refit entry point."""
+    sc = S.config2(width=64, height=36, spp=4, birds=3000)
+    sc.camera = dict(eye=(0.0, 24.0, 60.0), target=(0.0, 21.0, 0.0))
+    inst, ora, g, o32, o64, p = _render_pair(renderer, sc, 4)
+    gm, om = g[..., :3] / 4, o64[..., :3] / 4
+    noise = rel_mse(om, ora.render(sc.render_params(samples=4, seed=57))[1][..., :3] / 4)
+    assert rel_mse(gm, om) < 0.05 * noise
+    # a later frame, through VH_TLAS_REFIT on the same handle
+    frames = S.flock_frames(sc, 3)
+    sc2 = S.Scene(sc.meshes, frames[2], sc.materials, sc.lights, sc.camera, sc.sky, sc.width, sc.height, sc.spp,
+                  sc.max_depth, sc.seed, "config2")
+    inst.tlas.build(sc2.instance_array(), refit=True)
+    g2 = inst.render(p)
+    o2 = oracle.OracleScene(sc2).render(p)[1]
+    assert rel_mse(g2[..., :3] / 4, o2[..., :3] / 4) < 0.05 * noise
+    assert not np.array_equal(g2, g)

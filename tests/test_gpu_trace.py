@@ -178,3 +178,60 @@ def test_secondary_rays_match_oracle_on_ggx_scene(renderer):
     assert np.array_equal(inst.tlas.trace_any(rays), ora.trace_any(rays))
     sub = rays[:3000]
     _compare(inst.tlas.trace_closest(sub), ora.trace_closest(sub, brute_force=True))
+
+
+def test_flock_instances_match_oracle_bvh(renderer):
+    sc = S.config2(width=32, height=32, spp=1, birds=5000)
+    inst = renderer.install(sc)
+    ora = oracle.OracleScene(sc)
+    rng = np.random.default_rng(4)
+    n = 20000
+    o = rng.normal(size=(n, 3)) * np.array([40.0, 3.0, 6.0]) + np.array([0.0, 22.0, 0.0])
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, :3], rays[:, 4:7], rays[:, 7] = o, d, 1e30
+    g = inst.tlas.trace_closest(rays)
+    assert (g["t"] > 0).mean() > 0.05
+    _compare(g, ora.trace_closest(rays))
+    assert np.array_equal(inst.tlas.trace_any(rays), ora.trace_any(rays))
+
+
+def test_axis_parallel_rays_match_brute_force_and_stay_cheap(renderer):
+    """Rays with exactly zero direction components: same hits as brute force,
+    and no loss of pruning on the zero axis (a horizontal ray through the
+    flock must not enter every instance along its xz path)."""
+    sc = S.config0()
+    inst = renderer.install(sc)
+    ora = oracle.OracleScene(sc)
+    rng = np.random.default_rng(8)
+    n = 3000
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, :3] = rng.uniform(-6, 6, size=(n, 3)) * np.array([1.0, 0.4, 1.0]) + np.array([0.0, 1.0, 0.0])
+    d = rng.normal(size=(n, 3))
+    zero = rng.integers(0, 3, size=n)
+    d[np.arange(n), zero] = 0.0
+    two = rng.uniform(size=n) < 0.3
+    d[two, (zero[two] + 1) % 3] = 0.0
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays[:, 4:7] = d
+    rays[:, 7] = 1e30
+    rays[::7, 1] = 0.0  # some exactly in the grid's plane
+    _compare(inst.tlas.trace_closest(rays), ora.trace_closest(rays, brute_force=True))
+    assert np.array_equal(inst.tlas.trace_any(rays), ora.trace_any(rays, brute_force=True))
+
+    flock = S.config2(width=32, height=32, spp=1, birds=20000)
+    fi = renderer.install(flock)
+    m = 512
+    flat = np.zeros((m, 8), np.float32)
+    flat[:, 0] = -150.0
+    flat[:, 1] = np.linspace(16.0, 28.0, m)
+    flat[:, 2] = np.linspace(-4.0, 4.0, m)
+    flat[:, 4] = 1.0
+    flat[:, 7] = 1e30
+    tilted = flat.copy()
+    tilted[:, 5] = 1e-6
+    tilted[:, 4:7] /= np.linalg.norm(tilted[:, 4:7], axis=1, keepdims=True)
+    c_flat = fi.tlas.trace_counts(flat)[:, 0].astype(np.int64)
+    c_tilt = fi.tlas.trace_counts(tilted)[:, 0].astype(np.int64)
+    assert c_flat.sum() <= 2 * c_tilt.sum() + 1000, (c_flat.sum(), c_tilt.sum())

@@ -6,6 +6,8 @@ import time rather than as corrupted records.
 """
 import ctypes as C
 
+import numpy as np
+
 VH_OK, VH_ERR_USAGE, VH_ERR_SCENE, VH_ERR_DEVICE, VH_ERR_IO = 0, 2, 3, 4, 5
 VH_DEVICE_POINTERS = 1
 VH_RAY_CAMERA, VH_RAY_SHADOW, VH_RAY_DIFFUSE, VH_RAY_SPECULAR, VH_RAY_TRANSMISSION = 1, 2, 4, 8, 16
@@ -31,6 +33,12 @@ class BlasInfo(C.Structure):
 class Instance(C.Structure):
     _fields_ = [("transform", C.c_float * 12), ("blas", C.c_uint32), ("custom_index", C.c_uint32),
                 ("material", C.c_uint32), ("mask", C.c_uint8), ("flags", C.c_uint8), ("reserved", C.c_uint16)]
+
+
+# numpy view of vh_instance, for instance tables too large for Python lists
+INSTANCE_DTYPE = np.dtype([("transform", "<f4", (12,)), ("blas", "<u4"), ("custom_index", "<u4"),
+                           ("material", "<u4"), ("mask", "u1"), ("flags", "u1"), ("reserved", "<u2")])
+assert INSTANCE_DTYPE.itemsize == C.sizeof(Instance) == 64
 
 
 class Ray(C.Structure):

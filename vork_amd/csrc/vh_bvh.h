@@ -147,7 +147,16 @@ VH_HD bool intersect_triangle(f3 a, f3 b, f3 c, f3 o, f3 d, const Shear& sh, flo
     return true;
 }
 
-VH_HD f3 safe_inv(f3 d) { VH_EXACT return mk(1.f / d.x, 1.f / d.y, 1.f / d.z); }
+// Reciprocal direction of the box tests.  Components below 1e-30 in
+// magnitude (zero included) become +-1e-30 with their sign first, so an
+// axis-parallel ray still prunes on that axis: infinite reciprocals would
+// leave the axis unconstrained, and a horizontal ray through a dense instance
+// field would then enter every box along its xz path.  The substitute ray
+// drifts from the true one by at most 1e-30 * t on the axis, far inside the
+// box test's margin (2^-19 of the axis' coordinate magnitude) for any t below
+// 1e20.
+VH_HD float box_dir(float x) { return fabsf(x) < 1e-30f ? copysignf(1e-30f, x) : x; }
+VH_HD f3 box_inv(f3 d) { VH_EXACT return mk(1.f / box_dir(d.x), 1.f / box_dir(d.y), 1.f / box_dir(d.z)); }
 
 // (t, instance, primitive) lexicographic order: the rule at equal t.
 VH_HD bool better_hit(float t, uint32_t inst, uint32_t prim, float bt, uint32_t binst, uint32_t bprim) {

@@ -141,9 +141,9 @@ constexpr float kCullScale = 1.0f + 1.0f / 262144.0f;  // 1 + 2^-18
 // largest magnitude any term on the axis takes.  Near planes are shifted
 // down and far planes up by 2^-19 M_a, which is larger than that error, so
 // a child whose exact box meets [tmin, tmax] is always reported.  A zero
-// direction component gives infinite inv_a; the planes of that axis then come
-// out as -inf / +inf or NaN, and NaN drops out of fmin / fmax, so the axis
-// does not constrain the test.
+// direction component never reaches here (box_inv keeps inv_a finite); if an
+// overflow still makes a plane infinite or NaN, NaN drops out of fmin / fmax,
+// so the axis does not constrain the test and it stays conservative.
 __device__ __forceinline__ void child_groups(const NodeBits& nb, f3 o, f3 inv, uint32_t oct, float tmin, float tmax,
                                              uint32_t& hits, uint32_t& items) {
     const WideNode& n = nb.n;
@@ -224,7 +224,7 @@ struct Tracer {
     __device__ __forceinline__ void begin(const TlasView& tv, f3 o_, f3 d_, float tmin_, float tmax_, uint32_t mask_,
                                           Stack& st) {
         wo = o_; wd = d_;
-        o = o_; inv = safe_inv(d_); oct = octant(inv);
+        o = o_; inv = box_inv(d_); oct = octant(inv);
         ld = d_;
         lsh = make_shear(d_);
         tmin = tmin_; tmax = tmax_; mask = mask_;
@@ -239,7 +239,7 @@ struct Tracer {
 
     __device__ __forceinline__ void leave_instance() {
         o = wo;
-        inv = safe_inv(wd);
+        inv = box_inv(wd);
         oct = octant(inv);
         in_blas = false;
     }
@@ -257,7 +257,7 @@ struct Tracer {
         for (int k = 0; k < 12; k++) w2o[k] = r.w2o[k];
         o = xform_point(w2o, wo);
         ld = xform_dir(w2o, wd);
-        inv = safe_inv(ld);
+        inv = box_inv(ld);
         oct = octant(inv);
         lsh = make_shear(ld);
         cull = (mf >> 8) & 1u;

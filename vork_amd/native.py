@@ -179,12 +179,13 @@ class Blas:
 
 
 class Tlas:
-    def __init__(self, device, blases, instances):
-        """instances: a ctypes array of abi.Instance."""
+    def __init__(self, device, blases, instances, device_pointers=False, count=None):
+        """instances: a ctypes array of abi.Instance, or with device_pointers a
+        device tensor holding `count` of them."""
         self.device = device
         self.blases = list(blases)
         self.handle = C.c_void_p()
-        self.build(instances)
+        self.build(instances, device_pointers=device_pointers, count=count)
 
     def build(self, instances, refit=False, device_pointers=False, count=None):
         arr = (C.c_void_p * max(1, len(self.blases)))(*[b.handle.value for b in self.blases])

@@ -51,7 +51,8 @@ class Mesh:
 @dataclass
 class Scene:
     meshes: list
-    instances: list                # list of dict(transform=(12,), blas=int, material=int, mask=int, flags=int, custom=int)
+    instances: object              # list of dict(transform=(12,), blas=int, material=int, mask=int, flags=int,
+                                   # custom=int), or a numpy array of abi.INSTANCE_DTYPE
     materials: np.ndarray          # structured list of abi.Material-compatible tuples
     lights: list
     camera: dict
@@ -66,10 +67,17 @@ class Scene:
 
     @property
     def triangle_count(self):
+        if isinstance(self.instances, np.ndarray):
+            per = np.array([m.triangle_count for m in self.meshes], np.int64)
+            return int(per[self.instances["blas"]].sum())
         return sum(self.meshes[i["blas"]].triangle_count for i in self.instances)
 
     # ---------------------------------------------------------------- ABI
     def instance_array(self):
+        if isinstance(self.instances, np.ndarray):
+            a = np.ascontiguousarray(self.instances, dtype=abi.INSTANCE_DTYPE)
+            self.instances = a
+            return (abi.Instance * a.shape[0]).from_buffer(a)
         arr = (abi.Instance * len(self.instances))()
         for k, inst in enumerate(self.instances):
             arr[k].transform[:] = [float(v) for v in inst["transform"]]
@@ -248,8 +256,11 @@ def vertex_normals(pos, idx):
         for c in range(3):
             n[:, c] += np.bincount(idx[:, k], weights=f[:, c], minlength=p.shape[0])
     ln = np.linalg.norm(n, axis=1, keepdims=True)
-    ln[ln == 0] = 1
-    return (n / ln).astype(np.float32)
+    unused = ln[:, 0] == 0
+    ln[unused] = 1
+    n = n / ln
+    n[unused] = (0.0, 1.0, 0.0)  # vertices no triangle references
+    return n.astype(np.float32)
 
 
 # --------------------------------------------------------------- noise
@@ -392,3 +403,123 @@ def small_config1(seed=1, width=96, height=54, spp=16):
     """config1 at test size: the same generators, a level-4 icosphere and a
     64x64 terrain."""
     return config1(seed, width, height, spp, 8, sphere_level=4, terrain=64, materials=8)
+
+
+# ------------------------------------------------------------ config 2
+def bird(seed=1):
+    """A 2,000-triangle bird: a UV-sphere body (32 x 16, 960 triangles)
+    stretched along +z, and two 13 x 20-cell wings (520 triangles each) with
+    a slight dihedral; about 2.2 units of wingspan."""
+    body = uv_sphere(1.0, 32, 16)
+    bp = body.positions.astype(np.float64) * np.array([0.22, 0.18, 0.6])
+    parts_p, parts_i = [bp], [body.indices.astype(np.int64)]
+    off = bp.shape[0]
+    for side in (-1.0, 1.0):
+        w = grid((1.0, 0.5), (13, 20))
+        wp = w.positions.astype(np.float64)
+        x = (wp[:, 0] + 0.5) * side * 1.0 + side * 0.12      # 0.12 .. 1.12 out from the body
+        z = wp[:, 2] * (1.0 - 0.5 * (wp[:, 0] + 0.5))         # tapering chord
+        y = 0.15 * np.abs(x) + 0.02
+        q = np.stack([x, y, z], -1)
+        idx = w.indices.astype(np.int64)
+        if side < 0:
+            idx = idx[:, [0, 2, 1]]
+        parts_p.append(q)
+        parts_i.append(idx + off)
+        off += q.shape[0]
+    pos = np.concatenate(parts_p).astype(np.float32)
+    idx = np.concatenate(parts_i).astype(np.uint32)
+    m = Mesh(pos, vertex_normals(pos, idx), idx, name="bird")
+    m.triangle_materials = np.concatenate([np.zeros(960, np.uint32), np.ones(1040, np.uint32)])
+    assert m.triangle_count == 2000
+    return m
+
+
+def curl_velocity(p, t):
+    """Divergence-free velocity: the curl of a sum-of-sines vector potential,
+    evaluated analytically (units per second)."""
+    x, y, z = p[:, 0], p[:, 1], p[:, 2]
+    a, b, c = 0.071, 0.23, 0.13
+    # potential psi = (sin(b y + t), sin(c z + 0.7 t), sin(a x + 1.3 t)) * s
+    s = np.array([6.0, 4.0, 5.0])
+    dpz_dy = 0.0
+    dpy_dz = s[1] * c * np.cos(c * z + 0.7 * t)
+    dpx_dz = 0.0
+    dpz_dx = s[2] * a * np.cos(a * x + 1.3 * t)
+    dpy_dx = 0.0
+    dpx_dy = s[0] * b * np.cos(b * y + t)
+    return np.stack([dpz_dy - dpy_dz, dpx_dz - dpz_dx, dpy_dx - dpx_dy], -1) + np.array([4.0, 0.0, 0.0])
+
+
+def flock_instances(positions, velocities, scales, seed=1):
+    """Instance table of the flock: yaw from the velocity, uniform scale."""
+    n = positions.shape[0]
+    inst = np.zeros(n, abi.INSTANCE_DTYPE)
+    yaw = np.arctan2(velocities[:, 0], velocities[:, 2])
+    c, s = np.cos(yaw) * scales, np.sin(yaw) * scales
+    tr = np.zeros((n, 12))
+    tr[:, 0], tr[:, 2], tr[:, 3] = c, s, positions[:, 0]
+    tr[:, 5], tr[:, 7] = scales, positions[:, 1]
+    tr[:, 8], tr[:, 10], tr[:, 11] = -s, c, positions[:, 2]
+    inst["transform"] = tr.astype(np.float32)
+    inst["blas"] = 1
+    inst["custom_index"] = np.arange(n, dtype=np.uint32) & 0xFFFFFF
+    inst["material"] = 0
+    inst["mask"] = 0xFF
+    return inst
+
+
+def flock_frames(scene, frames, dt=1.0 / 24.0):
+    """Instance tables of `frames` successive animation frames of a config2
+    scene (frame 0 is the scene's own table): each frame advects the birds by
+    the curl-noise velocity for `dt` and re-yaws them."""
+    base = scene.instances
+    birds = base[1:]
+    tr = birds["transform"].astype(np.float64)
+    pos = tr[:, [3, 7, 11]].copy()
+    scales = tr[:, 5].copy()
+    t = scene.notes.get("frame", 0) * dt
+    out = [base.copy()]
+    for _ in range(frames - 1):
+        pos = pos + curl_velocity(pos, t) * dt
+        t += dt
+        f = base.copy()
+        f[1:] = flock_instances(pos, curl_velocity(pos, t), scales)
+        out.append(f)
+    return out
+
+
+def config2(seed=1, width=1920, height=1080, spp=4, max_depth=3, birds=1_000_000, frame=0, dt=1.0 / 24.0):
+    """Instanced flock: `birds` copies of the 2,000-triangle bird placed from a
+    Gaussian ribbon with sigma (40, 3, 6), advected by curl noise for
+    `frame` steps of `dt`, yawed along their velocity and scaled U[0.9, 1.1];
+    a ground plane, one rectangle light and a procedural sky; 1920x1080,
+    4 spp, depth 3."""
+    g = np.random.default_rng(seed)
+    pos = g.normal(size=(birds, 3)) * np.array([40.0, 3.0, 6.0]) + np.array([0.0, 22.0, 0.0])
+    scales = g.uniform(0.9, 1.1, size=birds)
+    t = 0.0
+    for _ in range(frame):
+        pos = pos + curl_velocity(pos, t) * dt
+        t += dt
+    vel = curl_velocity(pos, t)
+    inst = flock_instances(pos, vel, scales, seed)
+    ground = grid((600.0, 600.0), (4, 4))
+    b = bird(seed)
+    ground_inst = np.zeros(1, abi.INSTANCE_DTYPE)
+    ground_inst["transform"] = np.asarray(translate(0, 0, 0), np.float32)
+    ground_inst["blas"] = 0
+    ground_inst["material"] = 2
+    ground_inst["mask"] = 0xFF
+    ground_inst["custom_index"] = 0xFFFFFF
+    mats = [dict(base_color=(0.55, 0.5, 0.45), base_metalness=0.0, specular_weight=1.0, specular_roughness=0.6),
+            dict(base_color=(0.2, 0.22, 0.25), base_metalness=0.0, specular_weight=1.0, specular_roughness=0.35),
+            dict(base_color=(0.35, 0.4, 0.3), base_metalness=0.0, specular_weight=1.0, specular_roughness=0.9)]
+    e = 40.0 * math.pi * 400.0
+    lights = [dict(position=(-10.0, 80.0, 20.0), u=(20.0, 0.0, 0.0), v=(0.0, 0.0, 20.0), normal=(0.0, -1.0, 0.0),
+                   radiance=(e, e * 0.95, e * 0.9))]
+    camera = dict(eye=(0.0, 26.0, 130.0), target=(0.0, 21.0, 0.0))
+    sky = dict(kind=abi.VH_SKY_PROCEDURAL, color=(0.45, 0.5, 0.6), zenith=(0.12, 0.25, 0.6), ground=(0.08, 0.07, 0.06),
+               sun_direction=(0.3, 0.6, 0.25), sun_radiance=(6.0, 5.5, 4.5), sun_exponent=48.0)
+    return Scene([ground, b], np.concatenate([ground_inst, inst]), mats, lights, camera, sky, width, height, spp,
+                 max_depth, seed, "config2", notes=dict(frame=frame, birds=birds))
