@@ -30,6 +30,9 @@ sys.path.insert(0, ROOT)
 
 S_NODE, S_TRI, S_RAYIN, S_HITOUT = 80, 36, 32, 16   # bytes: wide node, triangle record, ray in, hit out
 HBM_PEAK_GBPS = 8000.0
+# HBM-side bytes per k_extend launch of the default config-1 step, measured
+# with rocprofv3 FETCH_SIZE / WRITE_SIZE and scaled by a plain copy kernel
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_extend_config1_1080p_64spp.traffic.json")
 
 
 def log(*a):
@@ -189,6 +192,13 @@ def main():
                     "triangle_tests_per_ray": round(cnt["extend_triangle_tests"] / max(cnt["extension_rays"], 1), 2),
                     "stage_ms": {"extend": round(prof["extend_ms"], 1), "shade": round(prof["shade_ms"], 1),
                                  "shadow": round(prof["shadow_ms"], 1), "other": round(prof["other_ms"], 1)}}
+
+    default_c1 = (args.workload == "config1" and (args.width, args.height, sc.spp, args.sphere_level, args.terrain)
+                  == (1920, 1080, 64, 9, 1024))
+    if roofline is not None and default_c1 and os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            roofline["traffic"] = json.load(f)["traffic_per_launch"]
+        roofline["traffic_source"] = "rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, copy-calibrated (profiles/)"
 
     cpu = None
     if not args.no_cpu_baseline and args.workload == "config1":
