@@ -195,7 +195,8 @@ typedef struct vh_material {
 } vh_material;
 
 /* Rectangle area light (kind 3 of emitter.slang:75-170): centre, full edge
- * vectors u and v, emitting side `normal`, and `radiance` = colour x energy,
+ * vectors u and v, emitting side `normal` (perpendicular to u and v, else
+ * vh_scene_create fails with VH_ERR_SCENE), and `radiance` = colour x energy,
  * packed as crates/vork-render/src/lights.rs:55-140 packs it. */
 typedef struct vh_rect_light {
     float position[3];

@@ -116,3 +116,23 @@ def test_instanced_flock_matches_oracle(renderer):
     o2 = oracle.OracleScene(sc2).render(p)[1]
     assert rel_mse(g2[..., :3] / 4, o2[..., :3] / 4) < 0.05 * noise
     assert not np.array_equal(g2, g)
+
+
+def test_scene_json_scenes_render_like_the_oracle(renderer, tmp_path):
+    """Scenes read by the scene.json loader: the reference's analytic fixture
+    (perspective camera, rectangle key light) and config 0 written and read
+    back."""
+    import os
+    from vork_amd import scene_json as J
+    from tests.test_scene_json import GOLDEN, analytic_perspective
+    sc, _ = J.load(analytic_perspective(), J.parse_set(["samples=8", "seed=3"]), base_dir=GOLDEN)
+    inst, ora, g, o32, o64, p = _render_pair(renderer, sc, 8, seed=3)
+    noise = rel_mse(o64[..., :3] / 8, ora.render(sc.render_params(samples=8, seed=44))[1][..., :3] / 8)
+    assert rel_mse(g[..., :3] / 8, o64[..., :3] / 8) < 0.05 * noise
+    assert g[..., :3].mean() > 0.01
+    path = os.path.join(tmp_path, "c0.json")
+    J.dump(S.config0(width=48, height=48), path)
+    sc0, _ = J.load(path, {"samples": 8, "seed": 1})
+    inst, ora, g, o32, o64, p = _render_pair(renderer, sc0, 8)
+    noise = rel_mse(o64[..., :3] / 8, ora.render(sc0.render_params(samples=8, seed=45))[1][..., :3] / 8)
+    assert rel_mse(g[..., :3] / 8, o64[..., :3] / 8) < 0.05 * noise

@@ -391,6 +391,23 @@ int vh_scene_create(vh_device* dev, const vh_tlas* tlas, const vh_scene_desc* de
     if (desc->material_count == 0 || !desc->materials) return fail(VH_ERR_SCENE, "vh_scene_create: no materials");
     if (desc->light_count && !desc->lights) return fail(VH_ERR_USAGE, "vh_scene_create: null lights");
     if (tlas->device != dev) return fail(VH_ERR_USAGE, "vh_scene_create: TLAS of another device");
+    for (uint32_t i = 0; i < desc->light_count; i++) {
+        // the reference's light record: sides along the node's X / Y, normal -Z
+        // (crates/vork-render/src/lights.rs:101-115), so the normal is
+        // perpendicular to both sides
+        const vh_rect_light& l = desc->lights[i];
+        double uu = 0, vv = 0, nn = 0, nu = 0, nv = 0;
+        bool finite = true;
+        for (int k = 0; k < 3; k++) {
+            uu += (double)l.u[k] * l.u[k]; vv += (double)l.v[k] * l.v[k]; nn += (double)l.normal[k] * l.normal[k];
+            nu += (double)l.normal[k] * l.u[k]; nv += (double)l.normal[k] * l.v[k];
+            finite = finite && std::isfinite(l.position[k]) && std::isfinite(l.u[k]) && std::isfinite(l.v[k]) &&
+                     std::isfinite(l.normal[k]) && std::isfinite(l.radiance[k]);
+        }
+        if (!finite || !(nn > 0) || std::fabs(nu) > 1e-4 * std::sqrt(nn * uu) || std::fabs(nv) > 1e-4 * std::sqrt(nn * vv))
+            return fail(VH_ERR_SCENE, "vh_scene_create: light " + std::to_string(i) +
+                                          " is not finite or its normal is not perpendicular to its sides");
+    }
     for (uint32_t i = 0; i < desc->material_count; i++) {
         const vh_material& m = desc->materials[i];
         bool ok = std::isfinite(m.base_metalness) && std::isfinite(m.specular_roughness) &&

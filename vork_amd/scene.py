@@ -112,19 +112,23 @@ class Scene:
         c = abi.Camera()
         cam = self.camera
         eye = np.asarray(cam["eye"], np.float64)
-        target = np.asarray(cam["target"], np.float64)
-        up = np.asarray(cam.get("up", (0, 1, 0)), np.float64)
-        f = target - eye
+        if "target" in cam:
+            f = np.asarray(cam["target"], np.float64) - eye
+            up = np.asarray(cam.get("up", (0, 1, 0)), np.float64)
+        else:  # an explicit frame: forward and the camera's up axis
+            f = np.asarray(cam["forward"], np.float64)
+            up = np.asarray(cam["up"], np.float64)
         f /= np.linalg.norm(f)
         r = np.cross(f, up)
         r /= np.linalg.norm(r)
         u = np.cross(r, f)
         aspect = self.width / self.height
-        # libraries/std/camera.slang:25-33: fit Auto spans the sensor width
-        # over the longer image side (36 mm sensor, lens in mm).
+        # libraries/std/camera.slang:25-33: the sensor side named by `fit`
+        # spans `span` (Auto: the longer image side), lens in mm.
         lens = cam.get("lens", 18.0 / math.tan(math.radians(22.5)))
-        sensor = cam.get("sensor_width", 36.0)
-        span = aspect if aspect > 1 else 1.0
+        fit = cam.get("fit", "auto")
+        span = aspect if fit == "horizontal" else 1.0 if fit == "vertical" else (aspect if aspect > 1 else 1.0)
+        sensor = cam.get("sensor_height", 24.0) if fit == "vertical" else cam.get("sensor_width", 36.0)
         height = sensor / span
         screen_x = 2 * lens / (height * aspect)
         screen_y = 2 * lens / height
@@ -389,8 +393,9 @@ def config1(seed=1, width=1920, height=1080, spp=64, max_depth=8, sphere_level=9
     e1, e2 = 12.0 * math.pi * 4.0, 8.0 * math.pi * 2.0
     lights = [dict(position=(4.0, 9.0, 3.0), u=(2.0, 0.0, 0.0), v=(0.0, 0.0, 2.0), normal=(0.0, -1.0, 0.0),
                    radiance=(e1, e1 * 0.9, e1 * 0.8)),
-              dict(position=(-7.0, 5.0, -2.0), u=(0.0, 2.0, 0.0), v=(0.0, 0.0, 1.0), normal=(1.0, -0.2, 0.0),
-                   radiance=(e2 * 0.7, e2 * 0.8, e2))]
+              # tilted 11.3 degrees down; the sides stay perpendicular to the normal
+              dict(position=(-7.0, 5.0, -2.0), u=(2.0 * 0.2 / math.sqrt(1.04), 2.0 / math.sqrt(1.04), 0.0),
+                   v=(0.0, 0.0, 1.0), normal=(1.0, -0.2, 0.0), radiance=(e2 * 0.7, e2 * 0.8, e2))]
     camera = dict(eye=(0.0, 4.5, 11.0), target=(0.0, 2.0, 0.0))
     sky = dict(kind=abi.VH_SKY_PROCEDURAL, color=(0.45, 0.5, 0.6), zenith=(0.12, 0.25, 0.6), ground=(0.08, 0.07, 0.06),
                sun_direction=(0.3, 0.6, 0.25), sun_radiance=(6.0, 5.5, 4.5), sun_exponent=48.0)
