@@ -136,3 +136,26 @@ def test_scene_json_scenes_render_like_the_oracle(renderer, tmp_path):
     inst, ora, g, o32, o64, p = _render_pair(renderer, sc0, 8)
     noise = rel_mse(o64[..., :3] / 8, ora.render(sc0.render_params(samples=8, seed=45))[1][..., :3] / 8)
     assert rel_mse(g[..., :3] / 8, o64[..., :3] / 8) < 0.05 * noise
+
+
+def test_render_call_writes_png_exr_and_meta(tmp_path):
+    """`python -m vork_amd.render SCENE --out FILE --set ...` end to end: the
+    EXR holds the film mean, the PNG its standard-view codes."""
+    import json
+    import os
+    from vork_amd import film as F, render as R, scene_json as J
+    scene = os.path.join(tmp_path, "c0.json")
+    J.dump(S.config0(width=32, height=32), scene)
+    exr = os.path.join(tmp_path, "c0.exr")
+    png = os.path.join(tmp_path, "c0.png")
+    assert R.main([scene, "--out", exr, "--set", "samples=4", "--set", "seed=2"]) == 0
+    assert R.main([scene, "--out", png, "--view", "standard", "--set", "samples=4", "--set", "seed=2"]) == 0
+    ch = F.read_exr(exr)
+    rgb = np.stack([ch["R"], ch["G"], ch["B"]], -1)
+    assert rgb.shape == (32, 32, 3) and np.isfinite(rgb).all() and rgb.mean() > 0
+    codes = F.read_png(png)
+    assert np.array_equal(codes, F.quantize(F.view_standard(rgb.astype(np.float64))))
+    meta = json.load(open(png + ".meta.json"))
+    assert meta["version"] == 3 and meta["render_settings"]["samples"] == 4
+    assert meta["render_settings"]["resolution"] == [32, 32]
+    assert R.main([scene, "--out", png, "--set", "bogus=1"]) == 2

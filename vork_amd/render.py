@@ -1,0 +1,69 @@
+"""`python -m vork_amd.render SCENE --out FILE [--set KEY=VALUE]...`: the
+render call of the std/path hot path on one GPU (the subset of
+`vork render SCENE --out FILE --set ...`, crates/vork-cli/src/main.rs:103-171).
+
+Loads scene.json (vork_amd.scene_json), installs it on the device (BLASes,
+TLAS, materials, lights), renders samples [0, samples) into an RGBA32F film
+and writes FILE (.png display codes through the agx or standard view, .exr
+scene-linear mean) plus FILE.meta.json.  Exit codes follow the reference's
+failure kinds: 2 usage, 3 scene, 4 device, 5 io.
+"""
+import argparse
+import os
+import sys
+import time
+
+
+def render_scene(scene, out, view="agx", agx_lut=None, warnings=()):
+    """Renders a loaded Scene on device 0 and writes `out` (+ .meta.json).
+    Returns the film (H, W, 4) and the render stats."""
+    from . import film as F
+    from . import native
+    t0 = time.perf_counter()
+    r = native.Renderer(0)
+    inst = r.install(scene)
+    p = scene.render_params()
+    img = inst.render(p)
+    stats = inst.stats()
+    wall = (time.perf_counter() - t0) * 1e3
+    warnings = list(warnings)
+    lut = F.load_agx_lut(agx_lut) if agx_lut else None
+    if view == "agx" and lut is None and out.lower().endswith(".png"):
+        warnings.append("view agx: analytic AgX (no AgX LUT given); codes differ from the baked view")
+    meta = F.metadata(scene, stats, warnings, p.samples, wall)
+    F.write_outputs(out, img, view=view, lut=lut, meta=meta)
+    return img, stats
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(prog="python -m vork_amd.render")
+    ap.add_argument("scene")
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--set", action="append", default=[], metavar="KEY=VALUE")
+    ap.add_argument("--view", choices=["agx", "standard"], default="agx")
+    ap.add_argument("--agx-lut", default=os.environ.get("VORK_AGX_LUT"),
+                    help="vork's libraries/std/agx.exr for the exact AgX view")
+    args = ap.parse_args(argv)
+    from . import scene_json as J
+    from .native import VorkError
+    try:
+        scene, warnings = J.load(args.scene, J.parse_set(args.set))
+    except J.SceneError as e:
+        print(f"vork_amd.render: scene: {e}", file=sys.stderr)
+        return 2 if str(e).startswith("--set") else 3
+    except OSError as e:
+        print(f"vork_amd.render: io: {e}", file=sys.stderr)
+        return 5
+    try:
+        render_scene(scene, args.out, args.view, args.agx_lut, warnings)
+    except VorkError as e:
+        print(f"vork_amd.render: device: {e}", file=sys.stderr)
+        return e.status if e.status in (2, 3, 4, 5) else 4
+    except OSError as e:
+        print(f"vork_amd.render: io: {e}", file=sys.stderr)
+        return 5
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
