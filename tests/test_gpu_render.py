@@ -153,8 +153,10 @@ def test_render_call_writes_png_exr_and_meta(tmp_path):
     ch = F.read_exr(exr)
     rgb = np.stack([ch["R"], ch["G"], ch["B"]], -1)
     assert rgb.shape == (32, 32, 3) and np.isfinite(rgb).all() and rgb.mean() > 0
-    codes = F.read_png(png)
-    assert np.array_equal(codes, F.quantize(F.view_standard(rgb.astype(np.float64))))
+    codes = F.read_png(png).astype(np.int32)
+    want = F.quantize(F.view_standard(rgb.astype(np.float64))).astype(np.int32)
+    # the EXR holds the mean rounded to float32: a code may sit one step away
+    assert np.abs(codes - want).max() <= 1 and (codes == want).mean() > 0.99
     meta = json.load(open(png + ".meta.json"))
     assert meta["version"] == 3 and meta["render_settings"]["samples"] == 4
     assert meta["render_settings"]["resolution"] == [32, 32]
