@@ -123,7 +123,7 @@ def main():
     tlas_ms = []
 
     def step(flags=0):
-        p = sc.render_params()
+        p = sc.render_params(adaptive=False)
         p.batch_samples = args.batch
         p.flags = flags
         if gpu_frames is None:
@@ -222,7 +222,8 @@ def main():
         "dtype": "f32", "data": "synthetic (seeded procedural scene, generated on the host, uploaded before timing)",
         "config": {"workload": workload,
                    "width": sc.width, "height": sc.height, "spp": sc.spp, "max_depth": sc.max_depth,
-                   "triangles": sc.triangle_count, "batch_samples": args.batch or "auto"},
+                   "triangles": sc.triangle_count, "batch_samples": args.batch or "auto",
+                   "sampling": "fixed spp (adaptive stopping off: every pixel traces every sample)"},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 1), "p99": round(float(np.percentile(lat, 99)), 1)},
         "blas_build_ms": [round(b["build_ms"], 2) for b in binfo],
         "blas_nodes": [b["node_count"] for b in binfo],

@@ -277,17 +277,31 @@ typedef struct vh_render_params {
     float light_threshold;
     uint32_t batch_samples;  /* exposure samples per wavefront batch, 0 = auto */
     uint32_t flags;          /* VH_RENDER_* */
-    uint32_t reserved;
+    uint32_t adaptive_min_samples;  /* sampling.w: 0 = max(16, sqrt(samples)) */
+    float adaptive_threshold;       /* adaptive.x: 0 = 0.01 */
+    uint32_t reserved[2];
 } vh_render_params;
 
 /* vh_render_params.flags: record HIP events around every stage launch (their
  * totals land in vh_render_stats), and count the nodes and triangles the
- * closest-hit stage visits (slower; for stating algorithmic bytes). */
+ * closest-hit stage visits (slower; for stating algorithmic bytes).
+ *
+ * VH_RENDER_ADAPTIVE is the adaptive path_pixel entry (path.slang:1792-1838,
+ * 1964-1966; on by default, constants.rs:264-266): per-pixel running mean and
+ * variance (48 B per pixel, kept in the scene between calls); from sample
+ * index `minimum` a pixel is settled when every RGBA channel's standard error
+ * is at most threshold * max(|mean|, 0.01), and it stops for the rest of the
+ * exposure once it and its whole 3x3 neighbourhood were settled on the
+ * previous sample.  A stopped pixel adds its mean for each remaining sample,
+ * so the film still holds the sum of `samples` values.  A call with
+ * sample_begin > 0 must continue the exposure the previous adaptive call on
+ * this scene rendered (same size, seed and sample count), else VH_ERR_USAGE. */
 #define VH_RENDER_PROFILE 1u
 #define VH_RENDER_COUNT_VISITS 2u
+#define VH_RENDER_ADAPTIVE 4u
 
 /* Fills `params` with the reference's production defaults for a
- * width x height x samples render. */
+ * width x height x samples render (adaptive sampling on). */
 void vh_render_params_default(vh_render_params* params, uint32_t width, uint32_t height, uint32_t samples);
 
 /* Renders exposure samples [sample_begin, sample_end) of every pixel and ADDS
@@ -300,7 +314,7 @@ int vh_render(vh_scene* scene, const vh_render_params* params, uint32_t flags, v
 
 /* Kernel statistics of the last vh_render on this scene. */
 typedef struct vh_render_stats {
-    uint64_t paths;
+    uint64_t paths;             /* traced paths: stopped adaptive pixels trace none */
     uint64_t extension_rays;
     uint64_t shadow_rays;
     uint64_t dropped_samples;   /* non-finite samples zeroed */
@@ -311,6 +325,8 @@ typedef struct vh_render_stats {
     /* with VH_RENDER_COUNT_VISITS: wide nodes and triangles the closest-hit stage visited */
     uint64_t extend_node_visits, extend_triangle_tests;
     uint64_t shadow_node_visits, shadow_triangle_tests;
+    /* with VH_RENDER_ADAPTIVE: samples of stopped pixels (their mean added, no path) */
+    uint64_t stopped_samples;
 } vh_render_stats;
 int vh_render_get_stats(const vh_scene* scene, vh_render_stats* stats);
 

@@ -1512,8 +1512,10 @@ static v3 next_event(const vo_scene* s, const vh_render_params* p, const closure
     return value;
 }
 
+/* One sample's RGBA value: (radiance, 1), or zero when non-finite
+ * (path.slang:1948-1952). */
 static v3 sample_radiance(const vo_scene* s, const vh_render_params* p, uint32_t px, uint32_t py, uint32_t index,
-                          counters_t* cnt) {
+                          counters_t* cnt, float* alpha) {
     sampler_t smp = make_sampler(px, py, index, p->samples, p->seed);
     uint32_t rng = vo_screen_ray_hash(px, py, index, p->seed);
     float cs[4];
@@ -1569,13 +1571,15 @@ static v3 sample_radiance(const vo_scene* s, const vh_render_params* p, uint32_t
     }
     cnt->dropped += dropped;
     if (!finite3(radiance)) nonfinite = 1;
+    *alpha = nonfinite ? 0.f : 1.f;
     return nonfinite ? mk(0.f, 0.f, 0.f) : radiance;
 }
 
 void vo_sample_radiance(const vo_scene* s, const vh_render_params* p, uint32_t px, uint32_t py, uint32_t index, float out[4]) {
     counters_t cnt = {0, 0, 0};
-    v3 r = sample_radiance(s, p, px, py, index, &cnt);
-    out[0] = r.x; out[1] = r.y; out[2] = r.z; out[3] = 1.f;
+    float alpha;
+    v3 r = sample_radiance(s, p, px, py, index, &cnt, &alpha);
+    out[0] = r.x; out[1] = r.y; out[2] = r.z; out[3] = alpha;
 }
 
 void vo_render(const vo_scene* s, const vh_render_params* p, float* film32, double* film64, int threads) {
@@ -1597,9 +1601,10 @@ void vo_render(const vo_scene* s, const vh_render_params* p, float* film32, doub
         double acc64[4] = {0, 0, 0, 0};
         if (film32) memcpy(acc, film32 + 4 * i, sizeof(acc));
         for (uint32_t k = p->sample_begin; k < p->sample_end; k++) {
-            v3 r = sample_radiance(s, p, px, py, k, &cnt);
-            if (film32) { acc[0] += r.x; acc[1] += r.y; acc[2] += r.z; acc[3] += 1.f; }
-            acc64[0] += r.x; acc64[1] += r.y; acc64[2] += r.z; acc64[3] += 1.0;
+            float alpha;
+            v3 r = sample_radiance(s, p, px, py, k, &cnt, &alpha);
+            if (film32) { acc[0] += r.x; acc[1] += r.y; acc[2] += r.z; acc[3] += alpha; }
+            acc64[0] += r.x; acc64[1] += r.y; acc64[2] += r.z; acc64[3] += alpha;
         }
         if (film32) memcpy(film32 + 4 * i, acc, sizeof(acc));
         if (film64)

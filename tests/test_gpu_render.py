@@ -161,3 +161,74 @@ def test_render_call_writes_png_exr_and_meta(tmp_path):
     assert meta["version"] == 3 and meta["render_settings"]["samples"] == 4
     assert meta["render_settings"]["resolution"] == [32, 32]
     assert R.main([scene, "--out", png, "--set", "bogus=1"]) == 2
+
+
+def _per_sample_values(inst, sc, samples, seed):
+    """Each sample index rendered alone into a zero film: its RGBA value."""
+    out = []
+    for k in range(samples):
+        p = sc.render_params(samples=samples, seed=seed, adaptive=False)
+        p.sample_begin, p.sample_end = k, k + 1
+        out.append(inst.render(p))
+    return np.stack(out)
+
+
+def test_adaptive_film_equals_the_restatement_over_the_gpu_samples(renderer):
+    from oracle import adaptive
+    sc = S.config0(width=48, height=40)
+    inst = renderer.install(sc)
+    spp, seed = 40, 4
+    values = _per_sample_values(inst, sc, spp, seed)
+    for threshold in (0.05, 0.2):
+        p = sc.render_params(samples=spp, seed=seed)
+        p.adaptive_threshold = threshold
+        g = inst.render(p)
+        film, traced, _, _ = adaptive.render(values, adaptive.default_minimum(spp), threshold)
+        st = inst.stats()
+        assert 0 < traced[-1].sum() < traced[0].sum(), traced.sum(axis=(1, 2))
+        assert st["paths"] == traced.sum()
+        assert st["stopped_samples"] == traced.size - traced.sum()
+        assert np.array_equal(g, film)
+    # ranges continue the exposure: [0, 25) then [25, 40) == [0, 40)
+    p1 = sc.render_params(samples=spp, seed=seed)
+    p1.adaptive_threshold, p1.sample_end = 0.05, 25
+    p2 = sc.render_params(samples=spp, seed=seed)
+    p2.adaptive_threshold, p2.sample_begin = 0.05, 25
+    whole = sc.render_params(samples=spp, seed=seed)
+    whole.adaptive_threshold = 0.05
+    a = inst.render(whole)
+    f = inst.render(p2, film=inst.render(p1))
+    assert np.array_equal(f, a)
+
+
+def test_adaptive_at_a_tiny_threshold_is_the_plain_render(renderer):
+    # only pixels whose samples are all equal (the sky) can settle; they add
+    # their exact mean, so the film is the plain one
+    from oracle import adaptive
+    sc = S.config0(width=40, height=32)
+    inst = renderer.install(sc)
+    plain = inst.render(sc.render_params(samples=24, seed=9, adaptive=False))
+    p = sc.render_params(samples=24, seed=9)
+    p.adaptive_threshold, p.adaptive_min_samples = 1e-30, 4
+    assert np.array_equal(inst.render(p), plain)
+    stopped = inst.stats()["stopped_samples"]
+    _, traced, _, _ = adaptive.render(_per_sample_values(inst, sc, 24, 9), 4, 1e-30)
+    assert stopped == traced.size - traced.sum()
+
+
+def test_adaptive_range_must_continue_its_exposure(renderer):
+    from vork_amd.native import VorkError
+    sc = S.config0(width=16, height=16)
+    inst = renderer.install(sc)
+    p = sc.render_params(samples=32, seed=2)
+    p.sample_begin = 20
+    with pytest.raises(VorkError) as e:
+        inst.render(p)
+    assert e.value.status == 2
+    q = sc.render_params(samples=32, seed=2)
+    q.sample_end = 20
+    inst.render(q)
+    r = sc.render_params(samples=32, seed=3)  # another seed is another exposure
+    r.sample_begin = 20
+    with pytest.raises(VorkError):
+        inst.render(r)

@@ -83,7 +83,8 @@ class RenderParams(C.Structure):
                 ("max_bounces", C.c_uint32), ("diffuse_bounces", C.c_uint32), ("glossy_bounces", C.c_uint32),
                 ("caustics", C.c_uint32), ("clamp_direct", C.c_float), ("clamp_indirect", C.c_float),
                 ("light_threshold", C.c_float), ("batch_samples", C.c_uint32), ("flags", C.c_uint32),
-                ("reserved", C.c_uint32)]
+                ("adaptive_min_samples", C.c_uint32), ("adaptive_threshold", C.c_float),
+                ("reserved", C.c_uint32 * 2)]
 
 
 class RenderStats(C.Structure):
@@ -93,16 +94,17 @@ class RenderStats(C.Structure):
                 ("other_ms", C.c_double), ("extend_launches", C.c_uint64), ("shade_launches", C.c_uint64),
                 ("shadow_launches", C.c_uint64), ("extend_node_visits", C.c_uint64),
                 ("extend_triangle_tests", C.c_uint64), ("shadow_node_visits", C.c_uint64),
-                ("shadow_triangle_tests", C.c_uint64)]
+                ("shadow_triangle_tests", C.c_uint64), ("stopped_samples", C.c_uint64)]
 
 
-VH_RENDER_PROFILE, VH_RENDER_COUNT_VISITS = 1, 2
+VH_RENDER_PROFILE, VH_RENDER_COUNT_VISITS, VH_RENDER_ADAPTIVE = 1, 2, 4
 
 
-def default_params(width, height, samples, seed=1, max_bounces=12):
+def default_params(width, height, samples, seed=1, max_bounces=12, adaptive=True):
     """Production transport policy (crates/vork-render/src/constants.rs:256-267,
     crates/vork-render/src/passes/path.rs:577-585): bounces 12/4/4, caustics on,
-    direct clamp off, indirect clamp 10, light threshold 0.01."""
+    direct clamp off, indirect clamp 10, light threshold 0.01, adaptive
+    sampling on (threshold 0.01, minimum max(16, sqrt(samples)))."""
     p = RenderParams()
     p.width, p.height, p.samples = width, height, samples
     p.sample_begin, p.sample_end, p.seed = 0, samples, seed
@@ -110,12 +112,13 @@ def default_params(width, height, samples, seed=1, max_bounces=12):
     p.caustics = 3
     p.clamp_direct, p.clamp_indirect, p.light_threshold = 0.0, 10.0, 0.01
     p.batch_samples = 0
+    p.flags = VH_RENDER_ADAPTIVE if adaptive else 0
     return p
 
 
 def check_layout():
     sizes = {Geometry: 48, BlasInfo: 40, Instance: 64, Ray: 32, Hit: 24, Material: 32, RectLight: 80,
-             Camera: 64, Sky: 80, RenderParams: 64, RenderStats: 128}
+             Camera: 64, Sky: 80, RenderParams: 76, RenderStats: 136}
     for t, n in sizes.items():
         if C.sizeof(t) != n:
             raise AssertionError(f"{t.__name__} is {C.sizeof(t)} bytes, header layout says {n}")

@@ -500,6 +500,10 @@ void vh_render_params_default(vh_render_params* p, uint32_t width, uint32_t heig
     p->clamp_indirect = 10.f;
     p->light_threshold = 0.01f;
     p->batch_samples = 0;
+    // constants.rs:264-266: adaptive sampling on, threshold 0.01, minimum from the sample count
+    p->flags = VH_RENDER_ADAPTIVE;
+    p->adaptive_min_samples = 0;
+    p->adaptive_threshold = 0.f;
 }
 
 int vh_render(vh_scene* sc, const vh_render_params* p, uint32_t flags, void* stream, float* film) {
@@ -509,6 +513,8 @@ int vh_render(vh_scene* sc, const vh_render_params* p, uint32_t flags, void* str
     if (p->samples == 0 || p->sample_end > p->samples || p->sample_begin > p->sample_end)
         return fail(VH_ERR_USAGE, "vh_render: sample range outside the exposure");
     if ((uint64_t)p->width * p->height > (1ull << 31)) return fail(VH_ERR_USAGE, "vh_render: image too large");
+    if (!(p->adaptive_threshold >= 0.f) || !std::isfinite(p->adaptive_threshold))
+        return fail(VH_ERR_USAGE, "vh_render: adaptive threshold must be finite and >= 0");
     vh_device* dev = sc->device;
     VH_HIP(hipSetDevice(dev->ordinal));
     hipStream_t s = pick(dev, stream);

@@ -101,6 +101,10 @@ struct vh_scene {
     // wavefront state, grown on demand
     vh::DeviceBuffer paths, rays_a, rays_b, shadow_rays, pending, counters;
     size_t capacity = 0;
+    // adaptive sampling moments (48 B per pixel) and the exposure they belong
+    // to: size, seed, sample count and the next sample index
+    vh::DeviceBuffer moments;
+    uint32_t moments_width = 0, moments_height = 0, moments_seed = 0, moments_samples = 0, moments_next = 0;
     vh_render_stats stats{};
 };
 

@@ -12,7 +12,16 @@
 //              RGBA32F film (one thread per pixel, no atomics)
 //
 // Every path owns its state slot, so results do not depend on queue order.
+//
+// Adaptive sampling (path.slang:1792-1838,1964-1966) keeps a moment per
+// pixel.  No pixel can stop before sample index minimum + 1 (it needs its own
+// test and its neighbourhood's of the previous sample, both from `minimum`
+// on), so samples up to `minimum` are batched like plain ones and the film
+// pass records their tests; each later sample index is a batch of its own,
+// preceded by the decide pass, and its camera pass emits paths only for the
+// pixels that still run.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -57,6 +66,8 @@ struct Consts {
     float clamp_direct, clamp_indirect, light_threshold;
     uint32_t light_count, light_nodes, material_count;
     uint32_t batch_first, batch_samples, pixels;
+    uint32_t minimum, decide;   // adaptive: first tested sample index; batch follows a decide pass
+    float threshold;
     vh_camera camera;
     vh_sky sky;
 };
@@ -71,6 +82,37 @@ __device__ __forceinline__ uint32_t slot_to_pixel(uint32_t t, uint32_t w, uint32
     return y * w + x;
 }
 
+// Per-pixel adaptive moment (PathMoment, path.slang:57): running mean and
+// variance sum of the RGBA sample values, the settled test of the last two
+// sample parities, and the stopped flag.
+struct Moment {
+    float4 mean, variance;
+    uint32_t settled[2], stopped, pad;
+};
+static_assert(sizeof(Moment) == 48, "moment layout");
+
+// path.slang:1812-1816: every channel's standard error within
+// threshold * max(|mean|, 0.01).  Only called with index >= 1.
+__device__ __forceinline__ bool moment_settled(const Moment& m, uint32_t index, float threshold) {
+    const float a = (float)max(index - 1u, 1u), b = (float)index;
+    const float v[4] = {m.variance.x, m.variance.y, m.variance.z, m.variance.w};
+    const float mu[4] = {m.mean.x, m.mean.y, m.mean.z, m.mean.w};
+    bool ok = true;
+    for (int k = 0; k < 4; k++) ok = ok && sqrtf(fmaxf(v[k] / a / b, 0.f)) <= threshold * fmaxf(fabsf(mu[k]), .01f);
+    return ok;
+}
+
+// path.slang:1964: Welford update with sample `index`, each operation rounded
+// as written.
+__device__ __forceinline__ void moment_add(Moment& m, float4 v, uint32_t index) {
+#pragma clang fp contract(off)
+    const float n = (float)(index + 1u);
+    float4 d = make_float4(v.x - m.mean.x, v.y - m.mean.y, v.z - m.mean.z, v.w - m.mean.w);
+    m.mean = make_float4(m.mean.x + d.x / n, m.mean.y + d.y / n, m.mean.z + d.z / n, m.mean.w + d.w / n);
+    m.variance = make_float4(m.variance.x + d.x * (v.x - m.mean.x), m.variance.y + d.y * (v.y - m.mean.y),
+                             m.variance.z + d.z * (v.z - m.mean.z), m.variance.w + d.w * (v.w - m.mean.w));
+}
+
 struct Scene {
     TlasView tv;
     const vh_material* materials;
@@ -83,18 +125,58 @@ struct Scene {
 // counters: [0] live count in, [1] live count out, [2] shadow rays queued,
 // [3] dropped non-finite contributions, [4..5] extension rays (u64),
 // [6..7] shadow rays (u64), [8] errors, [9] non-finite samples zeroed,
-// [12] paths with a pending next event, [16..19] extension node / triangle
-// visits, [20..23] shadow node / triangle visits (u64), [24] extension ray
-// claims, [25] shadow ray claims
+// [10..11] samples of stopped adaptive pixels (u64), [12] paths with a
+// pending next event, [16..19] extension node / triangle visits, [20..23]
+// shadow node / triangle visits (u64), [24] extension ray claims, [25] shadow
+// ray claims
 constexpr int kExtendClaims = 24, kShadowClaims = 25;
 
 __device__ __forceinline__ void reset_claims(uint32_t* counters) {
     counters[kExtendClaims] = 0;
     counters[kShadowClaims] = 0;
 }
-__global__ void k_camera(Paths P, Consts c, Scene S, uint32_t n, float4* rays, uint32_t* counters) {
+// Opens an adaptive batch: its camera pass appends to an empty queue.
+__global__ void k_open_batch(uint32_t* counters) {
+    if (threadIdx.x == 0) {
+        counters[0] = 0;
+        counters[1] = 0;
+        counters[2] = 0;
+        counters[12] = 0;
+        reset_claims(counters);
+    }
+}
+
+// Adaptive stop decision of sample `index` > minimum (path.slang:1804-1836):
+// publish this sample's test in its parity slot, then stop if it and the
+// whole clamped 3x3 neighbourhood's test of the previous sample passed.  The
+// neighbours' slot is one no thread writes in this pass.
+__global__ void k_decide(Moment* M, Consts c, uint32_t index) {
+    const uint32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pixel >= c.pixels) return;
+    Moment& m = M[pixel];
+    if (m.stopped) return;
+    const uint32_t slot = index & 1u;
+    const bool settled = moment_settled(m, index, c.threshold);
+    m.settled[slot] = settled ? 1u : 0u;
+    bool done = settled;
+    const int x = (int)(pixel % c.width), y = (int)(pixel / c.width);
+    const int w = (int)c.width, h = (int)c.height;
+    for (int dy = -1; dy <= 1 && done; dy++) {
+        for (int dx = -1; dx <= 1 && done; dx++) {
+            const int nx = min(max(x + dx, 0), w - 1), ny = min(max(y + dy, 0), h - 1);
+            done = M[ny * w + nx].settled[slot ^ 1u] != 0u;
+        }
+    }
+    if (done) m.stopped = 1u;
+}
+
+// Camera rays of the batch's paths.  Plain batches write ray p for path p;
+// adaptive ones append the rays of the pixels that still run.
+template <bool kAdaptive>
+__global__ void k_camera(Paths P, Consts c, Scene S, uint32_t n, float4* rays, uint32_t* counters,
+                         const Moment* M) {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p == 0) {
+    if (!kAdaptive && p == 0) {
         counters[0] = n;
         counters[1] = 0;
         counters[2] = 0;
@@ -103,6 +185,7 @@ __global__ void k_camera(Paths P, Consts c, Scene S, uint32_t n, float4* rays, u
     }
     if (p >= n) return;
     uint32_t s_local = p / c.pixels, pixel = slot_to_pixel(p % c.pixels, c.width, c.height);
+    if (kAdaptive && c.decide && M[pixel].stopped) return;
     uint32_t px = pixel % c.width, py = pixel / c.width;
     uint32_t index = c.batch_first + s_local;
     Sampler smp = make_sampler(px, py, index, c.samples, c.seed);
@@ -118,7 +201,7 @@ __global__ void k_camera(Paths P, Consts c, Scene S, uint32_t n, float4* rays, u
     float near_distance = cam.clip_start / fd, far_distance = cam.clip_end / fd;
     f3 o = eye + d * near_distance;
     float tmax = fmaxf(0.f, far_distance - length(o - eye));
-    put_ray(rays, p, o, tmax, d, p | (VH_RAY_CAMERA << kPathBits));
+    put_ray(rays, kAdaptive ? atomicAdd(&counters[0], 1u) : p, o, tmax, d, p | (VH_RAY_CAMERA << kPathBits));
     P.thr[p] = make_float4(1.f, 1.f, 1.f, 0.f);
     P.rad[p] = make_float4(0.f, 0.f, 0.f, 0.f);
     P.info[p] = make_uint4(pixel, index, screen_ray_hash(px, py, index, c.seed), VH_RAY_CAMERA << kEventShift);
@@ -471,25 +554,55 @@ __global__ void k_resolve(Paths P, const uint32_t* queue, uint32_t* counters) {
 }
 
 // One thread per pixel: adds the batch's samples, in sample order, to the
-// film.  Threads walk the path slots in order (consecutive slots hold an 8x8
-// pixel tile), so the path-state reads are contiguous.
-__global__ void k_film(Paths P, Consts c, float4* film) {
+// film (resolve.slang:228-238).  A sample is (radiance, 1), or zero when it
+// was non-finite (path.slang:1948-1952).  Threads walk the path slots in order
+// (consecutive slots hold an 8x8 pixel tile), so the path-state reads are
+// contiguous.  Adaptive batches also update the pixel's moment, add a stopped
+// pixel's mean in place of a sample, and record the settled test of sample
+// indices up to `minimum` (later ones are the decide pass's).
+template <bool kAdaptive>
+__global__ void k_film(Paths P, Consts c, float4* film, Moment* M, uint32_t* counters) {
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= c.pixels) return;
     const uint32_t pixel = slot_to_pixel(slot, c.width, c.height);
     float4 acc = film[pixel];
+    Moment m;
+    if (kAdaptive) m = M[pixel];
+    uint32_t bad_count = 0, stopped_count = 0;
     for (uint32_t s = 0; s < c.batch_samples; s++) {
-        uint32_t p = s * c.pixels + slot;
-        float4 r = P.rad[p];
-        bool bad = (P.info[p].w & kNonFinite) || !isfinite(r.x) || !isfinite(r.y) || !isfinite(r.z);
-        if (!bad) {
-            acc.x += r.x;
-            acc.y += r.y;
-            acc.z += r.z;
+        const uint32_t index = c.batch_first + s;
+        if (kAdaptive) {
+            if (index == 0) {
+                m.mean = make_float4(0.f, 0.f, 0.f, 0.f);
+                m.variance = m.mean;
+                m.stopped = 0u;
+            }
+            if (c.decide && m.stopped) {
+                acc.x += m.mean.x;
+                acc.y += m.mean.y;
+                acc.z += m.mean.z;
+                acc.w += m.mean.w;
+                stopped_count++;
+                continue;
+            }
+            if (index <= c.minimum)
+                m.settled[index & 1u] = index >= c.minimum && moment_settled(m, index, c.threshold) ? 1u : 0u;
         }
-        acc.w += 1.f;
+        const uint32_t p = s * c.pixels + slot;
+        const float4 r = P.rad[p];
+        const bool bad = (P.info[p].w & kNonFinite) || !isfinite(r.x) || !isfinite(r.y) || !isfinite(r.z);
+        const float4 v = bad ? make_float4(0.f, 0.f, 0.f, 0.f) : make_float4(r.x, r.y, r.z, 1.f);
+        bad_count += bad ? 1u : 0u;
+        acc.x += v.x;
+        acc.y += v.y;
+        acc.z += v.z;
+        acc.w += v.w;
+        if (kAdaptive) moment_add(m, v, index);
     }
     film[pixel] = acc;
+    if (kAdaptive) M[pixel] = m;
+    if (bad_count) atomicAdd(&counters[9], bad_count);
+    if (stopped_count) atomicAdd((unsigned long long*)&counters[10], (unsigned long long)stopped_count);
 }
 
 __global__ void k_swap_counts(uint32_t* counters) {
@@ -500,13 +613,6 @@ __global__ void k_swap_counts(uint32_t* counters) {
         counters[12] = 0;
         reset_claims(counters);
     }
-}
-
-__global__ void k_count_nonfinite(Paths P, uint32_t n, uint32_t* counters) {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    float4 r = P.rad[p];
-    if ((P.info[p].w & kNonFinite) || !isfinite(r.x) || !isfinite(r.y) || !isfinite(r.z)) atomicAdd(&counters[9], 1u);
 }
 
 // ------------------------------------------------------------------ LUTs
@@ -627,6 +733,30 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, hipStrea
         sc->capacity = cap;
     }
     VH_HIP(sc->counters.reserve(64 * sizeof(uint32_t)));
+    const bool adaptive = (rp->flags & VH_RENDER_ADAPTIVE) != 0;
+    // path.slang:1801-1802: minimum max(16, sqrt(samples)) and threshold 0.01
+    // unless set; a minimum at or beyond the exposure means no pixel stops.
+    const uint32_t minimum = std::min<uint32_t>(
+        rp->adaptive_min_samples ? rp->adaptive_min_samples
+                                 : std::max<uint32_t>(16u, (uint32_t)std::sqrt((float)rp->samples)),
+        rp->samples);
+    Moment* moments = nullptr;
+    if (adaptive) {
+        if (rp->sample_begin == 0) {
+            VH_HIP(sc->moments.reserve((size_t)pixels * sizeof(Moment)));
+            sc->moments_width = rp->width;
+            sc->moments_height = rp->height;
+            sc->moments_seed = rp->seed;
+            sc->moments_samples = rp->samples;
+        } else if (sc->moments_next != rp->sample_begin || sc->moments_width != rp->width ||
+                   sc->moments_height != rp->height || sc->moments_seed != rp->seed ||
+                   sc->moments_samples != rp->samples) {
+            return fail(VH_ERR_USAGE, "vh_render: an adaptive sample range must continue the exposure rendered "
+                                      "before it on this scene");
+        }
+        sc->moments_next = UINT32_MAX;  // set again once this range is rendered
+        moments = sc->moments.as<Moment>();
+    }
     Paths P;
     {
         char* base = sc->paths.as<char>();
@@ -658,6 +788,8 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, hipStrea
     c.clamp_direct = rp->clamp_direct; c.clamp_indirect = rp->clamp_indirect; c.light_threshold = rp->light_threshold;
     c.light_count = sc->light_count; c.light_nodes = sc->light_nodes; c.material_count = sc->material_count;
     c.pixels = pixels;
+    c.minimum = minimum;
+    c.threshold = rp->adaptive_threshold > 0.f ? rp->adaptive_threshold : 0.01f;
     c.camera = sc->camera;
     c.sky = sc->sky;
     const uint32_t grid = std::min<uint32_t>(blocks(cap, kBlock), (uint32_t)dev->cu_count * 8u);
@@ -686,8 +818,13 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, hipStrea
     VH_HIP(hipEventCreate(&e0));
     VH_HIP(hipEventCreate(&e1));
     VH_HIP(hipEventRecord(e0, s));
-    for (uint32_t first = rp->sample_begin; first < rp->sample_end; first += batch) {
-        const uint32_t nb = std::min(batch, rp->sample_end - first);
+    for (uint32_t first = rp->sample_begin; first < rp->sample_end;) {
+        uint32_t nb = std::min(batch, rp->sample_end - first);
+        c.decide = 0;
+        if (adaptive) {
+            if (first <= minimum) nb = (uint32_t)std::min<uint64_t>(nb, (uint64_t)minimum + 1u - first);
+            else { nb = 1; c.decide = 1; }
+        }
         c.batch_first = first;
         c.batch_samples = nb;
         const uint32_t n = pixels * nb;
@@ -695,7 +832,15 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, hipStrea
         float4* qb = sc->rays_b.as<float4>();
         float4* sq = sc->shadow_rays.as<float4>();
         uint32_t* pending = sc->pending.as<uint32_t>();
-        hipLaunchKernelGGL(k_camera, dim3(blocks(n, kBlock)), dim3(kBlock), 0, s, P, c, S, n, qa, counters);
+        if (adaptive) {
+            if (c.decide) hipLaunchKernelGGL(k_decide, dim3(blocks(pixels, kBlock)), dim3(kBlock), 0, s, moments, c, first);
+            hipLaunchKernelGGL(k_open_batch, dim3(1), dim3(64), 0, s, counters);
+            hipLaunchKernelGGL(k_camera<true>, dim3(blocks(n, kBlock)), dim3(kBlock), 0, s, P, c, S, n, qa, counters,
+                               moments);
+        } else {
+            hipLaunchKernelGGL(k_camera<false>, dim3(blocks(n, kBlock)), dim3(kBlock), 0, s, P, c, S, n, qa, counters,
+                               moments);
+        }
         for (uint32_t bounce = 0; bounce <= c.max_bounces; bounce++) {
             int rc2 = timed(0, [&] {
                 if (count) hipLaunchKernelGGL(k_extend<true>, dim3(grid), dim3(kBlock), 0, s, P, S, qa, counters, overflow);
@@ -713,10 +858,14 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, hipStrea
             hipLaunchKernelGGL(k_swap_counts, dim3(1), dim3(64), 0, s, counters);
             std::swap(qa, qb);
         }
-        hipLaunchKernelGGL(k_count_nonfinite, dim3(blocks(n, kBlock)), dim3(kBlock), 0, s, P, n, counters);
-        hipLaunchKernelGGL(k_film, dim3(blocks(pixels, kBlock)), dim3(kBlock), 0, s, P, c,
-                           reinterpret_cast<float4*>(film));
+        if (adaptive)
+            hipLaunchKernelGGL(k_film<true>, dim3(blocks(pixels, kBlock)), dim3(kBlock), 0, s, P, c,
+                               reinterpret_cast<float4*>(film), moments, counters);
+        else
+            hipLaunchKernelGGL(k_film<false>, dim3(blocks(pixels, kBlock)), dim3(kBlock), 0, s, P, c,
+                               reinterpret_cast<float4*>(film), moments, counters);
         VH_HIP(hipGetLastError());
+        first += nb;
     }
     VH_HIP(hipEventRecord(e1, s));
     uint32_t h[24];
@@ -736,7 +885,8 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, hipStrea
         (void)hipEventDestroy(m.a);
         (void)hipEventDestroy(m.b);
     }
-    uint64_t ext, shd, v[4];
+    uint64_t ext, shd, stopped, v[4];
+    std::memcpy(&stopped, &h[10], 8);
     std::memcpy(&ext, &h[4], 8);
     std::memcpy(&shd, &h[6], 8);
     std::memcpy(v, &h[16], 32);
@@ -751,12 +901,14 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, hipStrea
     sc->stats.extend_triangle_tests = v[1];
     sc->stats.shadow_node_visits = v[2];
     sc->stats.shadow_triangle_tests = v[3];
-    sc->stats.paths = (uint64_t)pixels * total;
+    sc->stats.paths = (uint64_t)pixels * total - stopped;
+    sc->stats.stopped_samples = stopped;
     sc->stats.extension_rays = ext;
     sc->stats.shadow_rays = shd;
     sc->stats.dropped_samples = h[9];
     sc->stats.device_ms = ms;
     if (h[8] & kStackOverflow) return fail(VH_ERR_DEVICE, "traversal stack overflow during render");
+    if (adaptive) sc->moments_next = rp->sample_end;
     return VH_OK;
 }
 

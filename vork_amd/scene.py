@@ -168,9 +168,10 @@ class Scene:
         d.sky = self.sky_struct()
         return d
 
-    def render_params(self, samples=None, seed=None, width=None, height=None):
+    def render_params(self, samples=None, seed=None, width=None, height=None, adaptive=True):
         p = abi.default_params(width or self.width, height or self.height, samples or self.spp,
-                               seed=self.seed if seed is None else seed, max_bounces=self.max_depth)
+                               seed=self.seed if seed is None else seed, max_bounces=self.max_depth,
+                               adaptive=adaptive)
         return p
 
 
