@@ -22,6 +22,12 @@
  *    TLAS or scene is immutable after it is built and may be read by several
  *    threads at once.  Calls that take a stream are ordered on that stream
  *    (NULL = the device's own stream).
+ *  - vh_render and vh_trace_* return once their work is done.  Each scene
+ *    owns its wavefront state, so vh_render calls on different scenes (which
+ *    may share a TLAS) run concurrently from several threads, and on
+ *    different streams their kernels overlap on the device; calls on one
+ *    scene wait for each other.  vh_trace_* calls on one device wait for
+ *    each other.
  */
 #ifndef VORK_HIP_H
 #define VORK_HIP_H

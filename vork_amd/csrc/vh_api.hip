@@ -518,6 +518,7 @@ int vh_render(vh_scene* sc, const vh_render_params* p, uint32_t flags, void* str
     vh_device* dev = sc->device;
     VH_HIP(hipSetDevice(dev->ordinal));
     hipStream_t s = pick(dev, stream);
+    std::lock_guard<std::mutex> hold(sc->render_lock);
     if (flags & VH_DEVICE_POINTERS) return render_scene(sc, p, film, s);
     size_t bytes = sizeof(float) * 4 * (size_t)p->width * p->height;
     DeviceBuffer f;
@@ -533,6 +534,7 @@ int vh_render(vh_scene* sc, const vh_render_params* p, uint32_t flags, void* str
 int vh_render_get_stats(const vh_scene* sc, vh_render_stats* stats) {
     clear();
     if (!sc || !stats) return fail(VH_ERR_USAGE, "vh_render_get_stats: null argument");
+    std::lock_guard<std::mutex> hold(const_cast<vh_scene*>(sc)->render_lock);
     *stats = sc->stats;
     return VH_OK;
 }

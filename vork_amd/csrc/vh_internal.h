@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -53,7 +55,9 @@ struct vh_device {
     int cu_count = 0;
     // GGX energy tables (moments 128x128 float4, dielectric 65x64x64 float)
     vh::DeviceBuffer lut_moments, lut_dielectric;
-    // shared scratch of the build and trace kernels
+    // scratch of vh_trace_* (overflow stacks, error word and ray claims),
+    // held under trace_lock for the whole (synchronous) call
+    std::mutex trace_lock;
     vh::DeviceBuffer stack_overflow;
     vh::DeviceBuffer scratch_u32;
 };
@@ -99,7 +103,11 @@ struct vh_scene {
     vh_camera camera{};
     vh_sky sky{};
     // wavefront state, grown on demand
-    vh::DeviceBuffer paths, rays_a, rays_b, shadow_rays, pending, counters;
+    // vh_render calls on one scene are serialized under render_lock: the
+    // wavefront state, overflow stacks and moments below are the scene's own,
+    // so renders of different scenes may overlap on the device
+    std::mutex render_lock;
+    vh::DeviceBuffer paths, rays_a, rays_b, shadow_rays, pending, counters, overflow;
     size_t capacity = 0;
     // adaptive sampling moments (48 B per pixel) and the exposure they belong
     // to: size, seed, sample count and the next sample index

@@ -707,7 +707,7 @@ int bake_luts(vh_device* dev) {
     return VH_OK;
 }
 
-int ensure_overflow(vh_device* dev, uint32_t threads);
+int ensure_overflow(DeviceBuffer& buf, uint32_t threads);
 
 int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, hipStream_t s) {
     vh_device* dev = sc->device;
@@ -793,10 +793,10 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, hipStrea
     c.camera = sc->camera;
     c.sky = sc->sky;
     const uint32_t grid = std::min<uint32_t>(blocks(cap, kBlock), (uint32_t)dev->cu_count * 8u);
-    int rc = ensure_overflow(dev, grid * kBlock);
+    int rc = ensure_overflow(sc->overflow, grid * kBlock);
     if (rc) return rc;
     uint32_t* counters = sc->counters.as<uint32_t>();
-    uint32_t* overflow = dev->stack_overflow.as<uint32_t>();
+    uint32_t* overflow = sc->overflow.as<uint32_t>();
     VH_HIP(hipMemsetAsync(counters, 0, 64 * sizeof(uint32_t), s));
     const bool profile = (rp->flags & VH_RENDER_PROFILE) != 0;
     const bool count = (rp->flags & VH_RENDER_COUNT_VISITS) != 0;

@@ -70,9 +70,9 @@ uint32_t trace_grid(vh_device* dev, uint32_t n) {
 }
 
 // Overflow stacks: kGlobalStack 8-byte entries per thread.
-int ensure_overflow(vh_device* dev, uint32_t threads) {
+int ensure_overflow(DeviceBuffer& buf, uint32_t threads) {
     size_t need = (size_t)threads * kGlobalStack * 2 * sizeof(uint32_t);
-    VH_HIP(dev->stack_overflow.reserve(need));
+    VH_HIP(buf.reserve(need));
     return VH_OK;
 }
 
@@ -80,7 +80,8 @@ int launch_trace(vh_device* dev, hipStream_t s, const TlasView& tv, const void* 
                  void* out, int mode) {
     if (count == 0) return VH_OK;
     uint32_t grid = trace_grid(dev, count);
-    int rc = ensure_overflow(dev, grid * kTraceBlock);
+    std::lock_guard<std::mutex> hold(dev->trace_lock);
+    int rc = ensure_overflow(dev->stack_overflow, grid * kTraceBlock);
     if (rc) return rc;
     VH_HIP(dev->scratch_u32.reserve(2 * sizeof(uint32_t)));
     uint32_t* errors = dev->scratch_u32.as<uint32_t>();  // [0] errors, [1] ray claims

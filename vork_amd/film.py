@@ -323,7 +323,7 @@ def write_outputs(path, film, view="agx", exposure=0.0, gamma=1.0, lut=None, dep
             json.dump(meta, f, indent=1)
 
 
-def metadata(scene, stats, warnings, samples, wall_ms, camera="camera", profile="std/path"):
+def metadata(scene, stats, warnings, samples, wall_ms, camera="camera", profile="std/path", adaptive=True):
     """The version-3 sidecar of crates/vork-render/src/shot.rs:1324-1331
     restricted to a still of the path integrator."""
     return {
@@ -333,7 +333,7 @@ def metadata(scene, stats, warnings, samples, wall_ms, camera="camera", profile=
                     "invalid_pixel_count": 0, "exposure": None}],
         "profile": {"name": profile, "level": 2},
         "settings": {"integrator": "path", "samples": samples, "seed": scene.seed},
-        "render_settings": {"profile": {"name": profile, "level": 2}, "samples": samples, "adaptive": False,
+        "render_settings": {"profile": {"name": profile, "level": 2}, "samples": samples, "adaptive": adaptive,
                             "denoise": False, "resolution": [scene.width, scene.height], "frames": 1},
         "denoiser": None, "wall_ms": wall_ms,
     }

@@ -14,10 +14,24 @@ import sys
 import time
 
 
+def write_render(scene, outs, img, stats, samples, wall_ms, view="agx", agx_lut=None, warnings=(),
+                 adaptive=True):
+    """Writes the film to each path of `outs` (+ .meta.json).  Returns the
+    sidecar (the same for every output)."""
+    from . import film as F
+    warnings = list(warnings)
+    lut = F.load_agx_lut(agx_lut) if agx_lut else None
+    if view == "agx" and lut is None and any(o.lower().endswith(".png") for o in outs):
+        warnings.append("view agx: analytic AgX (no AgX LUT given); codes differ from the baked view")
+    meta = F.metadata(scene, stats, warnings, samples, wall_ms, adaptive=adaptive)
+    for out in outs:
+        F.write_outputs(out, img, view=view, lut=lut, meta=meta)
+    return meta
+
+
 def render_scene(scene, out, view="agx", agx_lut=None, warnings=()):
     """Renders a loaded Scene on device 0 and writes `out` (+ .meta.json).
     Returns the film (H, W, 4) and the render stats."""
-    from . import film as F
     from . import native
     t0 = time.perf_counter()
     r = native.Renderer(0)
@@ -26,12 +40,7 @@ def render_scene(scene, out, view="agx", agx_lut=None, warnings=()):
     img = inst.render(p)
     stats = inst.stats()
     wall = (time.perf_counter() - t0) * 1e3
-    warnings = list(warnings)
-    lut = F.load_agx_lut(agx_lut) if agx_lut else None
-    if view == "agx" and lut is None and out.lower().endswith(".png"):
-        warnings.append("view agx: analytic AgX (no AgX LUT given); codes differ from the baked view")
-    meta = F.metadata(scene, stats, warnings, p.samples, wall)
-    F.write_outputs(out, img, view=view, lut=lut, meta=meta)
+    write_render(scene, [out], img, stats, p.samples, wall, view, agx_lut, warnings)
     return img, stats
 
 

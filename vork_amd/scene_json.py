@@ -130,6 +130,13 @@ def _num(v, where, n=None):
     return a
 
 
+def _scalar(v, where):
+    a = _num(v, where)
+    if a.shape != ():
+        raise SceneError(where, "expected a number")
+    return float(a)
+
+
 # ---------------------------------------------------------------- loader
 class _Loader:
     def __init__(self, doc, base_dir, overrides):
@@ -377,7 +384,7 @@ class _Loader:
         for k in ("temperature", "image", "medium", "illuminates", "shadowed_by"):
             if node.get(k) is not None:
                 raise SceneError(f"{where}.{k}", "not rendered by this build")
-        energy = float(node["energy"]) if node.get("energy") is not None else None
+        energy = _scalar(node["energy"], f"{where}.energy") if node.get("energy") is not None else None
         if shape == "environment":
             if node.get("material") is not None:
                 raise SceneError(f"{where}.material", "environment programs are not rendered by this build")
@@ -389,8 +396,8 @@ class _Loader:
             return
         if shape != "rectangle":
             raise SceneError(f"{where}.shape", f"light shape '{shape}' is not rendered by this build (rectangle, environment)")
-        width = float(node.get("size", AREA_DIMENSION))
-        height = float(node["size_y"]) if node.get("size_y") is not None else width
+        width = _scalar(node.get("size", AREA_DIMENSION), f"{where}.size")
+        height = _scalar(node["size_y"], f"{where}.size_y") if node.get("size_y") is not None else width
         color = _num(node.get("color", (1.0, 1.0, 1.0)), f"{where}.color", 3)
         e = LOCAL_INTENSITY if energy is None else energy
         axes = world[:3, :3]
