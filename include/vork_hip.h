@@ -22,12 +22,14 @@
  *    TLAS or scene is immutable after it is built and may be read by several
  *    threads at once.  Calls that take a stream are ordered on that stream
  *    (NULL = the device's own stream).
- *  - vh_render and vh_trace_* return once their work is done.  Each scene
- *    owns its wavefront state, so vh_render calls on different scenes (which
- *    may share a TLAS) run concurrently from several threads, and on
- *    different streams their kernels overlap on the device; calls on one
- *    scene wait for each other.  vh_trace_* calls on one device wait for
- *    each other.
+ *  - vh_render and vh_trace_* return once their work is done.  Their
+ *    scratch (wavefront queues and path state, traversal overflow stacks)
+ *    belongs to the stream: the device creates it on a stream's first call,
+ *    grows it to the largest call and frees it with the device.  Calls on
+ *    one stream wait for each other; calls on different streams run
+ *    concurrently from several threads and their kernels overlap on the
+ *    device.  vh_render calls on one scene also wait for each other (the
+ *    scene holds its adaptive moments and render stats).
  */
 #ifndef VORK_HIP_H
 #define VORK_HIP_H
@@ -72,7 +74,9 @@ typedef struct vh_scene vh_scene;
 
 /* ---------------------------------------------------------------- device */
 
-/* Opens HIP device `ordinal` with its own stream.  Replaces the wgpu device
+/* Opens HIP device `ordinal` with its own stream.  Every BLAS, TLAS and
+ * scene built on the device keeps it open: after vh_device_release the device
+ * is closed when the last of them is released, in whatever order they are.  Replaces the wgpu device
  * the reference opens in Renderer::new (crates/vork-render/src/lib.rs:726). */
 int vh_device_create(int ordinal, vh_device** out);
 void vh_device_release(vh_device* device);

@@ -232,3 +232,20 @@ def test_adaptive_range_must_continue_its_exposure(renderer):
     r.sample_begin = 20
     with pytest.raises(VorkError):
         inst.render(r)
+
+
+def test_objects_keep_their_device_open_in_any_release_order():
+    # the device handle released first: the scene still renders, and the
+    # objects release in dependency-free order afterwards
+    from vork_amd import native
+    sc = S.config0(width=24, height=16, spp=2)
+    r = native.Renderer(0)
+    inst = r.install(sc)
+    want = inst.render(sc.render_params(adaptive=False))
+    r.device.close()
+    got = inst.render(sc.render_params(adaptive=False))
+    assert np.array_equal(got, want)
+    for b in inst.blases:
+        b.close()
+    inst.tlas.close()
+    inst.close()

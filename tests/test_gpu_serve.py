@@ -1,0 +1,60 @@
+"""The render server on the GPU: requests of uneven scene, resolution and
+sample count rendered alone (one slot, one at a time) and in the mix (four
+slots on four HIP streams, all submitted at once, with a budget that forces
+evictions) give bit-identical films."""
+import numpy as np
+import pytest
+
+from vork_amd import scene as S
+from vork_amd import serve
+
+pytestmark = pytest.mark.gpu
+
+
+def requests():
+    scenes = {"c0": S.config0(width=64, height=48, spp=4), "c0b": S.config0(seed=2, width=48, height=48, spp=4),
+              "g": S.small_config1(width=96, height=54, spp=4)}
+    mix = [("c0", 64, 48, 4, 1, False), ("g", 96, 54, 8, 2, False), ("c0b", 32, 32, 1, 3, False),
+           ("c0", 128, 72, 2, 4, False), ("g", 48, 27, 16, 5, False), ("c0b", 80, 60, 4, 6, False),
+           ("c0", 64, 48, 24, 7, True), ("g", 96, 54, 20, 8, True), ("c0", 64, 48, 4, 1, False)]
+    return scenes, mix
+
+
+def run(backend, slots, budget=None):
+    scenes, mix = requests()
+    s = serve.Server(backend, slots=slots, budget=budget)
+    try:
+        reqs = [s.submit(k, scenes[k], w, h, spp, seed, adaptive) for k, w, h, spp, seed, adaptive in mix]
+        s.wait_all(120)
+        return [r.result() for r in reqs], s.report()
+    finally:
+        s.close()
+
+
+def test_results_are_the_same_alone_and_in_the_mix():
+    backend = serve.GpuBackend(0, slots=4)
+    try:
+        alone, rep1 = run(backend, 1)
+        for k in list(backend.scenes):
+            backend.release(k)
+        mixed, rep4 = run(backend, 4)
+        assert rep1["requests"] == rep4["requests"] == len(alone) and rep4["failed"] == 0
+        for a, m in zip(alone, mixed):
+            assert np.isfinite(a).all() and a[..., :3].mean() > 0
+            assert np.array_equal(a, m)
+        assert np.array_equal(alone[0], alone[-1])  # a repeat request of a resident scene
+        # every scene counted as 1 GiB against a budget with room for one and a
+        # half beside the largest films and scratch: one scene resident at a time
+        for k in list(backend.scenes):
+            backend.release(k)
+        scenes, mix = requests()
+        state = max(backend.state_bytes(w, h, spp) for _, w, h, spp, _, _ in mix)
+        film = max(w * h * (serve.FILM_BYTES + serve.MOMENT_BYTES) for _, w, h, _, _, _ in mix)
+        backend.scene_bytes = lambda scene: 1 << 30
+        tight = 4 * (backend.slot_base_bytes() + state + film) + (3 << 29)
+        evicted, rep = run(backend, 4, budget=tight)
+        assert rep["evictions"] > 0
+        for a, m in zip(alone, evicted):
+            assert np.array_equal(a, m)
+    finally:
+        backend.close()

@@ -25,6 +25,26 @@ int hip_fail(hipError_t e, const char* what) {
     return fail(code, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+void device_retain(vh_device* d) { d->refs.fetch_add(1, std::memory_order_relaxed); }
+
+void device_drop(vh_device* d) {
+    if (d->refs.fetch_sub(1, std::memory_order_acq_rel) != 1) return;
+    (void)hipSetDevice(d->ordinal);
+    if (d->stream) (void)hipStreamSynchronize(d->stream);
+    d->lut_moments.release();
+    d->lut_dielectric.release();
+    d->streams.clear();
+    if (d->stream) (void)hipStreamDestroy(d->stream);
+    delete d;
+}
+
+StreamState* stream_state(vh_device* dev, hipStream_t s) {
+    std::lock_guard<std::mutex> hold(dev->streams_lock);
+    std::unique_ptr<StreamState>& st = dev->streams[s];
+    if (!st) st.reset(new StreamState());
+    return st.get();
+}
+
 namespace {
 int clear() {
     g_last_status = VH_OK;
@@ -88,15 +108,7 @@ int vh_device_create(int ordinal, vh_device** out) {
 }
 
 void vh_device_release(vh_device* d) {
-    if (!d) return;
-    (void)hipSetDevice(d->ordinal);
-    if (d->stream) (void)hipStreamSynchronize(d->stream);
-    d->lut_moments.release();
-    d->lut_dielectric.release();
-    d->stack_overflow.release();
-    d->scratch_u32.release();
-    if (d->stream) (void)hipStreamDestroy(d->stream);
-    delete d;
+    if (d) device_drop(d);
 }
 
 int vh_device_synchronize(vh_device* d) {

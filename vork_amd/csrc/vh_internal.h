@@ -4,6 +4,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
+#include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -47,30 +50,67 @@ struct DeviceBuffer {
     template <class T> T* as() const { return static_cast<T*>(ptr); }
 };
 
+// Wavefront and traversal scratch of one stream.  Calls on one stream are
+// ordered on the device, so they share it; `lock` is held for the whole
+// (synchronous) call, so calls on one stream from several threads wait for
+// each other while calls on different streams overlap.
+struct StreamState {
+    std::mutex lock;
+    DeviceBuffer paths, rays_a, rays_b, shadow_rays, pending, counters, overflow, scratch_u32;
+    size_t capacity = 0;  // paths the wavefront buffers hold
+};
+
 }  // namespace vh
 
 struct vh_device {
+    // one reference for the caller's handle and one per BLAS, TLAS and scene
+    // built on the device: it is destroyed when the last is released
+    std::atomic<int> refs{1};
     int ordinal = 0;
     hipStream_t stream = nullptr;
     int cu_count = 0;
     // GGX energy tables (moments 128x128 float4, dielectric 65x64x64 float)
     vh::DeviceBuffer lut_moments, lut_dielectric;
-    // scratch of vh_trace_* (overflow stacks, error word and ray claims),
-    // held under trace_lock for the whole (synchronous) call
-    std::mutex trace_lock;
-    vh::DeviceBuffer stack_overflow;
-    vh::DeviceBuffer scratch_u32;
+    // per-stream scratch, created on a stream's first render or trace call
+    // and kept (grown to the largest call) until the device is released
+    std::mutex streams_lock;
+    std::map<hipStream_t, std::unique_ptr<vh::StreamState>> streams;
 };
 
+namespace vh {
+void device_retain(vh_device* d);
+void device_drop(vh_device* d);
+
+// A counted reference to the device an object was built on.  Declared first
+// in each object, so it is dropped after the object's device memory is freed.
+class DeviceRef {
+  public:
+    DeviceRef() = default;
+    DeviceRef(const DeviceRef&) = delete;
+    DeviceRef& operator=(const DeviceRef&) = delete;
+    ~DeviceRef() { reset(nullptr); }
+    DeviceRef& operator=(vh_device* d) { reset(d); return *this; }
+    operator vh_device*() const { return dev_; }
+    vh_device* operator->() const { return dev_; }
+  private:
+    void reset(vh_device* d) {
+        if (d) device_retain(d);
+        if (dev_) device_drop(dev_);
+        dev_ = d;
+    }
+    vh_device* dev_ = nullptr;
+};
+}  // namespace vh
+
 struct vh_blas {
-    vh_device* device = nullptr;
+    vh::DeviceRef device;  // first: dropped after the buffers below are freed
     vh::DeviceBuffer nodes, tris, tri_ids, positions, normals, indices, materials;
     vh::BlasRecord record{};
     vh_blas_info info{};
 };
 
 struct vh_tlas {
-    vh_device* device = nullptr;
+    vh::DeviceRef device;  // first: dropped after the buffers below are freed
     vh::DeviceBuffer nodes, inst_ids, instances, blas_records;
     std::vector<vh_blas*> blases;
     uint32_t instance_count = 0;
@@ -96,19 +136,15 @@ struct vh_light_node {
 };
 
 struct vh_scene {
-    vh_device* device = nullptr;
+    vh::DeviceRef device;  // first: dropped after the buffers below are freed
     const vh_tlas* tlas = nullptr;
     vh::DeviceBuffer materials, lights, light_tree, filter;
     uint32_t material_count = 0, light_count = 0, light_nodes = 0;
     vh_camera camera{};
     vh_sky sky{};
-    // wavefront state, grown on demand
-    // vh_render calls on one scene are serialized under render_lock: the
-    // wavefront state, overflow stacks and moments below are the scene's own,
-    // so renders of different scenes may overlap on the device
+    // vh_render calls on one scene are serialized under render_lock (the
+    // adaptive moments and the stats below are the scene's own)
     std::mutex render_lock;
-    vh::DeviceBuffer paths, rays_a, rays_b, shadow_rays, pending, counters, overflow;
-    size_t capacity = 0;
     // adaptive sampling moments (48 B per pixel) and the exposure they belong
     // to: size, seed, sample count and the next sample index
     vh::DeviceBuffer moments;
@@ -154,5 +190,7 @@ int bake_luts(vh_device* dev);
 
 // --------------------------------------------------------------- render
 int render_scene(vh_scene* scene, const vh_render_params* p, float* film, hipStream_t s);
+// The scratch of stream s on this device, created on first use.
+StreamState* stream_state(vh_device* dev, hipStream_t s);
 
 }  // namespace vh

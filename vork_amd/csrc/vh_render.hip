@@ -724,15 +724,17 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, hipStrea
     if (pixels > kPathMask) return fail(VH_ERR_USAGE, "vh_render: more than 2^24 - 1 pixels");
     batch = std::min<uint32_t>(batch, kPathMask / pixels);
     const size_t cap = (size_t)pixels * batch;
-    if (sc->capacity < cap) {
-        VH_HIP(sc->paths.allocate(cap * (sizeof(float4) * 6 + sizeof(uint32_t) + 2)));
-        VH_HIP(sc->rays_a.allocate(cap * 2 * sizeof(float4)));
-        VH_HIP(sc->rays_b.allocate(cap * 2 * sizeof(float4)));
-        VH_HIP(sc->shadow_rays.allocate(2 * cap * 2 * sizeof(float4)));
-        VH_HIP(sc->pending.allocate(cap * sizeof(uint32_t)));
-        sc->capacity = cap;
+    StreamState* st = stream_state(dev, s);
+    std::lock_guard<std::mutex> hold(st->lock);
+    if (st->capacity < cap) {
+        VH_HIP(st->paths.allocate(cap * (sizeof(float4) * 6 + sizeof(uint32_t) + 2)));
+        VH_HIP(st->rays_a.allocate(cap * 2 * sizeof(float4)));
+        VH_HIP(st->rays_b.allocate(cap * 2 * sizeof(float4)));
+        VH_HIP(st->shadow_rays.allocate(2 * cap * 2 * sizeof(float4)));
+        VH_HIP(st->pending.allocate(cap * sizeof(uint32_t)));
+        st->capacity = cap;
     }
-    VH_HIP(sc->counters.reserve(64 * sizeof(uint32_t)));
+    VH_HIP(st->counters.reserve(64 * sizeof(uint32_t)));
     const bool adaptive = (rp->flags & VH_RENDER_ADAPTIVE) != 0;
     // path.slang:1801-1802: minimum max(16, sqrt(samples)) and threshold 0.01
     // unless set; a minimum at or beyond the exposure means no pixel stops.
@@ -759,7 +761,7 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, hipStrea
     }
     Paths P;
     {
-        char* base = sc->paths.as<char>();
+        char* base = st->paths.as<char>();
         float4** f4[] = {&P.thr, &P.rad, (float4**)&P.info, &P.hit, &P.pend, &P.pend1};
         size_t off = 0;
         for (auto* q : f4) {
@@ -793,10 +795,10 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, hipStrea
     c.camera = sc->camera;
     c.sky = sc->sky;
     const uint32_t grid = std::min<uint32_t>(blocks(cap, kBlock), (uint32_t)dev->cu_count * 8u);
-    int rc = ensure_overflow(sc->overflow, grid * kBlock);
+    int rc = ensure_overflow(st->overflow, grid * kBlock);
     if (rc) return rc;
-    uint32_t* counters = sc->counters.as<uint32_t>();
-    uint32_t* overflow = sc->overflow.as<uint32_t>();
+    uint32_t* counters = st->counters.as<uint32_t>();
+    uint32_t* overflow = st->overflow.as<uint32_t>();
     VH_HIP(hipMemsetAsync(counters, 0, 64 * sizeof(uint32_t), s));
     const bool profile = (rp->flags & VH_RENDER_PROFILE) != 0;
     const bool count = (rp->flags & VH_RENDER_COUNT_VISITS) != 0;
@@ -828,10 +830,10 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, hipStrea
         c.batch_first = first;
         c.batch_samples = nb;
         const uint32_t n = pixels * nb;
-        float4* qa = sc->rays_a.as<float4>();
-        float4* qb = sc->rays_b.as<float4>();
-        float4* sq = sc->shadow_rays.as<float4>();
-        uint32_t* pending = sc->pending.as<uint32_t>();
+        float4* qa = st->rays_a.as<float4>();
+        float4* qb = st->rays_b.as<float4>();
+        float4* sq = st->shadow_rays.as<float4>();
+        uint32_t* pending = st->pending.as<uint32_t>();
         if (adaptive) {
             if (c.decide) hipLaunchKernelGGL(k_decide, dim3(blocks(pixels, kBlock)), dim3(kBlock), 0, s, moments, c, first);
             hipLaunchKernelGGL(k_open_batch, dim3(1), dim3(64), 0, s, counters);

@@ -80,14 +80,15 @@ int launch_trace(vh_device* dev, hipStream_t s, const TlasView& tv, const void* 
                  void* out, int mode) {
     if (count == 0) return VH_OK;
     uint32_t grid = trace_grid(dev, count);
-    std::lock_guard<std::mutex> hold(dev->trace_lock);
-    int rc = ensure_overflow(dev->stack_overflow, grid * kTraceBlock);
+    StreamState* st = stream_state(dev, s);
+    std::lock_guard<std::mutex> hold(st->lock);
+    int rc = ensure_overflow(st->overflow, grid * kTraceBlock);
     if (rc) return rc;
-    VH_HIP(dev->scratch_u32.reserve(2 * sizeof(uint32_t)));
-    uint32_t* errors = dev->scratch_u32.as<uint32_t>();  // [0] errors, [1] ray claims
+    VH_HIP(st->scratch_u32.reserve(2 * sizeof(uint32_t)));
+    uint32_t* errors = st->scratch_u32.as<uint32_t>();  // [0] errors, [1] ray claims
     VH_HIP(hipMemsetAsync(errors, 0, 2 * sizeof(uint32_t), s));
     const vh_ray* r = static_cast<const vh_ray*>(rays);
-    uint32_t* of = dev->stack_overflow.as<uint32_t>();
+    uint32_t* of = st->overflow.as<uint32_t>();
     if (mode == 0)
         hipLaunchKernelGGL(k_trace<0>, dim3(grid), dim3(kTraceBlock), 0, s, tv,
                            ArraySource<0>{r, count, mask, out, tv.instances}, of, errors);

@@ -239,17 +239,19 @@ class InstalledScene:
         self.handle = C.c_void_p()
         check(lib().vh_scene_create(device.handle, tlas.handle, C.byref(desc), C.byref(self.handle)))
 
-    def render(self, params=None, film=None, device_film=None):
+    def render(self, params=None, film=None, device_film=None, stream=None):
         """Adds samples [params.sample_begin, params.sample_end) to `film`
         (numpy (H,W,4) float32, created if None) or, with `device_film` (a torch
-        CUDA tensor), to device memory.  Returns the film."""
+        CUDA tensor), to device memory.  `stream` is a HIP stream handle (int)
+        or None for the device's own stream.  Returns the film."""
         p = params or self.scene.render_params()
+        s = C.c_void_p(stream) if stream else None
         if device_film is not None:
-            check(lib().vh_render(self.handle, C.byref(p), abi.VH_DEVICE_POINTERS, None, _dev_ptr(device_film)))
+            check(lib().vh_render(self.handle, C.byref(p), abi.VH_DEVICE_POINTERS, s, _dev_ptr(device_film)))
             return device_film
         if film is None:
             film = np.zeros((p.height, p.width, 4), np.float32)
-        check(lib().vh_render(self.handle, C.byref(p), 0, None, _ptr(film)))
+        check(lib().vh_render(self.handle, C.byref(p), 0, s, _ptr(film)))
         return film
 
     def stats(self):

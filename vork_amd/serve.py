@@ -1,0 +1,352 @@
+"""Continuous-batching render server on one GPU.
+
+A persistent request loop over the C-ABI: scenes stay resident in HBM (BLASes,
+TLAS, scene tables) across repeat requests, and renders of uneven resolution
+and sample count are admitted as slots and memory free up.  Each slot is a
+HIP stream with its own wavefront scratch (the library keeps one per stream),
+driven by its own host thread: the renders of the admitted requests run at
+once and their kernels share the CUs (waves of different requests are
+resident together), so small requests fill the chip that one alone leaves
+idle.  A request's result depends only on its scene and options: the
+counter-based sampler fixes every path, and each pixel's film sums its
+samples in sample order, so a result is the same alone and in the mix.
+
+Admission is first-fit over the queue in arrival order: a request is admitted
+when a slot is free and the HBM it needs fits the budget after evicting
+resident scenes no admitted request uses (least recently used first): its
+film, its scene if not resident, and the growth of the slot's stream scratch
+(kept at its high-water mark, as the library keeps it).
+
+`Server(backend, ...)` holds the scheduling; `GpuBackend` installs and renders
+through vork_amd.native.  A request's latency is finish - submit; the server
+reports p50/p99 latency beside throughput (samples per second over the run).
+"""
+import collections
+import threading
+import time
+
+import numpy as np
+
+PATH_BYTES = 234                   # path state 102 + extension queues 2x32 + shadow queue 2x32 + pending 4
+FILM_BYTES = 16                    # RGBA32F per pixel
+MOMENT_BYTES = 48                  # adaptive moment per pixel
+BATCH_PATHS = 4 << 20              # paths per wavefront batch the library aims for
+MAX_BATCH_PATHS = (1 << 24) - 1    # paths of one batch (24-bit path ids)
+
+
+def batch_paths(width, height, spp):
+    """Paths in flight per batch, as the library sizes them by default."""
+    pixels = width * height
+    batch = min(max(1, BATCH_PATHS // pixels), spp, MAX_BATCH_PATHS // pixels)
+    return pixels * batch
+
+
+class Request:
+    def __init__(self, rid, key, scene, width, height, spp, seed, adaptive):
+        self.rid, self.key, self.scene = rid, key, scene
+        self.width, self.height, self.spp, self.seed, self.adaptive = width, height, spp, seed, adaptive
+        self.submitted = time.perf_counter()
+        self.started = self.finished = None
+        self.slot = None
+        self.film = self.stats = self.error = None
+        self.done = threading.Event()
+        self.bytes = 0
+
+    @property
+    def samples(self):
+        return self.width * self.height * self.spp
+
+    @property
+    def latency(self):
+        return None if self.finished is None else self.finished - self.submitted
+
+    def result(self, timeout=None):
+        if not self.done.wait(timeout):
+            raise TimeoutError(f"request {self.rid} not finished")
+        if self.error is not None:
+            raise self.error
+        return self.film
+
+
+class _Scene:
+    def __init__(self, key, scene, nbytes):
+        self.key, self.scene, self.bytes = key, scene, nbytes
+        self.users = 0
+        self.ready = threading.Event()
+        self.error = None
+        self.installing = False
+
+
+class Server:
+    def __init__(self, backend, slots=4, budget=None):
+        self.backend = backend
+        self.slots = slots
+        self.budget = budget if budget is not None else backend.memory_budget()
+        self.cv = threading.Condition()
+        self.queue = collections.deque()
+        self.free = list(range(slots))
+        self.base_bytes = backend.slot_base_bytes() * slots     # the streams' traversal overflow stacks
+        self.slot_bytes = [0] * slots                           # wavefront scratch high-water mark per slot
+        self.resident = collections.OrderedDict()               # key -> _Scene, least recently used first
+        self.active = {}                                        # rid -> Request
+        self.finished = []
+        self.evictions = 0
+        self.next_id = 0
+        self.running = True
+        self.work = [collections.deque() for _ in range(slots)]
+        self.threads = [threading.Thread(target=self._slot_loop, args=(k,), daemon=True) for k in range(slots)]
+        for t in self.threads:
+            t.start()
+
+    # -------------------------------------------------------------- API
+    def submit(self, key, scene=None, width=None, height=None, spp=None, seed=None, adaptive=False):
+        """Queues a render of scene `key` (the Scene object is needed unless
+        the scene is resident or queued).  Returns a Request."""
+        with self.cv:
+            known = key in self.resident or any(r.key == key and r.scene is not None for r in self.queue)
+            if scene is None and not known:
+                raise KeyError(f"scene {key!r} is not resident; pass the scene")
+            base = scene if scene is not None else self._scene_of(key)
+            req = Request(self.next_id, key, scene, width or base.width, height or base.height, spp or base.spp,
+                          base.seed if seed is None else seed, adaptive)
+            self.next_id += 1
+            # what it needs with every other request finished and every other scene evicted
+            sb, total = self._scene_bytes(req), sum(self.slot_bytes)
+            need = self.base_bytes + min(total + self._need(req, hw, sb) for hw in self.slot_bytes)
+            if need > self.budget:
+                raise MemoryError(f"request {req.rid} needs {need} bytes, over the {self.budget}-byte budget")
+            self.queue.append(req)
+            self._admit()
+            return req
+
+    def wait_all(self, timeout=None):
+        end = None if timeout is None else time.perf_counter() + timeout
+        with self.cv:
+            while self.queue or self.active:
+                left = None if end is None else end - time.perf_counter()
+                if left is not None and left <= 0:
+                    raise TimeoutError("requests still running")
+                self.cv.wait(left)
+
+    def close(self):
+        with self.cv:
+            self.running = False
+            self.cv.notify_all()
+        for t in self.threads:
+            t.join()
+        self.threads = []   # the threads' targets refer back to the server
+
+    def report(self):
+        """Throughput and latency of the finished requests."""
+        done = [r for r in self.finished if r.error is None]
+        if not done:
+            return {"requests": 0}
+        lat = np.array([r.latency for r in done]) * 1e3
+        t0 = min(r.submitted for r in done)
+        t1 = max(r.finished for r in done)
+        samples = sum(r.samples for r in done)
+        return {"requests": len(done), "failed": len(self.finished) - len(done), "samples": samples,
+                "wall_s": t1 - t0, "msamples_per_s": samples / (t1 - t0) / 1e6,
+                "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                               "mean": float(lat.mean())},
+                "evictions": self.evictions}
+
+    # -------------------------------------------------------------- admission
+    def _scene_of(self, key):
+        if key in self.resident:
+            return self.resident[key].scene
+        for r in self.queue:
+            if r.key == key and r.scene is not None:
+                return r.scene
+        raise KeyError(key)
+
+    def _scene_bytes(self, req):
+        if req.key in self.resident:
+            return 0
+        return self.backend.scene_bytes(req.scene if req.scene is not None else self._scene_of(req.key))
+
+    def _need(self, req, slot_hw, scene_bytes):
+        pixels = req.width * req.height
+        state = self.backend.state_bytes(req.width, req.height, req.spp)
+        grow = max(0, state - slot_hw)
+        return pixels * (FILM_BYTES + (MOMENT_BYTES if req.adaptive else 0)) + grow + scene_bytes
+
+    def _used(self):
+        return (self.base_bytes + sum(s.bytes for s in self.resident.values()) + sum(self.slot_bytes) +
+                sum(r.bytes for r in self.active.values()))
+
+    def _admit(self):
+        """Admits queued requests, first fit in arrival order (holds cv)."""
+        progress = True
+        while progress and self.free and self.queue:
+            progress = False
+            for req in list(self.queue):
+                if not self.free:
+                    break
+                state = self.backend.state_bytes(req.width, req.height, req.spp)
+                # prefer a slot whose scratch already holds this request
+                slot = min(self.free, key=lambda k: (self.slot_bytes[k] < state, -self.slot_bytes[k]))
+                sb = self._scene_bytes(req)
+                need = self._need(req, self.slot_bytes[slot], sb)
+                if self._used() + need > self.budget and not self._evict(self._used() + need - self.budget, req.key):
+                    continue
+                if req.key not in self.resident:
+                    self.resident[req.key] = _Scene(req.key, req.scene if req.scene is not None
+                                                    else self._scene_of(req.key), sb)
+                self.queue.remove(req)
+                self.free.remove(slot)
+                self.slot_bytes[slot] = max(self.slot_bytes[slot], state)
+                req.bytes = req.width * req.height * (FILM_BYTES + (MOMENT_BYTES if req.adaptive else 0))
+                entry = self.resident[req.key]
+                entry.users += 1
+                self.resident.move_to_end(req.key)
+                req.slot = slot
+                self.active[req.rid] = req
+                self.work[slot].append(req)
+                self.cv.notify_all()
+                progress = True
+
+    def _evict(self, amount, keep):
+        """Drops idle resident scenes, least recently used first, until
+        `amount` bytes are free; False (and nothing dropped) if it cannot."""
+        idle = [s for k, s in self.resident.items() if s.users == 0 and k != keep and s.ready.is_set()]
+        if sum(s.bytes for s in idle) < amount:
+            return False
+        freed = 0
+        for s in idle:
+            if freed >= amount:
+                break
+            del self.resident[s.key]
+            self.backend.release(s.key)
+            self.evictions += 1
+            freed += s.bytes
+        return True
+
+    # -------------------------------------------------------------- slots
+    def _slot_loop(self, k):
+        while True:
+            with self.cv:
+                while self.running and not self.work[k]:
+                    self.cv.wait()
+                if not self.work[k]:
+                    return
+                req = self.work[k].popleft()
+                entry = self.resident[req.key]
+                install = not entry.installing and not entry.ready.is_set()
+                if install:
+                    entry.installing = True
+            try:
+                if install:
+                    try:
+                        nbytes = self.backend.install(req.key, entry.scene)
+                        with self.cv:
+                            entry.bytes = nbytes
+                    except Exception as e:  # the install failed: every request of the scene fails with it
+                        entry.error = e
+                    finally:
+                        entry.ready.set()
+                entry.ready.wait()
+                if entry.error is not None:
+                    raise entry.error
+                req.started = time.perf_counter()
+                req.film, req.stats = self.backend.render(k, req.key, entry.scene, req.width, req.height, req.spp,
+                                                          req.seed, req.adaptive)
+            except Exception as e:
+                req.error = e
+            req.finished = time.perf_counter()
+            with self.cv:
+                entry.users -= 1
+                if entry.error is not None and entry.users == 0 and self.resident.get(req.key) is entry:
+                    del self.resident[req.key]
+                del self.active[req.rid]
+                self.finished.append(req)
+                self.free.append(k)
+                self._admit()
+                self.cv.notify_all()
+            req.done.set()
+
+
+class GpuBackend:
+    """Installs and renders through vork_amd.native: one HIP stream per slot,
+    one vh_scene per (scene, slot, resolution) over the scene's resident
+    BLASes and TLAS, device films."""
+
+    def __init__(self, ordinal=0, slots=4):
+        import torch
+        from . import native
+        self.torch, self.native = torch, native
+        self.ordinal = ordinal
+        self.renderer = native.Renderer(ordinal)
+        self.streams = [torch.cuda.Stream(device=ordinal) for _ in range(slots)]
+        props = torch.cuda.get_device_properties(ordinal)
+        self.total = props.total_memory
+        self.cus = props.multi_processor_count
+        self.scenes = {}
+        self.lock = threading.Lock()
+
+    def memory_budget(self):
+        return int(0.85 * self.total)
+
+    def slot_base_bytes(self):
+        # traversal overflow stacks of one stream: 8 workgroups per CU x 256 lanes x 244 8-byte entries
+        return self.cus * 8 * 256 * 244 * 8
+
+    def state_bytes(self, width, height, spp):
+        return batch_paths(width, height, spp) * PATH_BYTES
+
+    def scene_bytes(self, scene):
+        tris = scene.triangle_count if not isinstance(scene.instances, np.ndarray) else \
+            sum(m.triangle_count for m in scene.meshes)
+        verts = sum(m.positions.shape[0] for m in scene.meshes)
+        n_inst = len(scene.instances)
+        # leaf-order triangles (36 B) + ids (4) + shading copies (index 12, material 4) + vertices (24) + nodes
+        return tris * (36 + 4 + 16 + 20) + verts * 24 + n_inst * 200 + (1 << 20)
+
+    def install(self, key, scene):
+        native = self.native
+        blases = [native.Blas(self.renderer.device, m.positions, m.indices, m.normals, m.triangle_materials)
+                  for m in scene.meshes]
+        tlas = native.Tlas(self.renderer.device, blases, scene.instance_array())
+        with self.lock:
+            self.scenes[key] = {"blases": blases, "tlas": tlas, "views": {}}
+        return self.scene_bytes(scene)
+
+    def release(self, key):
+        with self.lock:
+            entry = self.scenes.pop(key, None)
+        if entry:
+            for v in entry["views"].values():
+                v.close()
+            entry["tlas"].close()
+            for b in entry["blases"]:
+                b.close()
+
+    def close(self):
+        """Releases every resident scene and the device."""
+        for key in list(self.scenes):
+            self.release(key)
+        self.renderer.device.close()
+
+    def _view(self, slot, key, scene, width, height):
+        import copy
+        with self.lock:
+            entry = self.scenes[key]
+            v = entry["views"].get((slot, width, height))
+        if v is None:
+            sc = copy.copy(scene)
+            sc.width, sc.height = width, height
+            v = self.native.InstalledScene(self.renderer.device, sc, entry["blases"], entry["tlas"])
+            with self.lock:
+                entry["views"][(slot, width, height)] = v
+        return v
+
+    def render(self, slot, key, scene, width, height, spp, seed, adaptive):
+        torch = self.torch
+        view = self._view(slot, key, scene, width, height)
+        p = view.scene.render_params(samples=spp, seed=seed, adaptive=adaptive)
+        stream = self.streams[slot]
+        with torch.cuda.stream(stream):
+            film = torch.zeros((height, width, 4), dtype=torch.float32, device=f"cuda:{self.ordinal}")
+            view.render(p, device_film=film, stream=stream.cuda_stream)
+            out = film.cpu().numpy()
+        return out, view.stats()
