@@ -322,6 +322,37 @@ void vh_render_params_default(vh_render_params* params, uint32_t width, uint32_t
  * written back. */
 int vh_render(vh_scene* scene, const vh_render_params* params, uint32_t flags, void* stream, float* film);
 
+/* vh_render that also ADDS the denoiser guides of every sample's camera hit
+ * to `guides` (width*height*8 floats, row-major like the film): per pixel
+ * albedo rgb + coverage, then shading normal xyz + depth t/(t+1), 1 on a
+ * miss.  These are path_guides (path.slang:1553-1575,1956-1961) with the
+ * depth of field_depth (path.slang:1864,1883) in the normal's fourth
+ * channel; a dropped sample adds none.  Replaces the guide writes the
+ * denoise pass reads (crates/vork-render/src/denoise.rs). */
+int vh_render_guides(vh_scene* scene, const vh_render_params* params, uint32_t flags, void* stream, float* film,
+                     float* guides);
+
+/* ---------------------------------------------------------------- denoiser */
+
+/* The learned denoiser that replaces the OIDN pass of
+ * crates/vork-render/src/denoise.rs (its guides are vh_render_guides'):
+ * a 5-level U-Net of 3x3 convolutions (32, 64, 64, 96, 128 channels),
+ * bf16 weights and activations on MFMA with fp32 accumulation.  `weights`
+ * is the version-1 blob vork_amd/denoise.py exports ("VHDN", version,
+ * layer count, then per layer cin, cout, bf16 weights [tap][cout][cin] and
+ * fp32 biases); it is copied to the device. */
+typedef struct vh_denoiser vh_denoiser;
+int vh_denoiser_create(vh_device* device, const void* weights, size_t bytes, vh_denoiser** out);
+
+/* Denoises one image: `film` (width*height*4 floats) and `guides`
+ * (width*height*8) are the sums of `samples` samples from vh_render_guides;
+ * `out` (width*height*4) receives the denoised mean radiance with alpha 1.
+ * Pointers are device pointers with VH_DEVICE_POINTERS.  Calls on one
+ * denoiser wait for each other (it owns its activation scratch). */
+int vh_denoise(vh_denoiser* denoiser, uint32_t width, uint32_t height, float samples, const float* film,
+               const float* guides, uint32_t flags, void* stream, float* out);
+void vh_denoiser_release(vh_denoiser* denoiser);
+
 /* Kernel statistics of the last vh_render on this scene. */
 typedef struct vh_render_stats {
     uint64_t paths;             /* traced paths: stopped adaptive pixels trace none */

@@ -77,6 +77,7 @@ def lib():
     L.vo_rect_eval.argtypes = [vp, vp, vp, vp]
     L.vo_sample_radiance.argtypes = [vp, vp, u32, u32, u32, vp]
     L.vo_render.argtypes = [vp, vp, vp, vp, C.c_int]
+    L.vo_render_guides.argtypes = [vp, vp, vp, vp, vp, C.c_int]
     L.vo_camera_ray.argtypes = [vp, vp, u32, u32, u32, vp]
     L.vo_render_stats.argtypes = [vp, vp]
     _lib = L
@@ -148,6 +149,14 @@ class OracleScene:
         lib().vo_render(self.handle, C.byref(params), ptr(f32), ptr(f64), threads)
         shape = (params.height, params.width, 4)
         return (f32.reshape(shape) if f32 is not None else None, f64.reshape(shape) if f64 is not None else None)
+
+    def render_guides(self, params, threads=0):
+        """(film32 (H, W, 4), guides (H, W, 2, 4)): sums over the sample range."""
+        n = params.width * params.height
+        f32 = np.zeros(n * 4, np.float32)
+        g = np.zeros(n * 8, np.float32)
+        lib().vo_render_guides(self.handle, C.byref(params), ptr(f32), None, ptr(g), threads)
+        return f32.reshape(params.height, params.width, 4), g.reshape(params.height, params.width, 2, 4)
 
     def stats(self):
         out = np.zeros(4, np.uint64)

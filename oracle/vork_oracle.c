@@ -1064,6 +1064,24 @@ static v3 metal_compensation(const closure_t* c, v3 view) {
               missing * avg.y * e_avg / fmaxf(1.f - avg.y * (1.f - e_avg), 1e-6f),
               missing * avg.z * e_avg / fmaxf(1.f - avg.z * (1.f - e_avg), 1e-6f));
 }
+/* Denoiser guide albedo (path.slang:1553-1570) over this closure subset:
+ * (1 - metal) ((1 - E_spec) base + E_spec) + metal (F0 m.x + m.y)(1 + comp)
+ * spec_weight, clamped to [0, 1]; E_spec is the dielectric table at the
+ * modulated IOR, m the GGX moments at the view cosine. */
+static v3 guide_albedo(const closure_t* c, v3 view) {
+    float nv = fmaxf(dot(c->n, view), .0001f);
+    float ed = c->metal < 1.f ? vo_dielectric_albedo(nv, c->roughness, dielectric_ior(c)) : 0.f;
+    v3 a = add(scl(c->base_color, (1.f - c->metal) * (1.f - ed)), scl(mk(ed, ed, ed), 1.f - c->metal));
+    if (c->metal > 0.f) {
+        float m[4];
+        vo_brdf_moments(nv, c->roughness, m);
+        v3 comp = metal_compensation(c, view);
+        v3 e = scl(add(scl(c->base_color, m[0]), mk(m[1], m[1], m[1])), c->spec_weight);
+        a = add(a, scl(mk(e.x * (1.f + comp.x), e.y * (1.f + comp.y), e.z * (1.f + comp.z)), c->metal));
+    }
+    return mk(clampf(a.x, 0.f, 1.f), clampf(a.y, 0.f, 1.f), clampf(a.z, 0.f, 1.f));
+}
+
 /* bsdf_response.slang:98-110 */
 static v3 direct_diffuse(const closure_t* c, v3 view) {
     if (c->metal < 1.f) {
@@ -1513,9 +1531,12 @@ static v3 next_event(const vo_scene* s, const vh_render_params* p, const closure
 }
 
 /* One sample's RGBA value: (radiance, 1), or zero when non-finite
- * (path.slang:1948-1952). */
+ * (path.slang:1948-1952).  With `guide` (8 floats) also the camera hit's
+ * denoiser guides: albedo, 1, shading normal, depth t / (t + 1); a miss
+ * keeps zeros and depth 1, a non-finite sample zeros them all. */
 static v3 sample_radiance(const vo_scene* s, const vh_render_params* p, uint32_t px, uint32_t py, uint32_t index,
-                          counters_t* cnt, float* alpha) {
+                          counters_t* cnt, float* alpha, float* guide) {
+    if (guide) { memset(guide, 0, 8 * sizeof(float)); guide[7] = 1.f; }
     sampler_t smp = make_sampler(px, py, index, p->samples, p->seed);
     uint32_t rng = vo_screen_ray_hash(px, py, index, p->seed);
     float cs[4];
@@ -1538,6 +1559,11 @@ static v3 sample_radiance(const vo_scene* s, const vh_render_params* p, uint32_t
         surface_t sf = reconstruct(s, &h, direction);
         const vh_material* mat = &s->materials[sf.material];
         closure_t c = make_closure(mat, sf.smooth);
+        if (guide && vertex == 0) {
+            v3 a = guide_albedo(&c, neg(direction));
+            guide[0] = a.x; guide[1] = a.y; guide[2] = a.z; guide[3] = 1.f;
+            guide[4] = sf.smooth.x; guide[5] = sf.smooth.y; guide[6] = sf.smooth.z; guide[7] = h.t / (h.t + 1.f);
+        }
         v3 error = surface_position_error(s, h.inst, sf.world);
         float scatter[4];
         group_values(smp, 1u + vertex * 5u + 0u, scatter);
@@ -1572,17 +1598,25 @@ static v3 sample_radiance(const vo_scene* s, const vh_render_params* p, uint32_t
     cnt->dropped += dropped;
     if (!finite3(radiance)) nonfinite = 1;
     *alpha = nonfinite ? 0.f : 1.f;
+    if (guide && nonfinite) memset(guide, 0, 8 * sizeof(float));
     return nonfinite ? mk(0.f, 0.f, 0.f) : radiance;
 }
 
 void vo_sample_radiance(const vo_scene* s, const vh_render_params* p, uint32_t px, uint32_t py, uint32_t index, float out[4]) {
     counters_t cnt = {0, 0, 0};
     float alpha;
-    v3 r = sample_radiance(s, p, px, py, index, &cnt, &alpha);
+    v3 r = sample_radiance(s, p, px, py, index, &cnt, &alpha, NULL);
     out[0] = r.x; out[1] = r.y; out[2] = r.z; out[3] = alpha;
 }
 
 void vo_render(const vo_scene* s, const vh_render_params* p, float* film32, double* film64, int threads) {
+    vo_render_guides(s, p, film32, film64, NULL, threads);
+}
+
+/* vo_render that also sums every sample's guides in sample order into
+ * `guides32` (8 floats per pixel) when it is not NULL. */
+void vo_render_guides(const vo_scene* s, const vh_render_params* p, float* film32, double* film64, float* guides32,
+                      int threads) {
     int64_t pixels = (int64_t)p->width * p->height;
     uint64_t ext = 0, shadow = 0, dropped = 0;
     vo_scene* ms = (vo_scene*)s;
@@ -1601,8 +1635,10 @@ void vo_render(const vo_scene* s, const vh_render_params* p, float* film32, doub
         double acc64[4] = {0, 0, 0, 0};
         if (film32) memcpy(acc, film32 + 4 * i, sizeof(acc));
         for (uint32_t k = p->sample_begin; k < p->sample_end; k++) {
-            float alpha;
-            v3 r = sample_radiance(s, p, px, py, k, &cnt, &alpha);
+            float alpha, g[8];
+            v3 r = sample_radiance(s, p, px, py, k, &cnt, &alpha, guides32 ? g : NULL);
+            if (guides32)
+                for (int q = 0; q < 8; q++) guides32[8 * i + q] += g[q];
             if (film32) { acc[0] += r.x; acc[1] += r.y; acc[2] += r.z; acc[3] += alpha; }
             acc64[0] += r.x; acc64[1] += r.y; acc64[2] += r.z; acc64[3] += alpha;
         }

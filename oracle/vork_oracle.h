@@ -107,6 +107,8 @@ void vo_sample_radiance(const vo_scene* scene, const vh_render_params* params, u
  * `film32` (RGBA, float, summed in sample order as the GPU film is) and
  * `film64` (RGBA, double); either may be NULL.  threads <= 0: all cores. */
 void vo_render(const vo_scene* scene, const vh_render_params* params, float* film32, double* film64, int threads);
+void vo_render_guides(const vo_scene* scene, const vh_render_params* params, float* film32, double* film64,
+                      float* guides32, int threads);
 /* Camera ray of one sample: out = [origin xyz, tmax, dir xyz, 0]. */
 void vo_camera_ray(const vo_scene* scene, const vh_render_params* params, uint32_t px, uint32_t py,
                    uint32_t index, float out[8]);

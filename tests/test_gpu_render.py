@@ -249,3 +249,17 @@ def test_objects_keep_their_device_open_in_any_release_order():
         b.close()
     inst.tlas.close()
     inst.close()
+
+
+@pytest.mark.parametrize("make,spp", [(lambda: S.config0(width=48, height=40), 4),
+                                      (lambda: S.small_config1(width=64, height=36), 3)])
+def test_denoiser_guides_match_the_oracle(renderer, make, spp):
+    sc = make()
+    inst = renderer.install(sc)
+    p = sc.render_params(samples=spp, seed=5, adaptive=False)
+    film, g = inst.render_guides(p)
+    assert np.array_equal(film, inst.render(p))     # guides leave the film as it is
+    _, go = oracle.OracleScene(sc).render_guides(p)
+    assert np.array_equal(g[..., 0, 3], go[..., 0, 3])   # the same camera hits
+    close = np.abs(g - go) <= 1e-4 + 1e-4 * np.abs(go)
+    assert close.mean() > 0.999, close.mean()

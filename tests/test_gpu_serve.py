@@ -14,9 +14,10 @@ pytestmark = pytest.mark.gpu
 def requests():
     scenes = {"c0": S.config0(width=64, height=48, spp=4), "c0b": S.config0(seed=2, width=48, height=48, spp=4),
               "g": S.small_config1(width=96, height=54, spp=4)}
-    mix = [("c0", 64, 48, 4, 1, False), ("g", 96, 54, 8, 2, False), ("c0b", 32, 32, 1, 3, False),
-           ("c0", 128, 72, 2, 4, False), ("g", 48, 27, 16, 5, False), ("c0b", 80, 60, 4, 6, False),
-           ("c0", 64, 48, 24, 7, True), ("g", 96, 54, 20, 8, True), ("c0", 64, 48, 4, 1, False)]
+    mix = [("c0", 64, 48, 4, 1, False, False), ("g", 96, 54, 8, 2, False, False), ("c0b", 32, 32, 1, 3, False, False),
+           ("c0", 128, 72, 2, 4, False, False), ("g", 48, 27, 16, 5, False, False), ("c0b", 80, 60, 4, 6, False, False),
+           ("c0", 64, 48, 24, 7, True, False), ("g", 96, 54, 20, 8, True, False), ("c0", 64, 48, 4, 1, False, False),
+           ("g", 96, 54, 4, 9, False, True), ("c0b", 80, 60, 4, 10, False, True)]
     return scenes, mix
 
 
@@ -24,7 +25,8 @@ def run(backend, slots, budget=None):
     scenes, mix = requests()
     s = serve.Server(backend, slots=slots, budget=budget)
     try:
-        reqs = [s.submit(k, scenes[k], w, h, spp, seed, adaptive) for k, w, h, spp, seed, adaptive in mix]
+        reqs = [s.submit(k, scenes[k], w, h, spp, seed, adaptive, denoise)
+                for k, w, h, spp, seed, adaptive, denoise in mix]
         s.wait_all(120)
         return [r.result() for r in reqs], s.report()
     finally:
@@ -42,14 +44,15 @@ def test_results_are_the_same_alone_and_in_the_mix():
         for a, m in zip(alone, mixed):
             assert np.isfinite(a).all() and a[..., :3].mean() > 0
             assert np.array_equal(a, m)
-        assert np.array_equal(alone[0], alone[-1])  # a repeat request of a resident scene
+        assert (alone[-1][..., 3] == 1).all()   # denoised requests return the mean radiance
+        assert np.array_equal(alone[0], alone[8])  # a repeat request of a resident scene
         # every scene counted as 1 GiB against a budget with room for one and a
         # half beside the largest films and scratch: one scene resident at a time
         for k in list(backend.scenes):
             backend.release(k)
         scenes, mix = requests()
-        state = max(backend.state_bytes(w, h, spp) for _, w, h, spp, _, _ in mix)
-        film = max(w * h * (serve.FILM_BYTES + serve.MOMENT_BYTES) for _, w, h, _, _, _ in mix)
+        state = max(backend.state_bytes(w, h, spp) for _, w, h, spp, _, _, _ in mix)
+        film = max(w * h * (serve.FILM_BYTES + serve.MOMENT_BYTES + serve.DENOISE_BYTES) for _, w, h, *_ in mix)
         backend.scene_bytes = lambda scene: 1 << 30
         tight = 4 * (backend.slot_base_bytes() + state + film) + (3 << 29)
         evicted, rep = run(backend, 4, budget=tight)

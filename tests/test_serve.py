@@ -49,7 +49,7 @@ class FakeBackend:
         with self.lock:
             self.releases.append(key)
 
-    def render(self, slot, key, scene, width, height, spp, seed, adaptive):
+    def render(self, slot, key, scene, width, height, spp, seed, adaptive, denoise=False):
         with self.lock:
             self.running += 1
             self.peak = max(self.peak, self.running)

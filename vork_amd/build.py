@@ -16,7 +16,7 @@ ARCH = os.environ.get("VORK_HIP_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-function",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-munsafe-fp-atomics"]
-SOURCES = ["vh_api.hip", "vh_build.hip", "vh_trace.hip", "vh_render.hip"]
+SOURCES = ["vh_api.hip", "vh_build.hip", "vh_trace.hip", "vh_render.hip", "vh_denoise.hip"]
 
 
 def _compile(src):

@@ -519,6 +519,11 @@ void vh_render_params_default(vh_render_params* p, uint32_t width, uint32_t heig
 }
 
 int vh_render(vh_scene* sc, const vh_render_params* p, uint32_t flags, void* stream, float* film) {
+    return vh_render_guides(sc, p, flags, stream, film, nullptr);
+}
+
+int vh_render_guides(vh_scene* sc, const vh_render_params* p, uint32_t flags, void* stream, float* film,
+                     float* guides) {
     clear();
     if (!sc || !p || !film) return fail(VH_ERR_USAGE, "vh_render: null argument");
     if (p->width == 0 || p->height == 0) return fail(VH_ERR_USAGE, "vh_render: empty image");
@@ -531,14 +536,19 @@ int vh_render(vh_scene* sc, const vh_render_params* p, uint32_t flags, void* str
     VH_HIP(hipSetDevice(dev->ordinal));
     hipStream_t s = pick(dev, stream);
     std::lock_guard<std::mutex> hold(sc->render_lock);
-    if (flags & VH_DEVICE_POINTERS) return render_scene(sc, p, film, s);
+    if (flags & VH_DEVICE_POINTERS) return render_scene(sc, p, film, guides, s);
     size_t bytes = sizeof(float) * 4 * (size_t)p->width * p->height;
-    DeviceBuffer f;
+    DeviceBuffer f, g;
     VH_HIP(f.allocate(bytes));
     VH_HIP(hipMemcpyAsync(f.ptr, film, bytes, hipMemcpyHostToDevice, s));
-    int rc = render_scene(sc, p, f.as<float>(), s);
+    if (guides) {
+        VH_HIP(g.allocate(2 * bytes));
+        VH_HIP(hipMemcpyAsync(g.ptr, guides, 2 * bytes, hipMemcpyHostToDevice, s));
+    }
+    int rc = render_scene(sc, p, f.as<float>(), guides ? g.as<float>() : nullptr, s);
     if (rc) return rc;
     VH_HIP(hipMemcpyAsync(film, f.ptr, bytes, hipMemcpyDeviceToHost, s));
+    if (guides) VH_HIP(hipMemcpyAsync(guides, g.ptr, 2 * bytes, hipMemcpyDeviceToHost, s));
     VH_HIP(hipStreamSynchronize(s));
     return VH_OK;
 }

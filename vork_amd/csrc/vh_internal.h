@@ -189,7 +189,7 @@ int launch_trace(vh_device* dev, hipStream_t s, const TlasView& tv, const void* 
 int bake_luts(vh_device* dev);
 
 // --------------------------------------------------------------- render
-int render_scene(vh_scene* scene, const vh_render_params* p, float* film, hipStream_t s);
+int render_scene(vh_scene* scene, const vh_render_params* p, float* film, float* guides, hipStream_t s);
 // The scratch of stream s on this device, created on first use.
 StreamState* stream_state(vh_device* dev, hipStream_t s);
 

@@ -47,6 +47,8 @@ struct Paths {
     float4* pend;     // throughput x estimator 0 (rgb), clamp limit
     float4* pend1;    // throughput x estimator 1 (rgb)
     uint8_t* vis;     // per shadow ray (2 per path): 1 = occluded
+    float4* guide_a;  // denoiser guides of the camera hit: albedo, 1 (coverage)
+    float4* guide_n;  //   shading normal, depth t / (t + 1)
 };
 
 // Ray queues hold 32-byte records, float4 pairs: [2i] origin and tmax,
@@ -68,6 +70,7 @@ struct Consts {
     uint32_t batch_first, batch_samples, pixels;
     uint32_t minimum, decide;   // adaptive: first tested sample index; batch follows a decide pass
     float threshold;
+    uint32_t guides;            // write the denoiser guides
     vh_camera camera;
     vh_sky sky;
 };
@@ -205,6 +208,10 @@ __global__ void k_camera(Paths P, Consts c, Scene S, uint32_t n, float4* rays, u
     P.thr[p] = make_float4(1.f, 1.f, 1.f, 0.f);
     P.rad[p] = make_float4(0.f, 0.f, 0.f, 0.f);
     P.info[p] = make_uint4(pixel, index, screen_ray_hash(px, py, index, c.seed), VH_RAY_CAMERA << kEventShift);
+    if (c.guides) {  // a camera ray that misses keeps zero guides and depth 1
+        P.guide_a[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+        P.guide_n[p] = make_float4(0.f, 0.f, 0.f, 1.f);
+    }
 }
 
 // Sums per-thread visit counts of a block into two u64 counters.
@@ -379,6 +386,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VH_SHADE
         const f3 view = -direction;
         const f3 interface_face = dot(sf.smooth, view) > 0.f ? sf.smooth : sf.face;
         Prep pr = transport_prepare(S.luts, cl, view, sf.smooth, interface_face);
+        if (c.guides && bounce == 0) {
+            const f3 a = guide_albedo(S.luts, cl, view);
+            P.guide_a[p] = make_float4(a.x, a.y, a.z, 1.f);
+            P.guide_n[p] = make_float4(sf.smooth.x, sf.smooth.y, sf.smooth.z, h.x / (h.x + 1.f));
+        }
         BsdfSample bs = transport_sample(cl, view, scatter[0], scatter[1], scatter[2], pr);
         if (!finite3(bs.direction) || !finite3(bs.weight) || !isfinite(bs.pdf)) {
             P.info[p].w = flags | kNonFinite;
@@ -559,13 +571,22 @@ __global__ void k_resolve(Paths P, const uint32_t* queue, uint32_t* counters) {
 // (consecutive slots hold an 8x8 pixel tile), so the path-state reads are
 // contiguous.  Adaptive batches also update the pixel's moment, add a stopped
 // pixel's mean in place of a sample, and record the settled test of sample
-// indices up to `minimum` (later ones are the decide pass's).
+// indices up to `minimum` (later ones are the decide pass's).  With `guides`
+// it also sums the camera hits' guides (path.slang:1956-1961): albedo and
+// coverage at [2 pixel], normal and depth at [2 pixel + 1]; a dropped sample
+// adds none, and a stopped pixel's sums grow by (i + 1) / i so they stay
+// sums over every sample (path.slang:1806-1809).
 template <bool kAdaptive>
-__global__ void k_film(Paths P, Consts c, float4* film, Moment* M, uint32_t* counters) {
+__global__ void k_film(Paths P, Consts c, float4* film, float4* guides, Moment* M, uint32_t* counters) {
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= c.pixels) return;
     const uint32_t pixel = slot_to_pixel(slot, c.width, c.height);
     float4 acc = film[pixel];
+    float4 g0 = make_float4(0.f, 0.f, 0.f, 0.f), g1 = g0;
+    if (guides) {
+        g0 = guides[2 * pixel];
+        g1 = guides[2 * pixel + 1];
+    }
     Moment m;
     if (kAdaptive) m = M[pixel];
     uint32_t bad_count = 0, stopped_count = 0;
@@ -582,6 +603,11 @@ __global__ void k_film(Paths P, Consts c, float4* film, Moment* M, uint32_t* cou
                 acc.y += m.mean.y;
                 acc.z += m.mean.z;
                 acc.w += m.mean.w;
+                if (guides) {
+                    const float grow = (float)(index + 1u) / (float)index;
+                    g0 = make_float4(g0.x * grow, g0.y * grow, g0.z * grow, g0.w * grow);
+                    g1 = make_float4(g1.x * grow, g1.y * grow, g1.z * grow, g1.w * grow);
+                }
                 stopped_count++;
                 continue;
             }
@@ -597,9 +623,18 @@ __global__ void k_film(Paths P, Consts c, float4* film, Moment* M, uint32_t* cou
         acc.y += v.y;
         acc.z += v.z;
         acc.w += v.w;
+        if (guides && !bad) {
+            const float4 a = P.guide_a[p], n = P.guide_n[p];
+            g0 = make_float4(g0.x + a.x, g0.y + a.y, g0.z + a.z, g0.w + a.w);
+            g1 = make_float4(g1.x + n.x, g1.y + n.y, g1.z + n.z, g1.w + n.w);
+        }
         if (kAdaptive) moment_add(m, v, index);
     }
     film[pixel] = acc;
+    if (guides) {
+        guides[2 * pixel] = g0;
+        guides[2 * pixel + 1] = g1;
+    }
     if (kAdaptive) M[pixel] = m;
     if (bad_count) atomicAdd(&counters[9], bad_count);
     if (stopped_count) atomicAdd((unsigned long long*)&counters[10], (unsigned long long)stopped_count);
@@ -709,7 +744,7 @@ int bake_luts(vh_device* dev) {
 
 int ensure_overflow(DeviceBuffer& buf, uint32_t threads);
 
-int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, hipStream_t s) {
+int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, float* guides, hipStream_t s) {
     vh_device* dev = sc->device;
     const uint32_t pixels = rp->width * rp->height;
     const uint32_t total = rp->sample_end - rp->sample_begin;
@@ -727,7 +762,7 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, hipStrea
     StreamState* st = stream_state(dev, s);
     std::lock_guard<std::mutex> hold(st->lock);
     if (st->capacity < cap) {
-        VH_HIP(st->paths.allocate(cap * (sizeof(float4) * 6 + sizeof(uint32_t) + 2)));
+        VH_HIP(st->paths.allocate(cap * (sizeof(float4) * 8 + sizeof(uint32_t) + 2)));
         VH_HIP(st->rays_a.allocate(cap * 2 * sizeof(float4)));
         VH_HIP(st->rays_b.allocate(cap * 2 * sizeof(float4)));
         VH_HIP(st->shadow_rays.allocate(2 * cap * 2 * sizeof(float4)));
@@ -762,7 +797,7 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, hipStrea
     Paths P;
     {
         char* base = st->paths.as<char>();
-        float4** f4[] = {&P.thr, &P.rad, (float4**)&P.info, &P.hit, &P.pend, &P.pend1};
+        float4** f4[] = {&P.thr, &P.rad, (float4**)&P.info, &P.hit, &P.pend, &P.pend1, &P.guide_a, &P.guide_n};
         size_t off = 0;
         for (auto* q : f4) {
             *q = reinterpret_cast<float4*>(base + off);
@@ -792,6 +827,7 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, hipStrea
     c.pixels = pixels;
     c.minimum = minimum;
     c.threshold = rp->adaptive_threshold > 0.f ? rp->adaptive_threshold : 0.01f;
+    c.guides = guides != nullptr;
     c.camera = sc->camera;
     c.sky = sc->sky;
     const uint32_t grid = std::min<uint32_t>(blocks(cap, kBlock), (uint32_t)dev->cu_count * 8u);
@@ -860,12 +896,13 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, hipStrea
             hipLaunchKernelGGL(k_swap_counts, dim3(1), dim3(64), 0, s, counters);
             std::swap(qa, qb);
         }
+        float4* gf = reinterpret_cast<float4*>(guides);
         if (adaptive)
             hipLaunchKernelGGL(k_film<true>, dim3(blocks(pixels, kBlock)), dim3(kBlock), 0, s, P, c,
-                               reinterpret_cast<float4*>(film), moments, counters);
+                               reinterpret_cast<float4*>(film), gf, moments, counters);
         else
             hipLaunchKernelGGL(k_film<false>, dim3(blocks(pixels, kBlock)), dim3(kBlock), 0, s, P, c,
-                               reinterpret_cast<float4*>(film), moments, counters);
+                               reinterpret_cast<float4*>(film), gf, moments, counters);
         VH_HIP(hipGetLastError());
         first += nb;
     }

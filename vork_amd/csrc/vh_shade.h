@@ -255,6 +255,23 @@ __device__ __forceinline__ f3 direct_diffuse(const Luts& L, const Closure& c, f3
     }
     return mk(0.f, 0.f, 0.f);
 }
+// Denoiser guide albedo (path.slang:1553-1570): the closure's directional
+// albedo toward the viewer, diffuse (1 - metal)(1 - E_spec) base plus the
+// specular reflectance of the same GGX tables (dielectric: E_spec at the
+// modulated IOR; metal: F0 * m.x + m.y with the multiple-scattering
+// compensation, times the specular weight), clamped to [0, 1].
+__device__ __forceinline__ f3 guide_albedo(const Luts& L, const Closure& c, f3 view) {
+    const float nv = fmaxf(dot(c.n, view), .0001f);
+    const float ed = c.metal < 1.f ? dielectric_albedo(L, nv, c.roughness, dielectric_ior(c)) : 0.f;
+    f3 a = c.base_color * ((1.f - c.metal) * (1.f - ed)) + mk(ed, ed, ed) * (1.f - c.metal);
+    if (c.metal > 0.f) {
+        const float4 m = brdf_moments(L, nv, c.roughness);
+        const f3 comp = metal_compensation(L, c, view);
+        const f3 e = (c.base_color * m.x + mk(m.y, m.y, m.y)) * c.spec_weight;
+        a = a + mk(e.x * (1.f + comp.x), e.y * (1.f + comp.y), e.z * (1.f + comp.z)) * c.metal;
+    }
+    return mk(clampf(a.x, 0.f, 1.f), clampf(a.y, 0.f, 1.f), clampf(a.z, 0.f, 1.f));
+}
 // crates/vork-render/shaders/bsdf.slang:197-212
 __device__ __forceinline__ f3 valid_interface_normal(f3 normal, f3 geometric, f3 view) {
     float iz = dot(view, geometric);

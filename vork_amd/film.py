@@ -323,7 +323,8 @@ def write_outputs(path, film, view="agx", exposure=0.0, gamma=1.0, lut=None, dep
             json.dump(meta, f, indent=1)
 
 
-def metadata(scene, stats, warnings, samples, wall_ms, camera="camera", profile="std/path", adaptive=True):
+def metadata(scene, stats, warnings, samples, wall_ms, camera="camera", profile="std/path", adaptive=True,
+             denoised=False):
     """The version-3 sidecar of crates/vork-render/src/shot.rs:1324-1331
     restricted to a still of the path integrator."""
     return {
@@ -334,6 +335,7 @@ def metadata(scene, stats, warnings, samples, wall_ms, camera="camera", profile=
         "profile": {"name": profile, "level": 2},
         "settings": {"integrator": "path", "samples": samples, "seed": scene.seed},
         "render_settings": {"profile": {"name": profile, "level": 2}, "samples": samples, "adaptive": adaptive,
-                            "denoise": False, "resolution": [scene.width, scene.height], "frames": 1},
-        "denoiser": None, "wall_ms": wall_ms,
+                            "denoise": denoised, "resolution": [scene.width, scene.height], "frames": 1},
+        "denoiser": {"name": "vork_amd U-Net (bf16 MFMA)", "guides": ["albedo", "normal", "depth"]} if denoised else None,
+        "wall_ms": wall_ms,
     }

@@ -25,7 +25,8 @@ EXPORTS = [
     "vh_abi_version", "vh_last_error", "vh_device_create", "vh_device_release", "vh_device_synchronize",
     "vh_blas_build", "vh_blas_get_info", "vh_blas_release", "vh_tlas_build", "vh_tlas_release",
     "vh_trace_closest", "vh_trace_any", "vh_trace_closest_counts", "vh_scene_create", "vh_scene_set_camera",
-    "vh_scene_release", "vh_render_params_default", "vh_render", "vh_render_get_stats",
+    "vh_scene_release", "vh_render_params_default", "vh_render", "vh_render_guides", "vh_render_get_stats",
+    "vh_denoiser_create", "vh_denoise", "vh_denoiser_release",
 ]
 
 _lib = None
@@ -81,6 +82,14 @@ def lib():
     L.vh_render_params_default.argtypes = [C.POINTER(abi.RenderParams), u32, u32, u32]
     L.vh_render.restype = i32
     L.vh_render.argtypes = [vp, C.POINTER(abi.RenderParams), u32, vp, vp]
+    L.vh_render_guides.restype = i32
+    L.vh_render_guides.argtypes = [vp, C.POINTER(abi.RenderParams), u32, vp, vp, vp]
+    L.vh_denoiser_create.restype = i32
+    L.vh_denoiser_create.argtypes = [vp, vp, C.c_size_t, vp]
+    L.vh_denoise.restype = i32
+    L.vh_denoise.argtypes = [vp, u32, u32, C.c_float, vp, vp, u32, vp, vp]
+    L.vh_denoiser_release.restype = None
+    L.vh_denoiser_release.argtypes = [vp]
     L.vh_render_get_stats.restype = i32
     L.vh_render_get_stats.argtypes = [vp, C.POINTER(abi.RenderStats)]
     _lib = L
@@ -253,6 +262,23 @@ class InstalledScene:
             film = np.zeros((p.height, p.width, 4), np.float32)
         check(lib().vh_render(self.handle, C.byref(p), 0, s, _ptr(film)))
         return film
+
+    def render_guides(self, params=None, film=None, guides=None, stream=None, device=False):
+        """Like render, also adding the denoiser guides: `guides` (H, W, 2, 4)
+        float32 (albedo rgb + coverage, normal xyz + depth).  With device=True
+        both are torch CUDA tensors.  Returns (film, guides)."""
+        p = params or self.scene.render_params()
+        s = C.c_void_p(stream) if stream else None
+        if device:
+            check(lib().vh_render_guides(self.handle, C.byref(p), abi.VH_DEVICE_POINTERS, s, _dev_ptr(film),
+                                         _dev_ptr(guides)))
+            return film, guides
+        if film is None:
+            film = np.zeros((p.height, p.width, 4), np.float32)
+        if guides is None:
+            guides = np.zeros((p.height, p.width, 2, 4), np.float32)
+        check(lib().vh_render_guides(self.handle, C.byref(p), 0, s, _ptr(film), _ptr(guides)))
+        return film, guides
 
     def stats(self):
         s = abi.RenderStats()

@@ -14,8 +14,23 @@ import sys
 import time
 
 
+def render_image(device, inst, scene, params, denoiser=None, stream=None):
+    """The image of a render call: the film sums, or with the scene's
+    `denoise` setting the denoised mean of samples with guides
+    (crates/vork-render/src/denoise.rs).  Returns (image (H, W, 4), stats,
+    denoised)."""
+    if scene.notes.get("denoise"):
+        from .denoise import Denoiser
+        film, guides = inst.render_guides(params, stream=stream)
+        stats = inst.stats()
+        den = denoiser if denoiser is not None else Denoiser(device)
+        return den(film, guides, params.samples), stats, True
+    img = inst.render(params, stream=stream)
+    return img, inst.stats(), False
+
+
 def write_render(scene, outs, img, stats, samples, wall_ms, view="agx", agx_lut=None, warnings=(),
-                 adaptive=True):
+                 adaptive=True, denoised=False):
     """Writes the film to each path of `outs` (+ .meta.json).  Returns the
     sidecar (the same for every output)."""
     from . import film as F
@@ -23,7 +38,7 @@ def write_render(scene, outs, img, stats, samples, wall_ms, view="agx", agx_lut=
     lut = F.load_agx_lut(agx_lut) if agx_lut else None
     if view == "agx" and lut is None and any(o.lower().endswith(".png") for o in outs):
         warnings.append("view agx: analytic AgX (no AgX LUT given); codes differ from the baked view")
-    meta = F.metadata(scene, stats, warnings, samples, wall_ms, adaptive=adaptive)
+    meta = F.metadata(scene, stats, warnings, samples, wall_ms, adaptive=adaptive, denoised=denoised)
     for out in outs:
         F.write_outputs(out, img, view=view, lut=lut, meta=meta)
     return meta
@@ -37,10 +52,9 @@ def render_scene(scene, out, view="agx", agx_lut=None, warnings=()):
     r = native.Renderer(0)
     inst = r.install(scene)
     p = scene.render_params()
-    img = inst.render(p)
-    stats = inst.stats()
+    img, stats, denoised = render_image(r.device, inst, scene, p)
     wall = (time.perf_counter() - t0) * 1e3
-    write_render(scene, [out], img, stats, p.samples, wall, view, agx_lut, warnings)
+    write_render(scene, [out], img, stats, p.samples, wall, view, agx_lut, warnings, denoised=denoised)
     return img, stats
 
 

@@ -44,6 +44,7 @@ from . import abi
 from .scene import Mesh, Scene, vertex_normals
 
 PATH_SAMPLES = 64            # crates/vork-scene/src/v2/constants.rs:11
+PROFILE_DENOISE = True       # libraries/std/profiles/path.json:6
 MAX_RENDER_SAMPLES = 4096    # constants.rs:67
 RESOLUTION = (1920, 1080)    # constants.rs:28
 LOCAL_INTENSITY = 10.0       # constants.rs:53
@@ -495,8 +496,6 @@ def load(source, overrides=None, base_dir=None, max_depth=8):
     samples = int(ov.get("samples", PATH_SAMPLES))
     if not 1 <= samples <= MAX_RENDER_SAMPLES:
         raise SceneError("--set samples", f"1..{MAX_RENDER_SAMPLES}")
-    if ov.get("denoise"):
-        L.warnings.append("denoise: this build writes the undenoised film")
     camera = L.camera(w / h)
     n = len(L.instances)
     inst = np.zeros(n, abi.INSTANCE_DTYPE)
@@ -509,6 +508,8 @@ def load(source, overrides=None, base_dir=None, max_depth=8):
     sky = dict(kind=abi.VH_SKY_CONSTANT, color=L.environment if L.environment is not None else ENVIRONMENT_COLOR)
     sc = Scene(L.meshes, inst, L.materials, L.lights, camera, sky, int(w), int(h), samples, max_depth,
                int(ov.get("seed", 0)), doc.get("name", "scene"))
+    # the std/path profile denoises (libraries/std/profiles/path.json); --set denoise overrides
+    sc.notes["denoise"] = bool(ov.get("denoise", PROFILE_DENOISE))
     return sc, L.warnings
 
 

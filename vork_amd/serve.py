@@ -30,6 +30,7 @@ import numpy as np
 PATH_BYTES = 234                   # path state 102 + extension queues 2x32 + shadow queue 2x32 + pending 4
 FILM_BYTES = 16                    # RGBA32F per pixel
 MOMENT_BYTES = 48                  # adaptive moment per pixel
+DENOISE_BYTES = 32 + 16 + 1100     # guides, denoised output, U-Net activations (bf16, all levels) per pixel
 BATCH_PATHS = 4 << 20              # paths per wavefront batch the library aims for
 MAX_BATCH_PATHS = (1 << 24) - 1    # paths of one batch (24-bit path ids)
 
@@ -42,9 +43,10 @@ def batch_paths(width, height, spp):
 
 
 class Request:
-    def __init__(self, rid, key, scene, width, height, spp, seed, adaptive):
+    def __init__(self, rid, key, scene, width, height, spp, seed, adaptive, denoise=False):
         self.rid, self.key, self.scene = rid, key, scene
         self.width, self.height, self.spp, self.seed, self.adaptive = width, height, spp, seed, adaptive
+        self.denoise = denoise
         self.submitted = time.perf_counter()
         self.started = self.finished = None
         self.slot = None
@@ -99,16 +101,17 @@ class Server:
             t.start()
 
     # -------------------------------------------------------------- API
-    def submit(self, key, scene=None, width=None, height=None, spp=None, seed=None, adaptive=False):
+    def submit(self, key, scene=None, width=None, height=None, spp=None, seed=None, adaptive=False, denoise=False):
         """Queues a render of scene `key` (the Scene object is needed unless
-        the scene is resident or queued).  Returns a Request."""
+        the scene is resident or queued); `denoise` returns the denoised mean
+        radiance instead of the film sums.  Returns a Request."""
         with self.cv:
             known = key in self.resident or any(r.key == key and r.scene is not None for r in self.queue)
             if scene is None and not known:
                 raise KeyError(f"scene {key!r} is not resident; pass the scene")
             base = scene if scene is not None else self._scene_of(key)
             req = Request(self.next_id, key, scene, width or base.width, height or base.height, spp or base.spp,
-                          base.seed if seed is None else seed, adaptive)
+                          base.seed if seed is None else seed, adaptive, denoise)
             self.next_id += 1
             # what it needs with every other request finished and every other scene evicted
             sb, total = self._scene_bytes(req), sum(self.slot_bytes)
@@ -166,10 +169,13 @@ class Server:
         return self.backend.scene_bytes(req.scene if req.scene is not None else self._scene_of(req.key))
 
     def _need(self, req, slot_hw, scene_bytes):
-        pixels = req.width * req.height
-        state = self.backend.state_bytes(req.width, req.height, req.spp)
-        grow = max(0, state - slot_hw)
-        return pixels * (FILM_BYTES + (MOMENT_BYTES if req.adaptive else 0)) + grow + scene_bytes
+        return self._film_bytes(req) + max(0, self.backend.state_bytes(req.width, req.height, req.spp) - slot_hw) \
+            + scene_bytes
+
+    @staticmethod
+    def _film_bytes(req):
+        per = FILM_BYTES + (MOMENT_BYTES if req.adaptive else 0) + (DENOISE_BYTES if req.denoise else 0)
+        return req.width * req.height * per
 
     def _used(self):
         return (self.base_bytes + sum(s.bytes for s in self.resident.values()) + sum(self.slot_bytes) +
@@ -196,7 +202,7 @@ class Server:
                 self.queue.remove(req)
                 self.free.remove(slot)
                 self.slot_bytes[slot] = max(self.slot_bytes[slot], state)
-                req.bytes = req.width * req.height * (FILM_BYTES + (MOMENT_BYTES if req.adaptive else 0))
+                req.bytes = self._film_bytes(req)
                 entry = self.resident[req.key]
                 entry.users += 1
                 self.resident.move_to_end(req.key)
@@ -250,7 +256,7 @@ class Server:
                     raise entry.error
                 req.started = time.perf_counter()
                 req.film, req.stats = self.backend.render(k, req.key, entry.scene, req.width, req.height, req.spp,
-                                                          req.seed, req.adaptive)
+                                                          req.seed, req.adaptive, req.denoise)
             except Exception as e:
                 req.error = e
             req.finished = time.perf_counter()
@@ -283,6 +289,7 @@ class GpuBackend:
         self.cus = props.multi_processor_count
         self.scenes = {}
         self.lock = threading.Lock()
+        self.denoisers = [None] * slots
 
     def memory_budget(self):
         return int(0.85 * self.total)
@@ -322,9 +329,12 @@ class GpuBackend:
                 b.close()
 
     def close(self):
-        """Releases every resident scene and the device."""
+        """Releases every resident scene, the denoisers and the device."""
         for key in list(self.scenes):
             self.release(key)
+        for d in self.denoisers:
+            if d is not None:
+                d.close()
         self.renderer.device.close()
 
     def _view(self, slot, key, scene, width, height):
@@ -340,13 +350,21 @@ class GpuBackend:
                 entry["views"][(slot, width, height)] = v
         return v
 
-    def render(self, slot, key, scene, width, height, spp, seed, adaptive):
+    def render(self, slot, key, scene, width, height, spp, seed, adaptive, denoise=False):
         torch = self.torch
         view = self._view(slot, key, scene, width, height)
         p = view.scene.render_params(samples=spp, seed=seed, adaptive=adaptive)
         stream = self.streams[slot]
+        dev = f"cuda:{self.ordinal}"
         with torch.cuda.stream(stream):
-            film = torch.zeros((height, width, 4), dtype=torch.float32, device=f"cuda:{self.ordinal}")
-            view.render(p, device_film=film, stream=stream.cuda_stream)
-            out = film.cpu().numpy()
-        return out, view.stats()
+            film = torch.zeros((height, width, 4), dtype=torch.float32, device=dev)
+            if not denoise:
+                view.render(p, device_film=film, stream=stream.cuda_stream)
+                return film.cpu().numpy(), view.stats()
+            from .denoise import Denoiser
+            if self.denoisers[slot] is None:
+                self.denoisers[slot] = Denoiser(self.renderer.device)
+            guides = torch.zeros((height, width, 2, 4), dtype=torch.float32, device=dev)
+            view.render_guides(p, film, guides, stream=stream.cuda_stream, device=True)
+            out = self.denoisers[slot](film, guides, spp, stream=stream.cuda_stream)
+            return out.cpu().numpy(), view.stats()

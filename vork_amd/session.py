@@ -198,6 +198,7 @@ class Worker:
         self.installed = None       # dict: blases, tlas, scenes {options key: InstalledScene}, geometry
         self.pending = "reused"     # how the next render obtains its installation
         self.installs = 0
+        self.denoiser = None        # loaded on the first render that denoises
 
     # -------------------------------------------------------------- open
     def _device(self):
@@ -351,7 +352,7 @@ class Worker:
 
     def render(self, req):
         from . import native
-        from .render import write_render
+        from .render import render_image, write_render
         t0 = time.perf_counter()
         if self.open_state is None or self.installed is None:
             raise _usage("render: no scene is open (send open first)")
@@ -379,8 +380,10 @@ class Worker:
         p = scene.render_params()
         tr = time.perf_counter()
         try:
-            img = inst.render(p)
-            stats = inst.stats()
+            if scene.notes.get("denoise") and self.denoiser is None:
+                from .denoise import Denoiser
+                self.denoiser = Denoiser(self.renderer.device)
+            img, stats, denoised = render_image(self.renderer.device, inst, scene, p, self.denoiser)
         except native.VorkError as e:
             raise Failure({2: "usage", 3: "scene", 5: "io"}.get(e.status, "device"), str(e))
         loop_ms = (time.perf_counter() - tr) * 1e3
@@ -391,7 +394,7 @@ class Worker:
         if not no_write:
             try:
                 meta = write_render(scene, outs, img, stats, p.samples, wall, warnings=warnings,
-                                    adaptive=bool(p.flags & 4))
+                                    adaptive=bool(p.flags & 4), denoised=denoised)
             except OSError as e:
                 raise Failure("io", f"write {outs[0]}: {e.strerror or e}")
             sidecars = [o + ".meta.json" for o in outs]
@@ -407,6 +410,9 @@ class Worker:
 
     def close(self):
         self.installed = None
+        if self.denoiser is not None:
+            self.denoiser.close()
+            self.denoiser = None
         self.renderer = None
 
 
