@@ -376,15 +376,25 @@ __device__ bool trace_persistent(const TlasView& tv, const Src& src, uint32_t* n
     bool active = false, exhausted = false, ok = true;
     uint32_t key = 0;
     const uint64_t lt_mask = (1ull << __lane_id()) - 1ull;
+    // Every thread's first ray is its global index, without an atomic (all
+    // waves claiming one counter at once costs about 100 us per launch, and a
+    // launch with fewer rays than threads then claims nothing); refills claim
+    // rays from `first` on.
+    const uint32_t first = gridDim.x * blockDim.x;
+    bool fresh = true;
     while (true) {
         const uint64_t want = __ballot(!active && !exhausted);
         if (want) {
-            const int leader = __ffsll((unsigned long long)want) - 1;
-            uint32_t base = 0;
-            if ((int)__lane_id() == leader) base = atomicAdd(next, (uint32_t)__popcll(want));
-            base = __shfl(base, leader);
+            uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+            if (!fresh) {
+                const int leader = __ffsll((unsigned long long)want) - 1;
+                uint32_t base = 0;
+                if ((int)__lane_id() == leader) base = atomicAdd(next, (uint32_t)__popcll(want));
+                base = __shfl(base, leader);
+                idx = first + base + (uint32_t)__popcll(want & lt_mask);
+            }
+            fresh = false;
             if (!active && !exhausted) {
-                const uint32_t idx = base + (uint32_t)__popcll(want & lt_mask);
                 if (idx < n) {
                     RayData rd;
                     src.fetch(idx, rd);
