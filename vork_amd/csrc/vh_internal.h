@@ -56,7 +56,7 @@ struct DeviceBuffer {
 // each other while calls on different streams overlap.
 struct StreamState {
     std::mutex lock;
-    DeviceBuffer paths, rays_a, rays_b, shadow_rays, pending, counters, overflow, scratch_u32;
+    DeviceBuffer paths, rays_a, rays_b, shadow_rays, counters, overflow, scratch_u32;
     size_t capacity = 0;  // paths the wavefront buffers hold
 };
 

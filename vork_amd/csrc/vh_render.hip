@@ -44,9 +44,9 @@ struct Paths {
     uint4* info;      // pixel, sample index, rng, flags
     float4* hit;      // t, u, v, primitive bits
     uint32_t* hit_inst;
-    float4* pend;     // throughput x estimator 0 (rgb), clamp limit
-    float4* pend1;    // throughput x estimator 1 (rgb)
-    uint8_t* vis;     // per shadow ray (2 per path): 1 = occluded
+    float4* pend;     // per pending next event q: throughput x estimator 0 (rgb), clamp limit
+    float4* pend1;    //   throughput x estimator 1 (rgb), path index bits
+    uint8_t* vis;     // per shadow ray id 2q + k: 1 = occluded
     float4* guide_a;  // denoiser guides of the camera hit: albedo, 1 (coverage)
     float4* guide_n;  //   shading normal, depth t / (t + 1)
 };
@@ -125,14 +125,14 @@ struct Scene {
     Luts luts;
 };
 
-// counters: [0] live count in, [1] live count out, [2] shadow rays queued,
+// counters: [0] live count in, [2] estimator-b shadow rays queued,
 // [3] dropped non-finite contributions, [4..5] extension rays (u64),
 // [6..7] shadow rays (u64), [8] errors, [9] non-finite samples zeroed,
-// [10..11] samples of stopped adaptive pixels (u64), [12] paths with a
-// pending next event, [16..19] extension node / triangle visits, [20..23]
-// shadow node / triangle visits (u64), [24] extension ray claims, [25] shadow
-// ray claims
-constexpr int kExtendClaims = 24, kShadowClaims = 25;
+// [10..11] samples of stopped adaptive pixels (u64), [16..19] extension node /
+// triangle visits, [20..23] shadow node / triangle visits (u64), [24]
+// extension ray claims, [25] shadow ray claims, [28] live count out and [29]
+// pending next events, one u64 so that one atomic reserves both
+constexpr int kExtendClaims = 24, kShadowClaims = 25, kLiveOut = 28, kPending = 29, kShadowB = 2;
 
 __device__ __forceinline__ void reset_claims(uint32_t* counters) {
     counters[kExtendClaims] = 0;
@@ -142,9 +142,9 @@ __device__ __forceinline__ void reset_claims(uint32_t* counters) {
 __global__ void k_open_batch(uint32_t* counters) {
     if (threadIdx.x == 0) {
         counters[0] = 0;
-        counters[1] = 0;
-        counters[2] = 0;
-        counters[12] = 0;
+        counters[kLiveOut] = 0;
+        counters[kShadowB] = 0;
+        counters[kPending] = 0;
         reset_claims(counters);
     }
 }
@@ -181,9 +181,9 @@ __global__ void k_camera(Paths P, Consts c, Scene S, uint32_t n, float4* rays, u
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (!kAdaptive && p == 0) {
         counters[0] = n;
-        counters[1] = 0;
-        counters[2] = 0;
-        counters[12] = 0;
+        counters[kLiveOut] = 0;
+        counters[kShadowB] = 0;
+        counters[kPending] = 0;
         reset_claims(counters);
     }
     if (p >= n) return;
@@ -345,7 +345,7 @@ __device__ __forceinline__ f3 path_origin(const Surface& sf, f3 error, f3 direct
 #define VH_SHADE_WAVES_PER_EU 3
 #endif
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VH_SHADE_WAVES_PER_EU))) void k_shade(Paths P, Consts c, Scene S, const float4* rays, float4* next,
-                                                  float4* shadow_rays, uint32_t* pending_queue, uint32_t* counters) {
+                                                  float4* shadow_rays, float4* shadow_b, uint32_t* counters) {
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, threads = gridDim.x * blockDim.x;
     const uint32_t n = counters[0];
     uint32_t dropped = 0;
@@ -452,16 +452,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VH_SHADE
                 }
             }
         }
-        const bool has_a = nonzero(a), has_b = nonzero(b);
-        if (has_a || has_b) {
-            f3 ta = throughput * a, tb = throughput * b;
-            P.pend[p] = make_float4(ta.x, ta.y, ta.z, limit);
-            P.pend1[p] = make_float4(tb.x, tb.y, tb.z, 0.f);
-            pending_queue[atomicAdd(&counters[12], 1u)] = p;
-            uint32_t slot = atomicAdd(&counters[2], (has_a ? 1u : 0u) + (has_b ? 1u : 0u));
-            if (has_a) put_ray(shadow_rays, slot++, s0o, s0t, s0d, 2u * p);
-            if (has_b) put_ray(shadow_rays, slot, s1o, s1t, s1d, 2u * p + 1u);
-        }
+        const bool has_a = nonzero(a), has_b = nonzero(b), pending = has_a || has_b;
+        const f3 ta = throughput * a, tb = throughput * b;
         // ---- continuation (path.slang:1680-1705)
         bool alive = true;
         if (bounce == c.max_bounces) alive = false;
@@ -495,27 +487,63 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VH_SHADE
         flags = (flags & kNonFinite) | (min(diffuse, 15u) << kDiffuseShift) | (min(glossy, 15u) << kGlossyShift) |
                 ((bounce + 1) << kBounceShift) | (event << kEventShift);
         P.info[p] = make_uint4(info.x, info.y, rng, flags);
+        // One atomic per wave reserves the continuing paths' extension-ray
+        // slots (low word) and the pending next events' slots (high word): each
+        // waits a device-wide round trip, so the wave does one, not three.
+        const uint64_t am = __ballot(alive), pm = __ballot(pending);
+        const uint64_t lt = (1ull << __lane_id()) - 1ull;
+        uint32_t next_slot = 0, q = 0;
+        if (am | pm) {
+            const int leader = __ffsll((unsigned long long)__ballot(1)) - 1;
+            unsigned long long base = 0;
+            if ((int)__lane_id() == leader)
+                base = atomicAdd(reinterpret_cast<unsigned long long*>(&counters[kLiveOut]),
+                                 (unsigned long long)__popcll(am) | ((unsigned long long)__popcll(pm) << 32));
+            base = __shfl(base, leader);
+            next_slot = (uint32_t)base + (uint32_t)__popcll(am & lt);
+            q = (uint32_t)(base >> 32) + (uint32_t)__popcll(pm & lt);
+        }
+        if (pending) {
+            // event q: estimator a is shadow ray q (an empty one, tmax -1, when
+            // only b is pending), estimator b goes to the second queue
+            P.pend[q] = make_float4(ta.x, ta.y, ta.z, limit);
+            P.pend1[q] = make_float4(tb.x, tb.y, tb.z, u2f(p));
+            if (has_a) put_ray(shadow_rays, q, s0o, s0t, s0d, 2u * q);
+            else put_ray(shadow_rays, q, s1o, -1.f, s1d, 2u * q);
+        }
+        const uint64_t bm = __ballot(has_b);
+        if (bm) {
+            const int leader = __ffsll((unsigned long long)__ballot(1)) - 1;
+            uint32_t base = 0;
+            if ((int)__lane_id() == leader) base = atomicAdd(&counters[kShadowB], (uint32_t)__popcll(bm));
+            base = __shfl(base, leader);
+            if (has_b) put_ray(shadow_b, base + (uint32_t)__popcll(bm & lt), s1o, s1t, s1d, 2u * q + 1u);
+        }
         if (alive) {
             f3 origin = path_origin(sf, error, bs.direction);
             P.thr[p] = make_float4(throughput.x, throughput.y, throughput.z, 0.f);
-            put_ray(next, atomicAdd(&counters[1], 1u), origin, FLT_MAX, bs.direction, p | (event << kPathBits));
+            put_ray(next, next_slot, origin, FLT_MAX, bs.direction, p | (event << kPathBits));
         }
     }
     if (dropped) atomicAdd(&counters[3], dropped);
 }
 
-// Shadow rays of the pending next events: entry 2p+k is estimator k of path p.
+// Shadow rays of the pending next events: ray id 2q+k is estimator k of
+// event q.  The first `na` come from the estimator-a queue (slot q = event
+// q), the rest from the estimator-b queue.
 template <bool kCount>
 struct ShadowSource {
     Paths P;
     const float4* rays;
-    uint32_t n;
+    const float4* rays_b;
+    uint32_t na, n;
     uint64_t* nodes;
     uint64_t* tris;
     __device__ uint32_t count() const { return n; }
     __device__ void fetch(uint32_t i, RayData& r) const {
-        r.a = rays[2 * i];
-        r.b = rays[2 * i + 1];
+        const float4* q = i < na ? rays + 2 * (size_t)i : rays_b + 2 * (size_t)(i - na);
+        r.a = q[0];
+        r.b = q[1];
     }
     __device__ void unpack(uint32_t, const RayData& r, f3& o, f3& d, float& t0, float& t1, uint32_t& m,
                            uint32_t& key) const {
@@ -533,14 +561,14 @@ struct ShadowSource {
 };
 
 template <bool kCount>
-VH_TRACE_KERNEL void k_shadow(Paths P, Scene S, const float4* rays, uint32_t* counters,
+VH_TRACE_KERNEL void k_shadow(Paths P, Scene S, const float4* rays, const float4* rays_b, uint32_t* counters,
                                                    uint32_t* overflow) {
     __shared__ uint64_t lds_stack[kLdsStack * kBlock];
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     Stack st = make_stack(lds_stack, reinterpret_cast<uint64_t*>(overflow));
-    const uint32_t n = counters[2];
+    const uint32_t na = counters[kPending], n = na + counters[kShadowB];
     uint64_t nodes = 0, tris = 0;
-    ShadowSource<kCount> src{P, rays, n, &nodes, &tris};
+    ShadowSource<kCount> src{P, rays, rays_b, na, n, &nodes, &tris};
     if (!trace_persistent<true, kCount>(S.tv, src, &counters[kShadowClaims], st))
         atomicOr(&counters[8], kStackOverflow);
     if (gid == 0) atomicAdd((unsigned long long*)&counters[6], (unsigned long long)n);
@@ -549,15 +577,15 @@ VH_TRACE_KERNEL void k_shadow(Paths P, Scene S, const float4* rays, uint32_t* co
 
 // Adds each pending next event: the clamped sum of its unoccluded estimators
 // (sample_clamp.slang:7-16 over path.slang:925-950's value).
-__global__ void k_resolve(Paths P, const uint32_t* queue, uint32_t* counters) {
-    const uint32_t n = counters[12];
+__global__ void k_resolve(Paths P, uint32_t* counters) {
+    const uint32_t n = counters[kPending];
     uint32_t dropped = 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t p = queue[i];
-        const float4 pa = P.pend[p], pb = P.pend1[p];
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const float4 pa = P.pend[q], pb = P.pend1[q];
+        const uint32_t p = f2u(pb.w);
         f3 va = mk(pa.x, pa.y, pa.z), vb = mk(pb.x, pb.y, pb.z);
-        if (nonzero(va) && P.vis[2 * p]) va = mk(0.f, 0.f, 0.f);
-        if (nonzero(vb) && P.vis[2 * p + 1]) vb = mk(0.f, 0.f, 0.f);
+        if (nonzero(va) && P.vis[2 * q]) va = mk(0.f, 0.f, 0.f);
+        if (nonzero(vb) && P.vis[2 * q + 1]) vb = mk(0.f, 0.f, 0.f);
         const f3 add = clamp_sample(va + vb, pa.w, dropped);
         const float4 r = P.rad[p];
         P.rad[p] = make_float4(r.x + add.x, r.y + add.y, r.z + add.z, r.w);
@@ -642,10 +670,10 @@ __global__ void k_film(Paths P, Consts c, float4* film, float4* guides, Moment* 
 
 __global__ void k_swap_counts(uint32_t* counters) {
     if (threadIdx.x == 0) {
-        counters[0] = counters[1];
-        counters[1] = 0;
-        counters[2] = 0;
-        counters[12] = 0;
+        counters[0] = counters[kLiveOut];
+        counters[kLiveOut] = 0;
+        counters[kShadowB] = 0;
+        counters[kPending] = 0;
         reset_claims(counters);
     }
 }
@@ -765,8 +793,7 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, float* g
         VH_HIP(st->paths.allocate(cap * (sizeof(float4) * 8 + sizeof(uint32_t) + 2)));
         VH_HIP(st->rays_a.allocate(cap * 2 * sizeof(float4)));
         VH_HIP(st->rays_b.allocate(cap * 2 * sizeof(float4)));
-        VH_HIP(st->shadow_rays.allocate(2 * cap * 2 * sizeof(float4)));
-        VH_HIP(st->pending.allocate(cap * sizeof(uint32_t)));
+        VH_HIP(st->shadow_rays.allocate(2 * cap * 2 * sizeof(float4)));  // estimator a, then b
         st->capacity = cap;
     }
     VH_HIP(st->counters.reserve(64 * sizeof(uint32_t)));
@@ -869,7 +896,7 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, float* g
         float4* qa = st->rays_a.as<float4>();
         float4* qb = st->rays_b.as<float4>();
         float4* sq = st->shadow_rays.as<float4>();
-        uint32_t* pending = st->pending.as<uint32_t>();
+        float4* sqb = sq + 2 * cap;
         if (adaptive) {
             if (c.decide) hipLaunchKernelGGL(k_decide, dim3(blocks(pixels, kBlock)), dim3(kBlock), 0, s, moments, c, first);
             hipLaunchKernelGGL(k_open_batch, dim3(1), dim3(64), 0, s, counters);
@@ -885,14 +912,14 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, float* g
                 else hipLaunchKernelGGL(k_extend<false>, dim3(grid), dim3(kBlock), 0, s, P, S, qa, counters, overflow);
             });
             if (!rc2) rc2 = timed(1, [&] {
-                hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kBlock), 0, s, P, c, S, qa, qb, sq, pending, counters);
+                hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kBlock), 0, s, P, c, S, qa, qb, sq, sqb, counters);
             });
             if (!rc2) rc2 = timed(2, [&] {
-                if (count) hipLaunchKernelGGL(k_shadow<true>, dim3(grid), dim3(kBlock), 0, s, P, S, sq, counters, overflow);
-                else hipLaunchKernelGGL(k_shadow<false>, dim3(grid), dim3(kBlock), 0, s, P, S, sq, counters, overflow);
+                if (count) hipLaunchKernelGGL(k_shadow<true>, dim3(grid), dim3(kBlock), 0, s, P, S, sq, sqb, counters, overflow);
+                else hipLaunchKernelGGL(k_shadow<false>, dim3(grid), dim3(kBlock), 0, s, P, S, sq, sqb, counters, overflow);
             });
             if (rc2) return rc2;
-            hipLaunchKernelGGL(k_resolve, dim3(grid), dim3(kBlock), 0, s, P, pending, counters);
+            hipLaunchKernelGGL(k_resolve, dim3(grid), dim3(kBlock), 0, s, P, counters);
             hipLaunchKernelGGL(k_swap_counts, dim3(1), dim3(64), 0, s, counters);
             std::swap(qa, qb);
         }
