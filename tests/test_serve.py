@@ -124,6 +124,7 @@ def test_an_install_failure_reaches_every_request_of_its_scene():
 
 
 def test_batch_paths_follow_the_library_sizing():
-    assert serve.batch_paths(1920, 1080, 64) == 1920 * 1080 * 2
+    assert serve.batch_paths(1920, 1080, 64) == 1920 * 1080 * 64
+    assert serve.batch_paths(1920, 1080, 128) == 1920 * 1080 * 64
     assert serve.batch_paths(256, 256, 4) == 256 * 256 * 4
-    assert serve.batch_paths(256, 256, 1024) == 4 << 20
+    assert serve.batch_paths(256, 256, 4096) == 1 << 27

@@ -52,11 +52,20 @@ struct Paths {
 };
 
 // Ray queues hold 32-byte records, float4 pairs: [2i] origin and tmax,
-// [2i+1] direction and an id word.  Extension rays carry path | event << 24
-// (the event of the previous vertex is the ray's visibility mask), shadow
-// rays 2 * path + estimator.  Paths of one batch are therefore limited to
-// 2^24.
-constexpr uint32_t kPathBits = 24, kPathMask = (1u << kPathBits) - 1u;
+// [2i+1] direction and an id word.  Extension rays carry path | code << 30,
+// the code naming the event of the previous vertex (camera, diffuse,
+// specular, transmission), which is the ray's visibility mask; shadow rays
+// carry 2 * event slot + estimator.  Paths of one batch are limited to 2^30.
+constexpr uint32_t kPathBits = 30, kPathMask = (1u << kPathBits) - 1u;
+// path state per path: 8 float4 (throughput, radiance, info, hit, two pending
+// contributions, two guides), the hit instance and two visibility bytes
+constexpr size_t kPathStateBytes = sizeof(float4) * 8 + sizeof(uint32_t) + 2;
+__device__ __forceinline__ uint32_t event_code(uint32_t event) {
+    return event == VH_RAY_DIFFUSE ? 1u : event == VH_RAY_SPECULAR ? 2u : event == VH_RAY_TRANSMISSION ? 3u : 0u;
+}
+__device__ __forceinline__ uint32_t code_mask(uint32_t code) {
+    return code == 0u ? VH_RAY_CAMERA : code == 1u ? VH_RAY_DIFFUSE : code == 2u ? VH_RAY_SPECULAR : VH_RAY_TRANSMISSION;
+}
 __device__ __forceinline__ void put_ray(float4* q, uint32_t i, f3 o, float tmax, f3 d, uint32_t id) {
     q[2 * i] = make_float4(o.x, o.y, o.z, tmax);
     q[2 * i + 1] = make_float4(d.x, d.y, d.z, u2f(id));
@@ -204,7 +213,7 @@ __global__ void k_camera(Paths P, Consts c, Scene S, uint32_t n, float4* rays, u
     float near_distance = cam.clip_start / fd, far_distance = cam.clip_end / fd;
     f3 o = eye + d * near_distance;
     float tmax = fmaxf(0.f, far_distance - length(o - eye));
-    put_ray(rays, kAdaptive ? atomicAdd(&counters[0], 1u) : p, o, tmax, d, p | (VH_RAY_CAMERA << kPathBits));
+    put_ray(rays, kAdaptive ? atomicAdd(&counters[0], 1u) : p, o, tmax, d, p | (event_code(VH_RAY_CAMERA) << kPathBits));
     P.thr[p] = make_float4(1.f, 1.f, 1.f, 0.f);
     P.rad[p] = make_float4(0.f, 0.f, 0.f, 0.f);
     P.info[p] = make_uint4(pixel, index, screen_ray_hash(px, py, index, c.seed), VH_RAY_CAMERA << kEventShift);
@@ -247,7 +256,7 @@ struct ExtendSource {
         t0 = 0.f;
         t1 = r.a.w;
         const uint32_t id = f2u(r.b.w);
-        m = id >> kPathBits;
+        m = code_mask(id >> kPathBits);
         key = id & kPathMask;
     }
     __device__ void store(uint32_t p, const TraceHit& h, const TraceCounts& c) const {
@@ -522,7 +531,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VH_SHADE
         if (alive) {
             f3 origin = path_origin(sf, error, bs.direction);
             P.thr[p] = make_float4(throughput.x, throughput.y, throughput.z, 0.f);
-            put_ray(next, next_slot, origin, FLT_MAX, bs.direction, p | (event << kPathBits));
+            put_ray(next, next_slot, origin, FLT_MAX, bs.direction, p | (event_code(event) << kPathBits));
         }
     }
     if (dropped) atomicAdd(&counters[3], dropped);
@@ -780,22 +789,37 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, float* g
     if (total == 0) return VH_OK;
     uint32_t batch = rp->batch_samples;
     if (batch == 0) {
-        // about 4M paths in flight, whole exposure samples per batch
-        batch = std::max<uint32_t>(1u, (4u << 20) / std::max<uint32_t>(pixels, 1u));
+        // Up to 2^27 paths in flight (262 B each, about 35 GB: a whole
+        // 1920x1080 x 64-spp exposure), whole exposure samples per batch.
+        // Every wavefront launch pays a tail while its slowest rays finish,
+        // so fewer, longer launches are faster.
+        batch = std::max<uint32_t>(1u, (1u << 27) / std::max<uint32_t>(pixels, 1u));
     }
     batch = std::min(batch, total);
-    if (pixels > kPathMask) return fail(VH_ERR_USAGE, "vh_render: more than 2^24 - 1 pixels");
+    if (pixels > kPathMask) return fail(VH_ERR_USAGE, "vh_render: more than 2^30 - 1 pixels");
     batch = std::min<uint32_t>(batch, kPathMask / pixels);
-    const size_t cap = (size_t)pixels * batch;
     StreamState* st = stream_state(dev, s);
     std::lock_guard<std::mutex> hold(st->lock);
-    if (st->capacity < cap) {
-        VH_HIP(st->paths.allocate(cap * (sizeof(float4) * 8 + sizeof(uint32_t) + 2)));
-        VH_HIP(st->rays_a.allocate(cap * 2 * sizeof(float4)));
-        VH_HIP(st->rays_b.allocate(cap * 2 * sizeof(float4)));
-        VH_HIP(st->shadow_rays.allocate(2 * cap * 2 * sizeof(float4)));  // estimator a, then b
-        st->capacity = cap;
+    // grow the stream's wavefront state; halve the batch while HBM is short
+    while (st->capacity < (size_t)pixels * batch) {
+        const size_t want = (size_t)pixels * batch;
+        st->capacity = 0;
+        hipError_t e = st->paths.allocate(want * kPathStateBytes);
+        if (e == hipSuccess) e = st->rays_a.allocate(want * 2 * sizeof(float4));
+        if (e == hipSuccess) e = st->rays_b.allocate(want * 2 * sizeof(float4));
+        if (e == hipSuccess) e = st->shadow_rays.allocate(2 * want * 2 * sizeof(float4));  // estimator a, then b
+        if (e == hipSuccess) {
+            st->capacity = want;
+            break;
+        }
+        st->paths.release();
+        st->rays_a.release();
+        st->rays_b.release();
+        st->shadow_rays.release();
+        if (e != hipErrorOutOfMemory || batch == 1) return hip_fail(e, "vh_render: wavefront state");
+        batch = (batch + 1) / 2;
     }
+    const size_t cap = (size_t)pixels * batch;
     VH_HIP(st->counters.reserve(64 * sizeof(uint32_t)));
     const bool adaptive = (rp->flags & VH_RENDER_ADAPTIVE) != 0;
     // path.slang:1801-1802: minimum max(16, sqrt(samples)) and threshold 0.01

@@ -27,12 +27,12 @@ import time
 
 import numpy as np
 
-PATH_BYTES = 234                   # path state 102 + extension queues 2x32 + shadow queue 2x32 + pending 4
+PATH_BYTES = 262                   # path state 134 + extension queues 2x32 + shadow queues 2x32
 FILM_BYTES = 16                    # RGBA32F per pixel
 MOMENT_BYTES = 48                  # adaptive moment per pixel
 DENOISE_BYTES = 32 + 16 + 1100     # guides, denoised output, U-Net activations (bf16, all levels) per pixel
-BATCH_PATHS = 4 << 20              # paths per wavefront batch the library aims for
-MAX_BATCH_PATHS = (1 << 24) - 1    # paths of one batch (24-bit path ids)
+BATCH_PATHS = 1 << 27              # paths per wavefront batch the library aims for
+MAX_BATCH_PATHS = (1 << 30) - 1    # paths of one batch (30-bit path ids)
 
 
 def batch_paths(width, height, spp):
