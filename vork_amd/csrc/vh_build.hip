@@ -127,8 +127,10 @@ __global__ void k_ploc_init(const float* plo, const float* phi, const uint32_t* 
 // Nearest neighbour of each of the `count` clusters within +-kPlocRadius
 // positions: smallest surface area of the merged box, ties to the lower
 // position.  The block stages its clusters' boxes and the halo in LDS.
-__global__ __launch_bounds__(kPlocBlock) void k_ploc_nn(const uint32_t* clusters, uint32_t count, const float* nb,
+__global__ __launch_bounds__(kPlocBlock) void k_ploc_nn(const uint32_t* clusters, const uint32_t* state, const float* nb,
                                                         uint32_t* nn) {
+    const uint32_t count = state[0];
+    if (count <= 1) return;
     constexpr int kSpan = kPlocBlock + 2 * kPlocRadius;
     __shared__ float box[6][kSpan];
     const int64_t base = (int64_t)blockIdx.x * kPlocBlock - kPlocRadius;
@@ -162,9 +164,10 @@ __global__ __launch_bounds__(kPlocBlock) void k_ploc_nn(const uint32_t* clusters
 // Per cluster position: bit 0 = the cluster survives into the next round
 // (unpaired, or the lower position of a mutual pair, which becomes the
 // merged node), bit 32 = it creates a merged node.
-__global__ void k_ploc_flags(const uint32_t* nn, uint32_t count, uint64_t* flags) {
+__global__ void k_ploc_flags(const uint32_t* nn, const uint32_t* state, uint64_t* flags) {
+    const uint32_t count = state[0];
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
+    if (count <= 1 || i >= count) return;
     const uint32_t j = nn[i];
     const bool mutual = j != i && nn[j] == i;
     const bool keep = !mutual || i < j;
@@ -172,14 +175,70 @@ __global__ void k_ploc_flags(const uint32_t* nn, uint32_t count, uint64_t* flags
     flags[i] = (uint64_t)(keep ? 1u : 0u) | ((uint64_t)(owner ? 1u : 0u) << 32);
 }
 
+// Exclusive scan of the round's flags with the count in device memory:
+// k_scan_tiles scans each tile of kScanTile flags and stores its total,
+// k_scan_sums turns the totals into tile offsets, k_ploc_merge adds them.
+constexpr int kScanBlock = 256, kScanItems = 8, kScanTile = kScanBlock * kScanItems;
+
+__device__ __forceinline__ uint64_t block_inclusive_scan(uint64_t v, uint64_t* sh) {
+    const int tid = threadIdx.x;
+    sh[tid] = v;
+    __syncthreads();
+    for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+        const uint64_t x = tid >= off ? sh[tid - off] : 0;
+        __syncthreads();
+        sh[tid] += x;
+        __syncthreads();
+    }
+    return sh[tid];
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_tiles(const uint64_t* in, const uint32_t* state, uint64_t* out,
+                                                          uint64_t* sums) {
+    __shared__ uint64_t sh[kScanBlock];
+    const uint32_t count = state[0], base = blockIdx.x * kScanTile;
+    if (count <= 1 || base >= count) return;
+    uint64_t v[kScanItems], t = 0;
+    for (int k = 0; k < kScanItems; k++) {
+        const uint32_t i = base + threadIdx.x * kScanItems + k;
+        v[k] = i < count ? in[i] : 0;
+        t += v[k];
+    }
+    const uint64_t incl = block_inclusive_scan(t, sh);
+    uint64_t prefix = incl - t;
+    for (int k = 0; k < kScanItems; k++) {
+        const uint32_t i = base + threadIdx.x * kScanItems + k;
+        if (i < count) out[i] = prefix;
+        prefix += v[k];
+    }
+    if (threadIdx.x == kScanBlock - 1) sums[blockIdx.x] = incl;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_sums(uint64_t* sums, const uint32_t* state) {
+    __shared__ uint64_t sh[1024];
+    const uint32_t count = state[0];
+    if (count <= 1) return;
+    const uint32_t tiles = (count + kScanTile - 1) / kScanTile;
+    uint64_t carry = 0;
+    for (uint32_t start = 0; start < tiles; start += 1024) {
+        const uint32_t i = start + threadIdx.x;
+        const uint64_t v = i < tiles ? sums[i] : 0;
+        const uint64_t incl = block_inclusive_scan(v, sh);
+        if (i < tiles) sums[i] = carry + incl - v;
+        carry += sh[1023];
+        __syncthreads();
+    }
+}
+
 // Writes the merged nodes and the next round's cluster array; the last
-// thread stores the round's totals (survivors, merged nodes).
-__global__ void k_ploc_merge(const uint32_t* clusters, const uint32_t* nn, uint32_t count, const uint64_t* flags,
-                             const uint64_t* offsets, uint32_t n, uint32_t node_base, float* nb, uint32_t* left,
-                             uint32_t* right, uint32_t* size, uint32_t* next, uint32_t* totals) {
+// thread stores the round's totals (survivors, merged nodes) in state[2..3].
+__global__ void k_ploc_merge(const uint32_t* clusters, const uint32_t* nn, uint32_t* state, const uint64_t* flags,
+                             const uint64_t* offsets, const uint64_t* tile_offsets, uint32_t n, float* nb,
+                             uint32_t* left, uint32_t* right, uint32_t* size, uint32_t* next) {
+    const uint32_t count = state[0], node_base = state[1];
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
-    const uint64_t f = flags[i], o = offsets[i];
+    if (count <= 1 || i >= count) return;
+    const uint64_t f = flags[i], o = offsets[i] + tile_offsets[i / kScanTile];
     if (f & 1u) {
         uint32_t id = clusters[i];
         if (f >> 32) {
@@ -196,9 +255,25 @@ __global__ void k_ploc_merge(const uint32_t* clusters, const uint32_t* nn, uint3
         next[(uint32_t)o] = id;
     }
     if (i == count - 1) {
-        totals[0] = (uint32_t)o + (uint32_t)(f & 1u);
-        totals[1] = (uint32_t)(o >> 32) + (uint32_t)(f >> 32);
+        state[2] = (uint32_t)o + (uint32_t)(f & 1u);
+        state[3] = (uint32_t)(o >> 32) + (uint32_t)(f >> 32);
     }
+}
+
+// Next round: survivors become the clusters, merged nodes advance the node
+// base; a round that merged nothing sets the error flag and stops the rounds.
+__global__ void k_ploc_advance(uint32_t* state, uint32_t n_nodes) {
+    if (threadIdx.x != 0) return;
+    const uint32_t count = state[0];
+    if (count <= 1) return;
+    const uint32_t survivors = state[2], merged = state[3];
+    if (survivors >= count || merged == 0 || state[1] + merged > n_nodes) {
+        state[4] = 1u;
+        state[0] = 1u;
+        return;
+    }
+    state[0] = survivors;
+    state[1] += merged;
 }
 
 struct Tree {
@@ -408,8 +483,8 @@ int build_wide_bvh(vh_device* dev, hipStream_t s, const float* d_lo, const float
     if (n == 0) return fail(VH_ERR_USAGE, "cannot build a BVH over zero primitives");
     const uint32_t n_int = n > 1 ? n - 1 : 0;
     const size_t n_nodes = (size_t)n + n_int;
-    DeviceBuffer bounds, keys, keys_sorted, ids, ids_sorted, left, right, size, nb, ca, cb, nn, flags, offsets, tmp,
-        scan_tmp, qa, qb, counters, sah;
+    DeviceBuffer bounds, keys, keys_sorted, ids, ids_sorted, left, right, size, nb, ca, cb, nn, flags, offsets, tmp, qa,
+        qb, counters, sah;
     VH_HIP(bounds.allocate(6 * sizeof(uint32_t)));
     VH_HIP(keys.allocate(sizeof(uint64_t) * n));
     VH_HIP(keys_sorted.allocate(sizeof(uint64_t) * n));
@@ -440,31 +515,38 @@ int build_wide_bvh(vh_device* dev, hipStream_t s, const float* d_lo, const float
                                      ids.as<uint32_t>(), ids_sorted.as<uint32_t>(), n, 0, 63, s));
     hipLaunchKernelGGL(k_ploc_init, dim3(blocks(n, 256)), dim3(256), 0, s, d_lo, d_hi, ids_sorted.as<uint32_t>(), n,
                        nb.as<float>(), size.as<uint32_t>(), ca.as<uint32_t>());
-    uint32_t* totals = counters.as<uint32_t>() + 6;
-    size_t scan_bytes = 0;
-    VH_HIP(rocprim::exclusive_scan(nullptr, scan_bytes, flags.as<uint64_t>(), offsets.as<uint64_t>(), (uint64_t)0, n,
-                                   rocprim::plus<uint64_t>(), s));
-    VH_HIP(scan_tmp.allocate(scan_bytes));
-    uint32_t clusters = n, node_base = n;
+    // Clustering rounds run from device-resident state ([0] clusters, [1]
+    // next node id, [2..3] the round's totals, [4] no-progress flag); the host
+    // reads it back every kRounds rounds and sizes the grids by the last count
+    // it saw (later rounds have fewer clusters and exit early).
+    constexpr int kRounds = 6;
+    DeviceBuffer tile_sums;
+    VH_HIP(tile_sums.allocate(sizeof(uint64_t) * ((n + kScanTile - 1) / kScanTile + 1)));
+    uint32_t* state = counters.as<uint32_t>();
+    const uint32_t init_state[8] = {n, n, 0, 0, 0, 0, 0, 0};
+    VH_HIP(hipMemcpyAsync(state, init_state, sizeof(init_state), hipMemcpyHostToDevice, s));
+    uint32_t bound = n;
     uint32_t *cl_in = ca.as<uint32_t>(), *cl_out = cb.as<uint32_t>();
-    while (clusters > 1) {
-        hipLaunchKernelGGL(k_ploc_nn, dim3(blocks(clusters, kPlocBlock)), dim3(kPlocBlock), 0, s, cl_in, clusters,
-                           nb.as<float>(), nn.as<uint32_t>());
-        hipLaunchKernelGGL(k_ploc_flags, dim3(blocks(clusters, 256)), dim3(256), 0, s, nn.as<uint32_t>(), clusters,
-                           flags.as<uint64_t>());
-        VH_HIP(rocprim::exclusive_scan(scan_tmp.ptr, scan_bytes, flags.as<uint64_t>(), offsets.as<uint64_t>(),
-                                       (uint64_t)0, clusters, rocprim::plus<uint64_t>(), s));
-        hipLaunchKernelGGL(k_ploc_merge, dim3(blocks(clusters, 256)), dim3(256), 0, s, cl_in, nn.as<uint32_t>(), clusters,
-                           flags.as<uint64_t>(), offsets.as<uint64_t>(), n, node_base, nb.as<float>(),
-                           left.as<uint32_t>(), right.as<uint32_t>(), size.as<uint32_t>(), cl_out, totals);
-        uint32_t tot[2];
-        VH_HIP(hipMemcpyAsync(tot, totals, sizeof(tot), hipMemcpyDeviceToHost, s));
+    while (bound > 1) {
+        for (int r = 0; r < kRounds; r++) {
+            hipLaunchKernelGGL(k_ploc_nn, dim3(blocks(bound, kPlocBlock)), dim3(kPlocBlock), 0, s, cl_in, state,
+                               nb.as<float>(), nn.as<uint32_t>());
+            hipLaunchKernelGGL(k_ploc_flags, dim3(blocks(bound, 256)), dim3(256), 0, s, nn.as<uint32_t>(), state,
+                               flags.as<uint64_t>());
+            hipLaunchKernelGGL(k_scan_tiles, dim3(blocks(bound, kScanTile)), dim3(kScanBlock), 0, s,
+                               flags.as<uint64_t>(), state, offsets.as<uint64_t>(), tile_sums.as<uint64_t>());
+            hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, tile_sums.as<uint64_t>(), state);
+            hipLaunchKernelGGL(k_ploc_merge, dim3(blocks(bound, 256)), dim3(256), 0, s, cl_in, nn.as<uint32_t>(), state,
+                               flags.as<uint64_t>(), offsets.as<uint64_t>(), tile_sums.as<uint64_t>(), n,
+                               nb.as<float>(), left.as<uint32_t>(), right.as<uint32_t>(), size.as<uint32_t>(), cl_out);
+            hipLaunchKernelGGL(k_ploc_advance, dim3(1), dim3(64), 0, s, state, (uint32_t)n_nodes);
+            std::swap(cl_in, cl_out);
+        }
+        uint32_t host_state[8];
+        VH_HIP(hipMemcpyAsync(host_state, state, sizeof(host_state), hipMemcpyDeviceToHost, s));
         VH_HIP(hipStreamSynchronize(s));
-        if (tot[0] >= clusters || tot[1] == 0 || node_base + tot[1] > n_nodes)
-            return fail(VH_ERR_DEVICE, "BVH clustering made no progress");
-        clusters = tot[0];
-        node_base += tot[1];
-        std::swap(cl_in, cl_out);
+        if (host_state[4]) return fail(VH_ERR_DEVICE, "BVH clustering made no progress");
+        bound = host_state[0];
     }
     const uint32_t root_id = (uint32_t)(n_nodes - 1);
     // Upper bound of wide nodes: every wide node but the root has > leaf_max
@@ -482,19 +564,26 @@ int build_wide_bvh(vh_device* dev, hipStream_t s, const float* d_lo, const float
     VH_HIP(hipMemcpyAsync(qa.ptr, &root, sizeof(root), hipMemcpyHostToDevice, s));
     Tree t{left.as<uint32_t>(), right.as<uint32_t>(), size.as<uint32_t>(), ids_sorted.as<uint32_t>(), nb.as<float>(),
            n};
+    // Collapse level by level, reading the queue length back every kLevels
+    // levels; a level's queue is at most kWidth times the last one.
+    constexpr int kLevels = 4;
     uint32_t* cnt = counters.as<uint32_t>();
-    uint32_t count = 1, level = 0;
+    uint32_t queue = 1, level = 0;
     QueueItem *qin = qa.as<QueueItem>(), *qout = qb.as<QueueItem>();
     uint32_t *cin = cnt + 2, *cout = cnt + 3;
-    while (count) {
-        VH_HIP(hipMemsetAsync(cout, 0, sizeof(uint32_t), s));
-        hipLaunchKernelGGL(k_collapse, dim3(blocks(count, 128)), dim3(128), 0, s, t, qin, cin, qout, cout, cnt, cnt + 1,
-                           out->nodes.as<WideNode>(), out->order.as<uint32_t>(), leaf_max, n, cnt + 4, level);
-        VH_HIP(hipMemcpyAsync(&count, cout, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    while (queue) {
+        for (int k = 0; k < kLevels; k++) {
+            VH_HIP(hipMemsetAsync(cout, 0, sizeof(uint32_t), s));
+            hipLaunchKernelGGL(k_collapse, dim3(blocks(queue, 128)), dim3(128), 0, s, t, qin, cin, qout, cout, cnt,
+                               cnt + 1, out->nodes.as<WideNode>(), out->order.as<uint32_t>(), leaf_max, n, cnt + 4,
+                               level);
+            std::swap(qin, qout);
+            std::swap(cin, cout);
+            level++;
+            queue = (uint32_t)std::min<uint64_t>((uint64_t)queue * kWidth, max_nodes);
+        }
+        VH_HIP(hipMemcpyAsync(&queue, cin, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         VH_HIP(hipStreamSynchronize(s));
-        std::swap(qin, qout);
-        std::swap(cin, cout);
-        level++;
     }
     uint32_t host_counters[8];
     VH_HIP(hipMemcpyAsync(host_counters, cnt, sizeof(host_counters), hipMemcpyDeviceToHost, s));
@@ -507,7 +596,7 @@ int build_wide_bvh(vh_device* dev, hipStream_t s, const float* d_lo, const float
     if (host_counters[1] != n) return fail(VH_ERR_DEVICE, "BVH collapse lost primitives");
     out->node_count = host_counters[0];
     out->binary_nodes = n_int;
-    out->max_depth = level;
+    out->max_depth = host_counters[4] + 1;  // the deepest level holding wide nodes, counted from 1
     out->sah_cost = sah_sum;
     return VH_OK;
 }
