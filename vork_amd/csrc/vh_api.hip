@@ -143,45 +143,58 @@ int vh_blas_build(vh_device* dev, const vh_geometry* g, uint32_t flags, void* st
     VH_HIP(hipEventCreate(&e1));
     VH_HIP(hipEventRecord(e0, s));
     int rc;
-    DeviceBuffer src_pos, src_nrm, lo, hi;
-    size_t pos_bytes = (size_t)(nv - 1) * g->position_stride + 12;
-    if ((rc = upload(src_pos, g->positions, pos_bytes, on_device, s))) return bail(rc);
+    // temporaries come from the stream's build scratch; device sources are read in place
+    StreamState* st = stream_state(dev, s);
+    std::lock_guard<std::mutex> hold(st->lock);
+    BuildScratch& B = st->build;
+    const size_t pos_bytes = (size_t)(nv - 1) * g->position_stride + 12;
+    const float* pos_src = static_cast<const float*>(g->positions);
+    if (!on_device) {
+        VH_HIP(B.src.reserve(pos_bytes));
+        VH_HIP(hipMemcpyAsync(B.src.ptr, g->positions, pos_bytes, hipMemcpyHostToDevice, s));
+        pos_src = B.src.as<float>();
+    }
     if ((rc = upload(b->indices, g->indices, sizeof(uint32_t) * 3 * (size_t)n, on_device, s))) return bail(rc);
     VH_HIP(b->positions.allocate(sizeof(float) * 3 * (size_t)nv));
-    VH_HIP(lo.allocate(sizeof(float) * 3 * (size_t)n));
-    VH_HIP(hi.allocate(sizeof(float) * 3 * (size_t)n));
-    if ((rc = launch_triangle_bounds(s, src_pos.as<float>(), g->position_stride / 4, b->indices.as<uint32_t>(), n,
-                                     lo.as<float>(), hi.as<float>(), b->positions.as<float>(), nv)))
+    VH_HIP(B.lo.reserve(sizeof(float) * 3 * (size_t)n));
+    VH_HIP(B.hi.reserve(sizeof(float) * 3 * (size_t)n));
+    float* lo = B.lo.as<float>();
+    float* hi = B.hi.as<float>();
+    if ((rc = launch_triangle_bounds(s, B, pos_src, g->position_stride / 4, b->indices.as<uint32_t>(), n, lo, hi,
+                                     b->positions.as<float>(), nv)))
         return bail(rc);
-    src_pos.release();
     if (g->normals) {
-        size_t nb = (size_t)(nv - 1) * g->normal_stride + 12;
-        if ((rc = upload(src_nrm, g->normals, nb, on_device, s))) return bail(rc);
+        const size_t nb = (size_t)(nv - 1) * g->normal_stride + 12;
+        const float* nrm_src = static_cast<const float*>(g->normals);
+        if (!on_device) {
+            VH_HIP(B.src_nrm.reserve(nb));
+            VH_HIP(hipMemcpyAsync(B.src_nrm.ptr, g->normals, nb, hipMemcpyHostToDevice, s));
+            nrm_src = B.src_nrm.as<float>();
+        }
         VH_HIP(b->normals.allocate(sizeof(float) * 3 * (size_t)nv));
-        if ((rc = launch_compact3(s, src_nrm.as<float>(), g->normal_stride / 4, nv, b->normals.as<float>())))
-            return bail(rc);
+        if ((rc = launch_compact3(s, nrm_src, g->normal_stride / 4, nv, b->normals.as<float>()))) return bail(rc);
     }
     if (g->triangle_materials)
         if ((rc = upload(b->materials, g->triangle_materials, sizeof(uint32_t) * (size_t)n, on_device, s)))
             return bail(rc);
     WideBuild wb;
-    if ((rc = build_wide_bvh(dev, s, lo.as<float>(), hi.as<float>(), n, kLeafMax, &wb))) return bail(rc);
+    if ((rc = build_wide_bvh(dev, s, B, lo, hi, n, kLeafMax, &wb))) return bail(rc);
     VH_HIP(b->tris.allocate(sizeof(float) * 9 * (size_t)n));
     VH_HIP(b->tri_ids.allocate(sizeof(uint32_t) * (size_t)n));
-    if ((rc = launch_gather_triangles(s, b->positions.as<float>(), b->indices.as<uint32_t>(), wb.order.as<uint32_t>(),
-                                      n, b->tris.as<float>(), b->tri_ids.as<uint32_t>())))
+    if ((rc = launch_gather_triangles(s, b->positions.as<float>(), b->indices.as<uint32_t>(), wb.order, n,
+                                      b->tris.as<float>(), b->tri_ids.as<uint32_t>())))
         return bail(rc);
+    // the node array leaves the scratch at its exact size
+    VH_HIP(b->nodes.allocate(sizeof(WideNode) * (size_t)wb.node_count));
+    VH_HIP(hipMemcpyAsync(b->nodes.ptr, wb.nodes, sizeof(WideNode) * (size_t)wb.node_count, hipMemcpyDeviceToDevice, s));
     float bounds[6];
-    if ((rc = launch_bounds_reduce(s, lo.as<float>(), hi.as<float>(), n, bounds))) return bail(rc);
+    if ((rc = launch_bounds_reduce(s, B, lo, hi, n, bounds))) return bail(rc);
     VH_HIP(hipEventRecord(e1, s));
     VH_HIP(hipEventSynchronize(e1));
     float ms = 0;
     (void)hipEventElapsedTime(&ms, e0, e1);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    // take ownership of the node array
-    b->nodes.ptr = wb.nodes.ptr; b->nodes.bytes = wb.nodes.bytes;
-    wb.nodes.ptr = nullptr; wb.nodes.bytes = 0;
     BlasRecord& r = b->record;
     r.nodes = b->nodes.as<WideNode>();
     r.tris = b->tris.as<float>();
@@ -246,34 +259,44 @@ int vh_tlas_build(vh_device* dev, vh_blas* const* blases, uint32_t blas_count, c
     }
     auto bail = [&](int rc) { if (fresh) delete t; return rc; };
     int rc;
+    // temporaries come from the stream's build scratch; a refit reuses the TLAS's buffers
+    StreamState* st = stream_state(dev, s);
+    std::lock_guard<std::mutex> hold(st->lock);
+    BuildScratch& B = st->build;
     std::vector<BlasRecord> recs(blas_count);
     for (uint32_t i = 0; i < blas_count; i++) recs[i] = blases[i]->record;
-    if ((rc = upload(t->blas_records, recs.data(), sizeof(BlasRecord) * std::max<uint32_t>(blas_count, 1),
-                     false, s)))
-        return bail(rc);
-    DeviceBuffer src, lo, hi;
-    if ((rc = upload(src, instances, sizeof(vh_instance) * (size_t)instance_count, (flags & VH_DEVICE_POINTERS) != 0, s)))
-        return bail(rc);
+    const size_t rec_bytes = sizeof(BlasRecord) * std::max<uint32_t>(blas_count, 1);
+    VH_HIP(t->blas_records.reserve(rec_bytes));
+    if (blas_count)
+        VH_HIP(hipMemcpyAsync(t->blas_records.ptr, recs.data(), sizeof(BlasRecord) * blas_count,
+                              hipMemcpyHostToDevice, s));
+    const vh_instance* src = instances;
+    if (!(flags & VH_DEVICE_POINTERS)) {
+        VH_HIP(B.src.reserve(sizeof(vh_instance) * (size_t)instance_count));
+        VH_HIP(hipMemcpyAsync(B.src.ptr, instances, sizeof(vh_instance) * (size_t)instance_count,
+                              hipMemcpyHostToDevice, s));
+        src = B.src.as<vh_instance>();
+    }
     VH_HIP(t->instances.reserve(sizeof(InstanceRecord) * (size_t)instance_count));
-    if ((rc = launch_instance_records(s, src.as<vh_instance>(), instance_count, blas_count,
-                                      t->instances.as<InstanceRecord>())))
+    if ((rc = launch_instance_records(s, src, instance_count, blas_count, t->instances.as<InstanceRecord>())))
         return bail(rc);
-    VH_HIP(lo.allocate(sizeof(float) * 3 * (size_t)instance_count));
-    VH_HIP(hi.allocate(sizeof(float) * 3 * (size_t)instance_count));
+    VH_HIP(B.lo.reserve(sizeof(float) * 3 * (size_t)instance_count));
+    VH_HIP(B.hi.reserve(sizeof(float) * 3 * (size_t)instance_count));
+    float* lo = B.lo.as<float>();
+    float* hi = B.hi.as<float>();
     if ((rc = launch_instance_bounds(s, t->instances.as<InstanceRecord>(), t->blas_records.as<BlasRecord>(),
-                                     instance_count, lo.as<float>(), hi.as<float>())))
+                                     instance_count, lo, hi)))
         return bail(rc);
     // A refit keeps the handle and instance order; the 8-wide hierarchy is
     // rebuilt from the new boxes (a clustering rebuild costs little more than a
     // quantised refit pass and keeps the tree quality of moving instances).
     WideBuild wb;
-    if ((rc = build_wide_bvh(dev, s, lo.as<float>(), hi.as<float>(), instance_count, 1, &wb))) return bail(rc);
-    t->nodes.release();
-    t->nodes.ptr = wb.nodes.ptr; t->nodes.bytes = wb.nodes.bytes;
-    wb.nodes.ptr = nullptr; wb.nodes.bytes = 0;
-    t->inst_ids.release();
-    t->inst_ids.ptr = wb.order.ptr; t->inst_ids.bytes = wb.order.bytes;
-    wb.order.ptr = nullptr; wb.order.bytes = 0;
+    if ((rc = build_wide_bvh(dev, s, B, lo, hi, instance_count, 1, &wb))) return bail(rc);
+    VH_HIP(t->nodes.reserve(sizeof(WideNode) * (size_t)wb.node_count));
+    VH_HIP(hipMemcpyAsync(t->nodes.ptr, wb.nodes, sizeof(WideNode) * (size_t)wb.node_count, hipMemcpyDeviceToDevice, s));
+    VH_HIP(t->inst_ids.reserve(sizeof(uint32_t) * (size_t)instance_count));
+    VH_HIP(hipMemcpyAsync(t->inst_ids.ptr, wb.order, sizeof(uint32_t) * (size_t)instance_count,
+                          hipMemcpyDeviceToDevice, s));
     t->node_count = wb.node_count;
     t->instance_count = instance_count;
     t->blases.assign(blases, blases + blas_count);

@@ -478,29 +478,31 @@ inline uint32_t blocks(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1)
 
 }  // namespace
 
-int build_wide_bvh(vh_device* dev, hipStream_t s, const float* d_lo, const float* d_hi, uint32_t n, uint32_t leaf_max,
-                   WideBuild* out) {
+int build_wide_bvh(vh_device* dev, hipStream_t s, BuildScratch& B, const float* d_lo, const float* d_hi, uint32_t n,
+                   uint32_t leaf_max, WideBuild* out) {
     if (n == 0) return fail(VH_ERR_USAGE, "cannot build a BVH over zero primitives");
     const uint32_t n_int = n > 1 ? n - 1 : 0;
     const size_t n_nodes = (size_t)n + n_int;
-    DeviceBuffer bounds, keys, keys_sorted, ids, ids_sorted, left, right, size, nb, ca, cb, nn, flags, offsets, tmp, qa,
-        qb, counters, sah;
-    VH_HIP(bounds.allocate(6 * sizeof(uint32_t)));
-    VH_HIP(keys.allocate(sizeof(uint64_t) * n));
-    VH_HIP(keys_sorted.allocate(sizeof(uint64_t) * n));
-    VH_HIP(ids.allocate(sizeof(uint32_t) * n));
-    VH_HIP(ids_sorted.allocate(sizeof(uint32_t) * n));
-    VH_HIP(left.allocate(sizeof(uint32_t) * (n_int + 1)));
-    VH_HIP(right.allocate(sizeof(uint32_t) * (n_int + 1)));
-    VH_HIP(size.allocate(sizeof(uint32_t) * n_nodes));
-    VH_HIP(nb.allocate(sizeof(float) * 6 * n_nodes));
-    VH_HIP(ca.allocate(sizeof(uint32_t) * n));
-    VH_HIP(cb.allocate(sizeof(uint32_t) * n));
-    VH_HIP(nn.allocate(sizeof(uint32_t) * n));
-    VH_HIP(flags.allocate(sizeof(uint64_t) * n));
-    VH_HIP(offsets.allocate(sizeof(uint64_t) * n));
-    VH_HIP(counters.allocate(sizeof(uint32_t) * 8));
-    VH_HIP(sah.allocate(sizeof(double)));
+    DeviceBuffer &bounds = B.bounds, &keys = B.keys, &keys_sorted = B.keys_sorted, &ids = B.ids,
+                 &ids_sorted = B.ids_sorted, &left = B.left, &right = B.right, &size = B.size, &nb = B.nb, &ca = B.ca,
+                 &cb = B.cb, &nn = B.nn, &flags = B.flags, &offsets = B.offsets, &tmp = B.sort_tmp, &qa = B.qa,
+                 &qb = B.qb, &counters = B.counters, &sah = B.sah, &tile_sums = B.tile_sums;
+    VH_HIP(bounds.reserve(6 * sizeof(uint32_t)));
+    VH_HIP(keys.reserve(sizeof(uint64_t) * n));
+    VH_HIP(keys_sorted.reserve(sizeof(uint64_t) * n));
+    VH_HIP(ids.reserve(sizeof(uint32_t) * n));
+    VH_HIP(ids_sorted.reserve(sizeof(uint32_t) * n));
+    VH_HIP(left.reserve(sizeof(uint32_t) * (n_int + 1)));
+    VH_HIP(right.reserve(sizeof(uint32_t) * (n_int + 1)));
+    VH_HIP(size.reserve(sizeof(uint32_t) * n_nodes));
+    VH_HIP(nb.reserve(sizeof(float) * 6 * n_nodes));
+    VH_HIP(ca.reserve(sizeof(uint32_t) * n));
+    VH_HIP(cb.reserve(sizeof(uint32_t) * n));
+    VH_HIP(nn.reserve(sizeof(uint32_t) * n));
+    VH_HIP(flags.reserve(sizeof(uint64_t) * n));
+    VH_HIP(offsets.reserve(sizeof(uint64_t) * n));
+    VH_HIP(counters.reserve(sizeof(uint32_t) * 8));
+    VH_HIP(sah.reserve(sizeof(double)));
 
     hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, s, bounds.as<uint32_t>());
     uint32_t rb = std::min<uint32_t>(blocks(n, 256), 1024);
@@ -510,7 +512,7 @@ int build_wide_bvh(vh_device* dev, hipStream_t s, const float* d_lo, const float
     size_t tmp_bytes = 0;
     VH_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, keys.as<uint64_t>(), keys_sorted.as<uint64_t>(),
                                      ids.as<uint32_t>(), ids_sorted.as<uint32_t>(), n, 0, 63, s));
-    VH_HIP(tmp.allocate(tmp_bytes));
+    VH_HIP(tmp.reserve(tmp_bytes));
     VH_HIP(rocprim::radix_sort_pairs(tmp.ptr, tmp_bytes, keys.as<uint64_t>(), keys_sorted.as<uint64_t>(),
                                      ids.as<uint32_t>(), ids_sorted.as<uint32_t>(), n, 0, 63, s));
     hipLaunchKernelGGL(k_ploc_init, dim3(blocks(n, 256)), dim3(256), 0, s, d_lo, d_hi, ids_sorted.as<uint32_t>(), n,
@@ -520,8 +522,7 @@ int build_wide_bvh(vh_device* dev, hipStream_t s, const float* d_lo, const float
     // reads it back every kRounds rounds and sizes the grids by the last count
     // it saw (later rounds have fewer clusters and exit early).
     constexpr int kRounds = 6;
-    DeviceBuffer tile_sums;
-    VH_HIP(tile_sums.allocate(sizeof(uint64_t) * ((n + kScanTile - 1) / kScanTile + 1)));
+    VH_HIP(tile_sums.reserve(sizeof(uint64_t) * ((n + kScanTile - 1) / kScanTile + 1)));
     uint32_t* state = counters.as<uint32_t>();
     const uint32_t init_state[8] = {n, n, 0, 0, 0, 0, 0, 0};
     VH_HIP(hipMemcpyAsync(state, init_state, sizeof(init_state), hipMemcpyHostToDevice, s));
@@ -553,10 +554,10 @@ int build_wide_bvh(vh_device* dev, hipStream_t s, const float* d_lo, const float
     // primitives under it except through the tree shape; n internal binary
     // nodes bound it.
     uint32_t max_nodes = std::max<uint32_t>(n_int, 1) + 1;
-    VH_HIP(out->nodes.allocate(sizeof(WideNode) * max_nodes));
-    VH_HIP(out->order.allocate(sizeof(uint32_t) * n));
-    VH_HIP(qa.allocate(sizeof(QueueItem) * max_nodes));
-    VH_HIP(qb.allocate(sizeof(QueueItem) * max_nodes));
+    VH_HIP(B.nodes.reserve(sizeof(WideNode) * max_nodes));
+    VH_HIP(B.order.reserve(sizeof(uint32_t) * n));
+    VH_HIP(qa.reserve(sizeof(QueueItem) * max_nodes));
+    VH_HIP(qb.reserve(sizeof(QueueItem) * max_nodes));
     // counters: 0 node_counter, 1 item_counter, 2 count_a, 3 count_b, 4 max_level
     uint32_t init[8] = {1, 0, 1, 0, 0, 0, 0, 0};
     VH_HIP(hipMemcpyAsync(counters.ptr, init, sizeof(init), hipMemcpyHostToDevice, s));
@@ -575,7 +576,7 @@ int build_wide_bvh(vh_device* dev, hipStream_t s, const float* d_lo, const float
         for (int k = 0; k < kLevels; k++) {
             VH_HIP(hipMemsetAsync(cout, 0, sizeof(uint32_t), s));
             hipLaunchKernelGGL(k_collapse, dim3(blocks(queue, 128)), dim3(128), 0, s, t, qin, cin, qout, cout, cnt,
-                               cnt + 1, out->nodes.as<WideNode>(), out->order.as<uint32_t>(), leaf_max, n, cnt + 4,
+                               cnt + 1, B.nodes.as<WideNode>(), B.order.as<uint32_t>(), leaf_max, n, cnt + 4,
                                level);
             std::swap(qin, qout);
             std::swap(cin, cout);
@@ -594,6 +595,8 @@ int build_wide_bvh(vh_device* dev, hipStream_t s, const float* d_lo, const float
     VH_HIP(hipMemcpyAsync(&sah_sum, sah.ptr, sizeof(double), hipMemcpyDeviceToHost, s));
     VH_HIP(hipStreamSynchronize(s));
     if (host_counters[1] != n) return fail(VH_ERR_DEVICE, "BVH collapse lost primitives");
+    out->nodes = B.nodes.as<WideNode>();
+    out->order = B.order.as<uint32_t>();
     out->node_count = host_counters[0];
     out->binary_nodes = n_int;
     out->max_depth = host_counters[4] + 1;  // the deepest level holding wide nodes, counted from 1
@@ -716,15 +719,14 @@ int launch_compact3(hipStream_t s, const float* src, uint32_t stride_floats, uin
     return VH_OK;
 }
 
-int launch_bounds_reduce(hipStream_t s, const float* lo, const float* hi, uint32_t n, float out[6]) {
-    DeviceBuffer b, f;
-    VH_HIP(b.allocate(6 * sizeof(uint32_t)));
-    VH_HIP(f.allocate(6 * sizeof(float)));
-    hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, s, b.as<uint32_t>());
-    hipLaunchKernelGGL(k_bounds_reduce, dim3(std::min<uint32_t>(blocks(n, 256), 1024)), dim3(256), 0, s, lo, hi, n,
-                       b.as<uint32_t>());
-    hipLaunchKernelGGL(k_ordered_to_float, dim3(1), dim3(64), 0, s, b.as<uint32_t>(), f.as<float>());
-    VH_HIP(hipMemcpyAsync(out, f.ptr, 6 * sizeof(float), hipMemcpyDeviceToHost, s));
+int launch_bounds_reduce(hipStream_t s, BuildScratch& B, const float* lo, const float* hi, uint32_t n, float out[6]) {
+    VH_HIP(B.small.reserve(16 * sizeof(uint32_t)));
+    uint32_t* b = B.small.as<uint32_t>();
+    float* f = reinterpret_cast<float*>(b + 8);
+    hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, s, b);
+    hipLaunchKernelGGL(k_bounds_reduce, dim3(std::min<uint32_t>(blocks(n, 256), 1024)), dim3(256), 0, s, lo, hi, n, b);
+    hipLaunchKernelGGL(k_ordered_to_float, dim3(1), dim3(64), 0, s, b, f);
+    VH_HIP(hipMemcpyAsync(out, f, 6 * sizeof(float), hipMemcpyDeviceToHost, s));
     VH_HIP(hipStreamSynchronize(s));
     return VH_OK;
 }
@@ -736,10 +738,11 @@ int launch_instance_records(hipStream_t s, const vh_instance* inst, uint32_t n, 
     return VH_OK;
 }
 
-int launch_triangle_bounds(hipStream_t s, const float* positions, uint32_t stride_floats, const uint32_t* indices,
-                           uint32_t n, float* lo, float* hi, float* compact_positions, uint32_t vertex_count) {
-    DeviceBuffer bad;
-    VH_HIP(bad.allocate(sizeof(uint32_t)));
+int launch_triangle_bounds(hipStream_t s, BuildScratch& B, const float* positions, uint32_t stride_floats,
+                           const uint32_t* indices, uint32_t n, float* lo, float* hi, float* compact_positions,
+                           uint32_t vertex_count) {
+    VH_HIP(B.small.reserve(16 * sizeof(uint32_t)));
+    DeviceBuffer& bad = B.small;
     VH_HIP(hipMemsetAsync(bad.ptr, 0, sizeof(uint32_t), s));
     hipLaunchKernelGGL(k_compact_positions, dim3(blocks(vertex_count, 256)), dim3(256), 0, s, positions, stride_floats,
                        vertex_count, compact_positions);

@@ -50,6 +50,14 @@ struct DeviceBuffer {
     template <class T> T* as() const { return static_cast<T*>(ptr); }
 };
 
+// Scratch of the BVH builder: every temporary of a BLAS or TLAS build, grown
+// to the largest build on the stream and reused (allocating and freeing a few
+// hundred MB per build cost more than the build's kernels).
+struct BuildScratch {
+    DeviceBuffer src, src_nrm, lo, hi, bounds, keys, keys_sorted, ids, ids_sorted, left, right, size, nb, ca, cb, nn,
+        flags, offsets, sort_tmp, qa, qb, counters, sah, tile_sums, nodes, order, small;
+};
+
 // Wavefront and traversal scratch of one stream.  Calls on one stream are
 // ordered on the device, so they share it; `lock` is held for the whole
 // (synchronous) call, so calls on one stream from several threads wait for
@@ -58,6 +66,7 @@ struct StreamState {
     std::mutex lock;
     DeviceBuffer paths, rays_a, rays_b, shadow_rays, counters, overflow, scratch_u32;
     size_t capacity = 0;  // paths the wavefront buffers hold
+    BuildScratch build;
 };
 
 }  // namespace vh
@@ -159,25 +168,26 @@ namespace vh {
 // arrays of 3 floats per primitive.  Writes the nodes and the leaf-order
 // permutation `order` (leaf item -> primitive) and returns the node count.
 struct WideBuild {
-    DeviceBuffer nodes;   // WideNode[]
-    DeviceBuffer order;   // uint32 per item
+    const WideNode* nodes = nullptr;  // in the scratch: valid until the next build on the stream
+    const uint32_t* order = nullptr;  // leaf item -> primitive, likewise
     uint32_t node_count = 0;
     uint32_t binary_nodes = 0;
     uint32_t max_depth = 0;
     double sah_cost = 0;
 };
-int build_wide_bvh(vh_device* dev, hipStream_t stream, const float* d_lo, const float* d_hi, uint32_t n,
-                   uint32_t leaf_max, WideBuild* out);
+int build_wide_bvh(vh_device* dev, hipStream_t stream, BuildScratch& scratch, const float* d_lo, const float* d_hi,
+                   uint32_t n, uint32_t leaf_max, WideBuild* out);
 
 // Kernels of the BLAS build that depend on the mesh layout.
-int launch_triangle_bounds(hipStream_t s, const float* positions, uint32_t stride_floats, const uint32_t* indices,
+int launch_triangle_bounds(hipStream_t s, BuildScratch& scratch, const float* positions, uint32_t stride_floats, const uint32_t* indices,
                            uint32_t n, float* lo, float* hi, float* compact_positions, uint32_t vertex_count);
 int launch_gather_triangles(hipStream_t s, const float* positions, const uint32_t* indices, const uint32_t* order,
                             uint32_t n, float* tris, uint32_t* tri_ids);
 int launch_instance_bounds(hipStream_t s, const InstanceRecord* inst, const BlasRecord* blases, uint32_t n, float* lo,
                            float* hi);
 int launch_compact3(hipStream_t s, const float* src, uint32_t stride_floats, uint32_t count, float* dst);
-int launch_bounds_reduce(hipStream_t s, const float* lo, const float* hi, uint32_t n, float out[6]);
+int launch_bounds_reduce(hipStream_t s, BuildScratch& scratch, const float* lo, const float* hi, uint32_t n,
+                         float out[6]);
 int launch_instance_records(hipStream_t s, const vh_instance* inst, uint32_t n, uint32_t blas_count,
                             InstanceRecord* out);
 
