@@ -285,7 +285,8 @@ typedef struct vh_render_params {
     float clamp_direct;      /* 0 = off */
     float clamp_indirect;
     float light_threshold;
-    uint32_t batch_samples;  /* exposure samples per wavefront batch, 0 = auto */
+    uint32_t batch_samples;  /* exposure samples per wavefront batch, 0 = auto: up to 2^27 paths (about
+                                262 B each in the stream's scratch), halved while device memory is short */
     uint32_t flags;          /* VH_RENDER_* */
     uint32_t adaptive_min_samples;  /* sampling.w: 0 = max(16, sqrt(samples)) */
     float adaptive_threshold;       /* adaptive.x: 0 = 0.01 */
