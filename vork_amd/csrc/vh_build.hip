@@ -232,9 +232,86 @@ __global__ __launch_bounds__(1024) void k_scan_sums(uint64_t* sums, const uint32
 
 // Writes the merged nodes and the next round's cluster array; the last
 // thread stores the round's totals (survivors, merged nodes) in state[2..3].
+// ---- SAH-optimal collapse (Ylitie, Karras and Laine 2017, section 3): C(x, i)
+// is the least SAH cost of the subtree at binary node x as a forest of at
+// most i children of a wide node; a leaf of at most p_max primitives costs
+// area * c_prim * primitives, a wide node area * c_node plus the best forest
+// of 8 roots of its two subtrees.  Decisions per internal node, byte i - 1:
+// kDpLeaf / kDpInternal for i = 1, else kDpFewer (the forest of i - 1 roots)
+// or the number k of roots given to the left subtree; byte 8 is the split of
+// the wide node's 8 roots.  A node's table is computed by the clustering
+// round that creates it, from its children's tables of earlier rounds.
+#ifndef VH_DP_CNODE
+#define VH_DP_CNODE 1.0f
+#endif
+#ifndef VH_DP_CPRIM
+#define VH_DP_CPRIM 0.3f
+#endif
+constexpr uint8_t kDpFewer = 0, kDpLeaf = 0xF0, kDpInternal = 0xF1;
+constexpr uint32_t kDpStride = 9;
+
+__device__ __forceinline__ float node_area(const float* nb, uint32_t r) {
+    float lo[3] = {nb[6 * (size_t)r], nb[6 * (size_t)r + 1], nb[6 * (size_t)r + 2]};
+    float hi[3] = {nb[6 * (size_t)r + 3], nb[6 * (size_t)r + 4], nb[6 * (size_t)r + 5]};
+    return half_area(lo, hi);
+}
+
+// The table of internal node id = (l, r) of box area a over prims primitives.
+__device__ void dp_table(const float* nb, const float* cost, uint32_t n, uint32_t l, uint32_t r, uint32_t id, float a,
+                         uint32_t prims, uint32_t p_max, float* cost_out, uint8_t* dec_out) {
+    float cl[8], cr[8];
+    for (int s2 = 0; s2 < 2; s2++) {
+        const uint32_t c = s2 ? r : l;
+        float* out = s2 ? cr : cl;
+        if (c < n) {
+            const float v = node_area(nb, c) * VH_DP_CPRIM;
+            for (int i = 0; i < 8; i++) out[i] = v;
+        } else {
+            const float4* src = reinterpret_cast<const float4*>(cost + (size_t)(c - n) * 8);
+            const float4 u = src[0], v = src[1];
+            out[0] = u.x; out[1] = u.y; out[2] = u.z; out[3] = u.w;
+            out[4] = v.x; out[5] = v.y; out[6] = v.z; out[7] = v.w;
+        }
+    }
+    float dist[9];
+    uint8_t arg[9];
+    for (int jj = 2; jj <= 8; jj++) {
+        float best = INFINITY;
+        uint8_t bk = 1;
+        for (int k = 1; k < jj; k++) {
+            const float v = cl[k - 1] + cr[jj - k - 1];
+            if (v < best) { best = v; bk = (uint8_t)k; }
+        }
+        dist[jj] = best;
+        arg[jj] = bk;
+    }
+    const float c_leaf = prims <= p_max ? a * VH_DP_CPRIM * (float)prims : INFINITY;
+    const float c_int = dist[8] + a * VH_DP_CNODE;
+    float co[8];
+    uint8_t de[kDpStride];
+    float prev = fminf(c_leaf, c_int);
+    co[0] = prev;
+    de[0] = c_leaf <= c_int ? kDpLeaf : kDpInternal;
+    for (int i = 2; i <= 8; i++) {
+        if (dist[i] < prev) {
+            prev = dist[i];
+            de[i - 1] = arg[i];
+        } else {
+            de[i - 1] = kDpFewer;
+        }
+        co[i - 1] = prev;
+    }
+    de[8] = arg[8];
+    float4* dst = reinterpret_cast<float4*>(cost_out + (size_t)(id - n) * 8);
+    dst[0] = make_float4(co[0], co[1], co[2], co[3]);
+    dst[1] = make_float4(co[4], co[5], co[6], co[7]);
+    for (int i = 0; i < (int)kDpStride; i++) dec_out[(size_t)(id - n) * kDpStride + i] = de[i];
+}
+
 __global__ void k_ploc_merge(const uint32_t* clusters, const uint32_t* nn, uint32_t* state, const uint64_t* flags,
                              const uint64_t* offsets, const uint64_t* tile_offsets, uint32_t n, float* nb,
-                             uint32_t* left, uint32_t* right, uint32_t* size, uint32_t* next) {
+                             uint32_t* left, uint32_t* right, uint32_t* size, uint32_t* next, float* dp_cost,
+                             uint8_t* dp_dec, uint32_t p_max) {
     const uint32_t count = state[0], node_base = state[1];
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (count <= 1 || i >= count) return;
@@ -244,13 +321,18 @@ __global__ void k_ploc_merge(const uint32_t* clusters, const uint32_t* nn, uint3
         if (f >> 32) {
             const uint32_t a = id, b = clusters[nn[i]];
             id = node_base + (uint32_t)(o >> 32);
+            float lo[3], hi[3];
             for (int k = 0; k < 3; k++) {
-                nb[6 * (size_t)id + k] = fminf(nb[6 * (size_t)a + k], nb[6 * (size_t)b + k]);
-                nb[6 * (size_t)id + 3 + k] = fmaxf(nb[6 * (size_t)a + 3 + k], nb[6 * (size_t)b + 3 + k]);
+                lo[k] = fminf(nb[6 * (size_t)a + k], nb[6 * (size_t)b + k]);
+                hi[k] = fmaxf(nb[6 * (size_t)a + 3 + k], nb[6 * (size_t)b + 3 + k]);
+                nb[6 * (size_t)id + k] = lo[k];
+                nb[6 * (size_t)id + 3 + k] = hi[k];
             }
             left[id - n] = a;
             right[id - n] = b;
-            size[id] = size[a] + size[b];
+            const uint32_t prims = size[a] + size[b];
+            size[id] = prims;
+            dp_table(nb, dp_cost, n, a, b, id, half_area(lo, hi), prims, p_max, dp_cost, dp_dec);
         }
         next[(uint32_t)o] = id;
     }
@@ -280,6 +362,7 @@ struct Tree {
     const uint32_t *left, *right, *size, *sorted_ids;
     const float* nb;
     uint32_t n;
+    const uint8_t* dp_dec;     // per internal node, the decisions of dp_table
 };
 
 __device__ __forceinline__ bool is_leaf(const Tree& t, uint32_t r) { return r < t.n; }
@@ -338,6 +421,35 @@ __device__ void quantise_axis(float origin, float node_hi, int m, const float* c
 
 struct QueueItem { uint32_t binary; uint32_t wide; };
 
+// A subtree becomes a wide node unless the collapse chose a leaf for it; a
+// leaf never holds more than leaf_max primitives, whatever the table says.
+__device__ __forceinline__ bool dp_internal(const Tree& t, uint32_t r, uint32_t leaf_max) {
+    return !is_leaf(t, r) && (t.size[r] > leaf_max || t.dp_dec[(size_t)(r - t.n) * kDpStride] != kDpLeaf);
+}
+
+// The roots of the best forest of a wide node at internal binary node r.
+__device__ __forceinline__ int dp_children(const Tree& t, uint32_t r, uint32_t* c) {
+    uint32_t sn[16], si[16];
+    int sp = 0, m = 0;
+    const uint8_t* d0 = t.dp_dec + (size_t)(r - t.n) * kDpStride;
+    const uint32_t k8 = d0[8] >= 1 && d0[8] <= 7 ? d0[8] : 4u;
+    sn[sp] = t.right[r - t.n]; si[sp++] = 8 - k8;
+    sn[sp] = t.left[r - t.n]; si[sp++] = k8;
+    while (sp > 0) {
+        const uint32_t x = sn[--sp];
+        uint32_t i = si[sp];
+        if (is_leaf(t, x)) { c[m++] = x; continue; }
+        const uint8_t* d = t.dp_dec + (size_t)(x - t.n) * kDpStride;
+        while (i > 1 && d[i - 1] == kDpFewer) i--;
+        if (i == 1) { c[m++] = x; continue; }
+        const uint32_t k = d[i - 1];
+        if (k == 0 || k >= i) { c[m++] = x; continue; }
+        sn[sp] = t.right[x - t.n]; si[sp++] = i - k;
+        sn[sp] = t.left[x - t.n]; si[sp++] = k;
+    }
+    return m;
+}
+
 __global__ void k_collapse(Tree t, const QueueItem* in, const uint32_t* in_count, QueueItem* out, uint32_t* out_count,
                            uint32_t* node_counter, uint32_t* item_counter, WideNode* nodes, uint32_t* order,
                            uint32_t leaf_max, uint32_t n_prims, uint32_t* max_level, uint32_t level) {
@@ -350,23 +462,7 @@ __global__ void k_collapse(Tree t, const QueueItem* in, const uint32_t* in_count
         // a single-primitive tree: one leaf child
         c[m++] = it.binary;
     } else {
-        c[m++] = t.left[it.binary - t.n];
-        c[m++] = t.right[it.binary - t.n];
-        while (m < kWidth) {
-            int best = -1;
-            float best_area = -1.f;
-            for (int k = 0; k < m; k++) {
-                if (is_leaf(t, c[k]) || t.size[c[k]] <= leaf_max) continue;
-                float lo[3], hi[3];
-                ref_box(t, c[k], lo, hi);
-                float a = half_area(lo, hi);
-                if (a > best_area) { best_area = a; best = k; }
-            }
-            if (best < 0) break;
-            uint32_t r = c[best];
-            c[best] = t.left[r - t.n];
-            c[m++] = t.right[r - t.n];
-        }
+        m = dp_children(t, it.binary, c);
     }
     float clo[3][kWidth], chi[3][kWidth];
     float nlo[3] = {INFINITY, INFINITY, INFINITY}, nhi[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -421,8 +517,7 @@ __global__ void k_collapse(Tree t, const QueueItem* in, const uint32_t* in_count
     node.ex = e[0]; node.ey = e[1]; node.ez = e[2];
     uint32_t n_internal = 0, n_items = 0;
     for (int k = 0; k < m; k++) {
-        bool internal = !is_leaf(t, c[k]) && t.size[c[k]] > leaf_max;
-        if (internal) n_internal++; else n_items += t.size[c[k]];
+        if (dp_internal(t, c[k], leaf_max)) n_internal++; else n_items += t.size[c[k]];
     }
     uint32_t child_base = n_internal ? atomicAdd(node_counter, n_internal) : 0u;
     uint32_t item_base = n_items ? atomicAdd(item_counter, n_items) : 0u;
@@ -433,8 +528,7 @@ __global__ void k_collapse(Tree t, const QueueItem* in, const uint32_t* in_count
         node.meta[q] = (uint8_t)kMetaNoItems;
         const int k = slot_child[q];
         if (k < 0) continue;
-        bool internal = !is_leaf(t, c[k]) && t.size[c[k]] > leaf_max;
-        if (internal) {
+        if (dp_internal(t, c[k], leaf_max)) {
             imask |= 1u << q;
             uint32_t slot = atomicAdd(out_count, 1u);
             out[slot].binary = c[k];
@@ -503,6 +597,8 @@ int build_wide_bvh(vh_device* dev, hipStream_t s, BuildScratch& B, const float* 
     VH_HIP(offsets.reserve(sizeof(uint64_t) * n));
     VH_HIP(counters.reserve(sizeof(uint32_t) * 8));
     VH_HIP(sah.reserve(sizeof(double)));
+    VH_HIP(B.dp_cost.reserve(sizeof(float) * 8 * (n_int + 1)));
+    VH_HIP(B.dp_dec.reserve((size_t)kDpStride * (n_int + 1)));
 
     hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, s, bounds.as<uint32_t>());
     uint32_t rb = std::min<uint32_t>(blocks(n, 256), 1024);
@@ -539,7 +635,8 @@ int build_wide_bvh(vh_device* dev, hipStream_t s, BuildScratch& B, const float* 
             hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, tile_sums.as<uint64_t>(), state);
             hipLaunchKernelGGL(k_ploc_merge, dim3(blocks(bound, 256)), dim3(256), 0, s, cl_in, nn.as<uint32_t>(), state,
                                flags.as<uint64_t>(), offsets.as<uint64_t>(), tile_sums.as<uint64_t>(), n,
-                               nb.as<float>(), left.as<uint32_t>(), right.as<uint32_t>(), size.as<uint32_t>(), cl_out);
+                               nb.as<float>(), left.as<uint32_t>(), right.as<uint32_t>(), size.as<uint32_t>(), cl_out,
+                               B.dp_cost.as<float>(), B.dp_dec.as<uint8_t>(), leaf_max);
             hipLaunchKernelGGL(k_ploc_advance, dim3(1), dim3(64), 0, s, state, (uint32_t)n_nodes);
             std::swap(cl_in, cl_out);
         }
@@ -564,7 +661,7 @@ int build_wide_bvh(vh_device* dev, hipStream_t s, BuildScratch& B, const float* 
     QueueItem root = {root_id, 0u};
     VH_HIP(hipMemcpyAsync(qa.ptr, &root, sizeof(root), hipMemcpyHostToDevice, s));
     Tree t{left.as<uint32_t>(), right.as<uint32_t>(), size.as<uint32_t>(), ids_sorted.as<uint32_t>(), nb.as<float>(),
-           n};
+           n, B.dp_dec.as<uint8_t>()};
     // Collapse level by level, reading the queue length back every kLevels
     // levels; a level's queue is at most kWidth times the last one.
     constexpr int kLevels = 4;

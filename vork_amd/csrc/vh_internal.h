@@ -55,7 +55,7 @@ struct DeviceBuffer {
 // hundred MB per build cost more than the build's kernels).
 struct BuildScratch {
     DeviceBuffer src, src_nrm, lo, hi, bounds, keys, keys_sorted, ids, ids_sorted, left, right, size, nb, ca, cb, nn,
-        flags, offsets, sort_tmp, qa, qb, counters, sah, tile_sums, nodes, order, small;
+        flags, offsets, sort_tmp, qa, qb, counters, sah, tile_sums, nodes, order, small, dp_cost, dp_dec;
 };
 
 // Wavefront and traversal scratch of one stream.  Calls on one stream are
