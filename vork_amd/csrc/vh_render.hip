@@ -570,7 +570,7 @@ struct ShadowSource {
 };
 
 template <bool kCount>
-VH_TRACE_KERNEL void k_shadow(Paths P, Scene S, const float4* rays, const float4* rays_b, uint32_t* counters,
+VH_ANY_KERNEL void k_shadow(Paths P, Scene S, const float4* rays, const float4* rays_b, uint32_t* counters,
                                                    uint32_t* overflow) {
     __shared__ uint64_t lds_stack[kLdsStack * kBlock];
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;

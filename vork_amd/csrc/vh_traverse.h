@@ -48,11 +48,17 @@ constexpr uint32_t kItemGroup = 0x80000000u;
 #define VH_TRACE_ITEMS_PER_STEP 1
 #endif
 // Waves per SIMD the trace kernels are compiled for (bounds their VGPRs:
-// 512 / waves).
+// 512 / waves).  Any-hit rays keep less state (no barycentrics, ids or tie
+// rule), so the shadow kernel fits 6 waves with few spills; closest hit is
+// fastest at 4 (its spills grow faster than the latency hidden by more waves).
 #ifndef VH_TRACE_WAVES_PER_EU
 #define VH_TRACE_WAVES_PER_EU 4
 #endif
+#ifndef VH_ANY_WAVES_PER_EU
+#define VH_ANY_WAVES_PER_EU 6
+#endif
 #define VH_TRACE_KERNEL __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VH_TRACE_WAVES_PER_EU)))
+#define VH_ANY_KERNEL __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VH_ANY_WAVES_PER_EU)))
 constexpr uint32_t kRefillMin = VH_TRACE_REFILL_MIN;
 constexpr int kItemsPerStep = VH_TRACE_ITEMS_PER_STEP;
 
