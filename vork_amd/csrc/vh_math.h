@@ -48,6 +48,68 @@ VH_HD f3 lerp3(f3 a, f3 b, float s) { return a * (1.f - s) + b * s; }
 VH_HD f3 reflect3(f3 i, f3 n) { return i - n * (2.f * dot(n, i)); }
 VH_HD f3 load3(const float* p) { return mk(p[0], p[1], p[2]); }
 
+// Approximate arithmetic for shading, which is checked against the oracle
+// image by image (relMSE far below the noise level), not bit by bit: the
+// hardware reciprocal, square root, reciprocal square root, base-2
+// exponential and logarithm, and sine/cosine of revolutions (about 1 ulp),
+// in place of the correctly rounded sequences of 10-150 instructions each.
+// Ray generation, traversal, surface offsets and the film keep the exact
+// operators.  The host pass keeps the exact functions.
+VH_HD float rcp_a(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.f / x;
+#endif
+}
+VH_HD float sqrt_a(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_sqrtf(x);
+#else
+    return sqrtf(x);
+#endif
+}
+VH_HD float rsqrt_a(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_rsqf(x);
+#else
+    return 1.f / sqrtf(x);
+#endif
+}
+VH_HD float exp2_a(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_exp2f(x);
+#else
+    return exp2f(x);
+#endif
+}
+VH_HD float log2_a(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_logf(x);
+#else
+    return log2f(x);
+#endif
+}
+// sin(2 pi r) and cos(2 pi r) for |r| below 256 revolutions.
+VH_HD float sin_rev(float r) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_sinf(r);
+#else
+    return sinf(6.28318530717958647692f * r);
+#endif
+}
+VH_HD float cos_rev(float r) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_cosf(r);
+#else
+    return cosf(6.28318530717958647692f * r);
+#endif
+}
+VH_HD float div_a(float a, float b) { return a * rcp_a(b); }
+VH_HD f3 div_a(f3 a, float b) { return a * rcp_a(b); }
+VH_HD float length_a(f3 a) { return sqrt_a(dot(a, a)); }
+VH_HD f3 normalize_a(f3 a) { return a * rsqrt_a(dot(a, a)); }
+
 // Plain 4 x u32 vector (HIP's uint4 is a class whose members only take
 // generic-address-space objects).
 typedef uint32_t vu4 __attribute__((ext_vector_type(4)));
@@ -116,5 +178,6 @@ constexpr float kGamma3 = (3.f * (FLT_EPSILON * 0.5f)) / (1.f - 3.f * (FLT_EPSIL
 constexpr float kGamma7 = (7.f * (FLT_EPSILON * 0.5f)) / (1.f - 7.f * (FLT_EPSILON * 0.5f));
 constexpr float kPi = 3.14159265358979323846f;
 constexpr float kTau = 6.28318530717958647692f;
+constexpr float kInvPi = 0.318309886183790671538f;
 
 }  // namespace vh

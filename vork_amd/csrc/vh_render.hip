@@ -350,8 +350,10 @@ __device__ __forceinline__ f3 path_origin(const Surface& sf, f3 error, f3 direct
 }
 
 // Waves per SIMD k_shade is compiled for (bounds its VGPRs to 512 / waves).
+// Its time goes to waiting on the chain of scene loads, so it runs at the
+// most waves that keep its spills small (4: 20 B per lane; 5 spills 164 B).
 #ifndef VH_SHADE_WAVES_PER_EU
-#define VH_SHADE_WAVES_PER_EU 3
+#define VH_SHADE_WAVES_PER_EU 4
 #endif
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VH_SHADE_WAVES_PER_EU))) void k_shade(Paths P, Consts c, Scene S, const float4* rays, float4* next,
                                                   float4* shadow_rays, float4* shadow_b, uint32_t* counters) {
@@ -398,7 +400,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VH_SHADE
         if (c.guides && bounce == 0) {
             const f3 a = guide_albedo(S.luts, cl, view);
             P.guide_a[p] = make_float4(a.x, a.y, a.z, 1.f);
-            P.guide_n[p] = make_float4(sf.smooth.x, sf.smooth.y, sf.smooth.z, h.x / (h.x + 1.f));
+            P.guide_n[p] = make_float4(sf.smooth.x, sf.smooth.y, sf.smooth.z, div_a(h.x, h.x + 1.f));
         }
         BsdfSample bs = transport_sample(cl, view, scatter[0], scatter[1], scatter[2], pr);
         if (!finite3(bs.direction) || !finite3(bs.weight) || !isfinite(bs.pdf)) {
@@ -419,14 +421,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VH_SHADE
                 const vh_rect_light l = S.lights[li];
                 f3 ldir;
                 EmitEval src = rect_sample(l, sf.world, u[0], u[1], &ldir);
-                f3 power = src.pdf > 0.f ? src.radiance / src.pdf : mk(0.f, 0.f, 0.f);
+                f3 power = src.pdf > 0.f ? div_a(src.radiance, src.pdf) : mk(0.f, 0.f, 0.f);
                 f3 response = analytic_mis_response(cl, view, ldir, sf.face, src.pdf, pr);
-                a = (response * power) / probability;
+                a = div_a(response * power, probability);
                 float threshold = c.light_threshold;
                 float importance = maxc(a);
                 if (importance < threshold && threshold > 0.f) {
-                    float keep = importance / threshold;
-                    if (path_random(rng) >= keep) a = mk(0.f, 0.f, 0.f); else a = a / keep;
+                    float keep = div_a(importance, threshold);
+                    if (path_random(rng) >= keep) a = mk(0.f, 0.f, 0.f); else a = div_a(a, keep);
                 }
                 if (nonzero(a)) {
                     f3 origin = path_origin(sf, error, ldir);
@@ -442,11 +444,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VH_SHADE
                     EmitEval e = rect_eval(l, sf.world, bs.direction);
                     if (e.pdf > 0.f) {
                         float weight = bs.pdf > 0.f ? mis_power(bs.pdf, e.pdf) : 1.f;
-                        b = (bs.weight * e.radiance) * (weight / probability);
+                        b = (bs.weight * e.radiance) * div_a(weight, probability);
                         float imp2 = maxc(b);
                         if (imp2 < threshold && threshold > 0.f) {
-                            float keep = imp2 / threshold;
-                            if (path_random(rng) >= keep) b = mk(0.f, 0.f, 0.f); else b = b / keep;
+                            float keep = div_a(imp2, threshold);
+                            if (path_random(rng) >= keep) b = mk(0.f, 0.f, 0.f); else b = div_a(b, keep);
                         }
                         if (nonzero(b)) {
                             f3 origin = path_origin(sf, error, bs.direction);
@@ -491,7 +493,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VH_SHADE
             float d[4];
             group_values(smp, 1u + vertex * 5u + 4u, d);
             if (d[2] > survive) alive = false;
-            else throughput = throughput / survive;
+            else throughput = div_a(throughput, survive);
         }
         flags = (flags & kNonFinite) | (min(diffuse, 15u) << kDiffuseShift) | (min(glossy, 15u) << kGlossyShift) |
                 ((bounce + 1) << kBounceShift) | (event << kEventShift);

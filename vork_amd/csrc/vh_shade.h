@@ -113,52 +113,52 @@ VH_HD float specular_alpha(float r) { return fmaxf(r * r, kGgxAlphaMin); }  // o
 // openpbr_core.slang:13-21
 __device__ __forceinline__ float modulated_ior(float eta, float weight) {
     if (weight == 1.f || eta == 1.f) return eta;
-    float contrast = (eta - 1.f) / (eta + 1.f);
+    float contrast = div_a(eta - 1.f, eta + 1.f);
     float limit = u2f(0x3f7fffffu);
-    float epsilon = clampf(sqrtf(fmaxf(weight, 0.f)) * contrast, -limit, limit);
-    return (1.f + epsilon) / (1.f - epsilon);
+    float epsilon = clampf(sqrt_a(fmaxf(weight, 0.f)) * contrast, -limit, limit);
+    return div_a(1.f + epsilon, 1.f - epsilon);
 }
 // openpbr_core.slang:22-27
 VH_HD float fresnel(float c, float eta) {
     c = clampf(c, 0.f, 1.f);
     if (eta == 1.f) return 0.f;
-    float si2 = (1.f - c * c) / (eta * eta);
+    float si2 = div_a(1.f - c * c, eta * eta);
     if (si2 >= 1.f) return 1.f;
-    float ct = sqrtf(fmaxf(0.f, 1.f - si2));
-    float rs = (c - eta * ct) / (c + eta * ct), rp = (eta * c - ct) / (eta * c + ct);
+    float ct = sqrt_a(fmaxf(0.f, 1.f - si2));
+    float rs = div_a(c - eta * ct, c + eta * ct), rp = div_a(eta * c - ct, eta * c + ct);
     return (rs * rs + rp * rp) * .5f;
 }
 // openpbr_core.slang:105-116
 VH_HD float distribution(float nh, float alpha) {
     float x = (1.f - nh * nh) + nh * nh * alpha * alpha;
-    return alpha * alpha / (kPi * x * x);
+    return alpha * alpha * rcp_a(kPi * x * x);
 }
 VH_HD float smith_denominator(float no, float ni, float alpha) {
     float ci = fabsf(ni), a2 = alpha * alpha;
-    float po = sqrtf(a2 + (1.f - a2) * no * no), pi = sqrtf(a2 + (1.f - a2) * ci * ci);
+    float po = sqrt_a(a2 + (1.f - a2) * no * no), pi = sqrt_a(a2 + (1.f - a2) * ci * ci);
     return ci * po + no * pi;
 }
-VH_HD float g2(float no, float ni, float alpha) { return 2.f * no * fabsf(ni) / smith_denominator(no, ni, alpha); }
+VH_HD float g2(float no, float ni, float alpha) { return div_a(2.f * no * fabsf(ni), smith_denominator(no, ni, alpha)); }
 // openpbr_core.slang:126-138 (Heitz 2018)
 VH_HD f3 ggx_visible_normal(f3 view, float ax, float ay, float u0, float u1) {
-    f3 v = normalize(mk(ax * view.x, ay * view.y, view.z));
+    f3 v = normalize_a(mk(ax * view.x, ay * view.y, view.z));
     float ls = v.x * v.x + v.y * v.y;
-    f3 t = ls > 0.f ? mk(-v.y, v.x, 0.f) / sqrtf(ls) : mk(1.f, 0.f, 0.f);
+    f3 t = ls > 0.f ? mk(-v.y, v.x, 0.f) * rsqrt_a(ls) : mk(1.f, 0.f, 0.f);
     f3 b = cross(v, t);
-    float radius = sqrtf(u0), phi = 2.f * kPi * u1;
-    float x = radius * cosf(phi);
-    float y = lerpf(sqrtf(fmaxf(1.f - x * x, 0.f)), radius * sinf(phi), .5f * (1.f + v.z));
-    f3 p = t * x + b * y + v * sqrtf(fmaxf(1.f - x * x - y * y, 0.f));
-    return normalize(mk(ax * p.x, ay * p.y, fmaxf(p.z, 0.f)));
+    float radius = sqrt_a(u0);
+    float x = radius * cos_rev(u1);
+    float y = lerpf(sqrt_a(fmaxf(1.f - x * x, 0.f)), radius * sin_rev(u1), .5f * (1.f + v.z));
+    f3 p = t * x + b * y + v * sqrt_a(fmaxf(1.f - x * x - y * y, 0.f));
+    return normalize_a(mk(ax * p.x, ay * p.y, fmaxf(p.z, 0.f)));
 }
 // openpbr_core.slang:141-152
 VH_HD float ggx_visible_weight(float no, float ni, float alpha) {
     if (ni <= 0.f) return 0.f;
     float a2 = alpha * alpha;
-    return ni * (no + sqrtf(a2 + (1.f - a2) * no * no)) / smith_denominator(no, ni, alpha);
+    return div_a(ni * (no + sqrt_a(a2 + (1.f - a2) * no * no)), smith_denominator(no, ni, alpha));
 }
 VH_HD float ggx_visible_reflection(float no, float alpha, float u0, float u1, float* vh) {
-    f3 view = mk(sqrtf(fmaxf(1.f - no * no, 0.f)), 0.f, no);
+    f3 view = mk(sqrt_a(fmaxf(1.f - no * no, 0.f)), 0.f, no);
     f3 h = ggx_visible_normal(view, alpha, alpha, u0, u1);
     f3 light = reflect3(-view, h);
     *vh = clampf(dot(view, h), 0.f, 1.f);
@@ -166,7 +166,7 @@ VH_HD float ggx_visible_reflection(float no, float alpha, float u0, float u1, fl
 }
 // orthonormal_basis.slang:2-9 (Duff et al. 2017)
 VH_HD f3 basis_direction(f3 n, f3 v) {
-    float s = n.z >= 0.f ? 1.f : -1.f, a = -1.f / (s + n.z), b = n.x * n.y * a;
+    float s = n.z >= 0.f ? 1.f : -1.f, a = -rcp_a(s + n.z), b = n.x * n.y * a;
     f3 r0 = mk(1.f + s * n.x * n.x * a, s * b, -s * n.x), r1 = mk(b, s + n.y * n.y * a, -n.y);
     return r0 * v.x + r1 * v.y + n * v.z;
 }
@@ -193,14 +193,14 @@ __device__ __forceinline__ float4 bilinear4(const float4* g, uint32_t w, uint32_
 }
 // crates/vork-render/shaders/frame.slang:41-51
 __device__ __forceinline__ float4 brdf_moments(const Luts& L, float nv, float roughness) {
-    float ux = sqrtf(sqrtf(clampf(nv, 0.f, 1.f)));
-    float uy = log2f(clampf(roughness * roughness, kGgxAlphaMin, 1.f) / kGgxAlphaMin) / log2f(1.f / kGgxAlphaMin);
+    float ux = sqrt_a(sqrt_a(clampf(nv, 0.f, 1.f)));
+    float uy = log2_a(clampf(roughness * roughness, kGgxAlphaMin, 1.f) * (1.f / kGgxAlphaMin)) * (1.f / 13.2877123795494f);
     return bilinear4(L.moments, kBrdfSize, kBrdfSize, ux * (float)(kBrdfSize - 1), uy * (float)(kBrdfSize - 1));
 }
 // frame.slang:52-66
 __device__ __forceinline__ float dielectric_albedo(const Luts& L, float nv, float roughness, float eta) {
     if (eta == 1.f) return 0.f;
-    float at = eta / (1.f + eta) * (float)kBrdfIorSteps;
+    float at = div_a(eta, 1.f + eta) * (float)kBrdfIorSteps;
     uint32_t first = min((uint32_t)at, kBrdfIorSteps - 1);
     const uint32_t n = kBrdfAlbedoSize;
     float x = clampf(nv, 0.f, 1.f) * (float)(n - 1), y = clampf(roughness, 0.f, 1.f) * (float)(n - 1);
@@ -215,7 +215,8 @@ __device__ __forceinline__ f3 reflection_fresnel_at_ior(f3 base, float metal, fl
     float fd = fresnel(cosine, eta);
     f3 dielectric = mk(fd, fd, fd);
     if (metal <= 0.f) return dielectric;
-    float f = powf(1.f - cosine, 5.f);
+    const float x = 1.f - cosine, x2 = x * x;
+    float f = x2 * x2 * x;
     f3 m = mk(fmaxf(base.x + (1.f - base.x) * f, 0.f), fmaxf(base.y + (1.f - base.y) * f, 0.f),
               fmaxf(base.z + (1.f - base.z) * f, 0.f));
     return lerp3(dielectric, m * sw, metal);
@@ -239,19 +240,20 @@ __device__ __forceinline__ f3 metal_compensation(const Luts& L, const Closure& c
     float4 m = brdf_moments(L, fmaxf(dot(c.n, view), .0001f), c.roughness);
     float energy = m.x + m.y;
     float e_avg = brdf_moments(L, 0.f, c.roughness).w;
-    float missing = 1.f / fmaxf(energy, 1e-6f) - 1.f;
+    float missing = rcp_a(fmaxf(energy, 1e-6f)) - 1.f;
     f3 f0 = c.base_color;
-    f3 avg = mk(clampf(f0.x + (1.f - f0.x) / 21.f, 0.f, 1.f), clampf(f0.y + (1.f - f0.y) / 21.f, 0.f, 1.f),
-                clampf(f0.z + (1.f - f0.z) / 21.f, 0.f, 1.f));
-    return mk(missing * avg.x * e_avg / fmaxf(1.f - avg.x * (1.f - e_avg), 1e-6f),
-              missing * avg.y * e_avg / fmaxf(1.f - avg.y * (1.f - e_avg), 1e-6f),
-              missing * avg.z * e_avg / fmaxf(1.f - avg.z * (1.f - e_avg), 1e-6f));
+    const float k21 = 1.f / 21.f;
+    f3 avg = mk(clampf(f0.x + (1.f - f0.x) * k21, 0.f, 1.f), clampf(f0.y + (1.f - f0.y) * k21, 0.f, 1.f),
+                clampf(f0.z + (1.f - f0.z) * k21, 0.f, 1.f));
+    return mk(missing * avg.x * e_avg * rcp_a(fmaxf(1.f - avg.x * (1.f - e_avg), 1e-6f)),
+              missing * avg.y * e_avg * rcp_a(fmaxf(1.f - avg.y * (1.f - e_avg), 1e-6f)),
+              missing * avg.z * e_avg * rcp_a(fmaxf(1.f - avg.z * (1.f - e_avg), 1e-6f)));
 }
 // crates/vork-render/shaders/bsdf_response.slang:98-110
 __device__ __forceinline__ f3 direct_diffuse(const Luts& L, const Closure& c, f3 view) {
     if (c.metal < 1.f) {
         float albedo = dielectric_albedo(L, fmaxf(dot(c.n, view), .0001f), c.roughness, dielectric_ior(c));
-        return c.base_color * ((1.f - c.metal) * (1.f - albedo) / kPi);
+        return c.base_color * ((1.f - c.metal) * (1.f - albedo) * kInvPi);
     }
     return mk(0.f, 0.f, 0.f);
 }
@@ -282,10 +284,10 @@ __device__ __forceinline__ f3 valid_interface_normal(f3 normal, f3 geometric, f3
     f3 tangent = normal - geometric * dot(normal, geometric);
     float ts = dot(tangent, tangent);
     if (ts <= 0.f) return geometric;
-    f3 axis = tangent * (1.f / sqrtf(ts));
+    f3 axis = tangent * rsqrt_a(ts);
     float ix = dot(view, axis), ls = ix * ix + iz * iz;
-    float c2 = clampf(.5f * (1.f + (iz * threshold - ix * sqrtf(fmaxf(ls - threshold * threshold, 0.f))) / ls), 0.f, 1.f);
-    return axis * sqrtf(1.f - c2) + geometric * sqrtf(c2);
+    float c2 = clampf(.5f * (1.f + div_a(iz * threshold - ix * sqrt_a(fmaxf(ls - threshold * threshold, 0.f)), ls)), 0.f, 1.f);
+    return axis * sqrt_a(1.f - c2) + geometric * sqrt_a(c2);
 }
 // bsdf.slang:173-196
 __device__ __forceinline__ void transport_probabilities(const Closure& c, f3 view, float& p0, float& p1) {
@@ -296,8 +298,8 @@ __device__ __forceinline__ void transport_probabilities(const Closure& c, f3 vie
     float reflection = fmaxf(lerpf(dielectric, c.spec_weight, c.metal), 0.f);
     float total = diffuse + reflection;
     if (total > 0.f) {
-        float t = fmaxf(total, 1e-20f);
-        p0 = diffuse / t; p1 = reflection / t;
+        float t = rcp_a(fmaxf(total, 1e-20f));
+        p0 = diffuse * t; p1 = reflection * t;
     } else {
         p0 = p1 = 0.f;
     }
@@ -309,7 +311,7 @@ __device__ __forceinline__ Prep transport_prepare(const Luts& L, const Closure& 
     base.n = valid_interface_normal(c.n, geometric, view);
     transport_probabilities(base, view, r.p0, r.p1);
     float deviation = fabsf(dot(smooth, c.n));
-    r.bump_a2 = clampf((1.f - deviation * deviation) / fmaxf(8.f * deviation * deviation, 1e-20f), 0.f, 1.f);
+    r.bump_a2 = clampf(div_a(1.f - deviation * deviation, fmaxf(8.f * deviation * deviation, 1e-20f)), 0.f, 1.f);
     r.diffuse = direct_diffuse(L, base, view);
     r.metal_comp = metal_compensation(L, base, view);
     r.smooth = smooth;
@@ -322,12 +324,12 @@ __device__ __forceinline__ float bump_shadow(const Closure& c, f3 direction, con
     if (cosine * dot(pr.smooth, c.n) * dot(c.n, direction) < 0.f) return 0.f;
     if (pr.bump_a2 == 0.f) return 1.f;
     float mu = fabsf(cosine);
-    return mu > 0.f ? 2.f * mu / (mu + sqrtf(pr.bump_a2 + (1.f - pr.bump_a2) * mu * mu)) : 0.f;
+    return mu > 0.f ? div_a(2.f * mu, mu + sqrt_a(pr.bump_a2 + (1.f - pr.bump_a2) * mu * mu)) : 0.f;
 }
 // crates/vork-render/shaders/bsdf_microfacet.slang:42-62
 __device__ __forceinline__ float visible_scale(float roughness, f3 n, f3 view) {
     float nv = dot(n, view), a = specular_alpha(roughness);
-    return 2.f / (nv + sqrtf(a * a + (1.f - a * a) * nv * nv));
+    return 2.f * rcp_a(nv + sqrt_a(a * a + (1.f - a * a) * nv * nv));
 }
 __device__ __forceinline__ f3 visible_normal(float roughness, f3 n, f3 view, float u0, float u1) {
     f3 x = basis_direction(n, mk(1.f, 0.f, 0.f)), y = basis_direction(n, mk(0.f, 1.f, 0.f));
@@ -341,9 +343,9 @@ __device__ __forceinline__ float pdf_mask(const Closure& c, f3 view, f3 directio
     f3 n = pr.interface_n;
     float nl = dot(n, direction);
     float cd = dot(c.n, direction);
-    float pdf = (mask & VH_RAY_DIFFUSE) ? pr.p0 * (cd > 0.f ? 1.f : 0.f) * fabsf(cd) / kPi : 0.f;
+    float pdf = (mask & VH_RAY_DIFFUSE) ? pr.p0 * (cd > 0.f ? 1.f : 0.f) * fabsf(cd) * kInvPi : 0.f;
     if (nl <= 0.f) return pdf;
-    f3 h = normalize(view + direction);
+    f3 h = normalize_a(view + direction);
     float d = distribution(fmaxf(dot(n, h), 0.f), specular_alpha(c.roughness));
     return pdf + ((mask & VH_RAY_SPECULAR) ? pr.p1 * (d * visible_scale(c.roughness, n, view) / 4.f) : 0.f);
 }
@@ -355,13 +357,13 @@ __device__ __forceinline__ f3 eval_mask(const Closure& c, f3 view, f3 direction,
         f3 n = pr.interface_n;
         float nl = fmaxf(dot(n, direction), 0.f), nv = fmaxf(dot(n, view), .0001f);
         if (nl > 0.f) {
-            f3 h = normalize(view + direction);
+            f3 h = normalize_a(view + direction);
             float nh = fmaxf(dot(n, h), 0.f), vh = fmaxf(dot(view, h), 0.f);
             Closure base = c;
             base.n = n;
             f3 f = compensated_surface_reflection(base, vh, pr.metal_comp);
             float a = specular_alpha(c.roughness);
-            spec = f * (distribution(nh, a) * g2(nv, nl, a) / (4.f * nv * nl)) * nl;
+            spec = f * (distribution(nh, a) * g2(nv, nl, a) * rcp_a(4.f * nv * nl)) * nl;
         }
     }
     f3 diff = zero;
@@ -384,8 +386,8 @@ __device__ __forceinline__ BsdfSample transport_sample(const Closure& c, f3 view
     res.event = 0;
     f3 n = pr.interface_n;
     if (r2 < pr.p0) {
-        float radius = sqrtf(r0), phi = kTau * r1;
-        res.direction = basis_direction(c.n, mk(radius * cosf(phi), radius * sinf(phi), sqrtf(1.f - r0)));
+        float radius = sqrt_a(r0);
+        res.direction = basis_direction(c.n, mk(radius * cos_rev(r1), radius * sin_rev(r1), sqrt_a(1.f - r0)));
         res.event = VH_RAY_DIFFUSE;
     } else if (r2 < pr.p0 + pr.p1) {
         f3 h = visible_normal(c.roughness, n, view, r0, r1);
@@ -400,14 +402,14 @@ __device__ __forceinline__ BsdfSample transport_sample(const Closure& c, f3 view
         return res;
     }
     res.pdf = pdf_mask(c, view, res.direction, res.event, pr);
-    if (res.pdf > 0.f) res.weight = eval_mask(c, view, res.direction, res.event, pr) / res.pdf;
+    if (res.pdf > 0.f) res.weight = eval_mask(c, view, res.direction, res.event, pr) * rcp_a(res.pdf);
     return res;
 }
 // path.slang:833-850
 __device__ __forceinline__ float mis_power(float sampled, float other) {
     if (sampled <= 0.f) return 0.f;
-    float ratio = other / sampled;
-    return 1.f / (1.f + ratio * ratio);
+    float ratio = div_a(other, sampled);
+    return rcp_a(1.f + ratio * ratio);
 }
 __device__ __forceinline__ bool event_hemisphere(uint32_t event, f3 direction, f3 face, f3 smooth) {
     if (event == VH_RAY_TRANSMISSION) return dot(face, direction) < 0.f && dot(smooth, direction) < 0.f;
@@ -445,23 +447,25 @@ __device__ __forceinline__ EmitEval rect_eval(const vh_rect_light& l, f3 world, 
     if (square <= 0.f) return r;
     float denominator = dot(normal, direction);
     if (denominator == 0.f) return r;
-    float distance = dot(normal, center) / denominator;
+    float distance = div_a(dot(normal, center), denominator);
     if (distance <= 0.f) return r;
     f3 q = direction * distance - center;
     float uu = dot(u, u), uv = dot(u, v), vv = dot(v, v), qu = dot(q, u), qv = dot(q, v);
-    float cx = (qu * vv - qv * uv) / square, cy = (qv * uu - qu * uv) / square;
+    const float inv_square = rcp_a(square);
+    float cx = (qu * vv - qv * uv) * inv_square, cy = (qv * uu - qu * uv) * inv_square;
     if (fabsf(cx) > 1.f || fabsf(cy) > 1.f) return r;
-    float area = 4.f * sqrtf(square);
-    r.pdf = distance * distance / (area * fabsf(denominator) / sqrtf(square));
-    r.radiance = load3(l.radiance) / (kPi * area);
+    const float root = sqrt_a(square);
+    float area = 4.f * root;
+    r.pdf = div_a(distance * distance, area * fabsf(denominator) * rcp_a(root));
+    r.radiance = load3(l.radiance) * rcp_a(kPi * area);
     r.distance = distance;
     return r;
 }
 // emitter.slang:88-110 (area-uniform point) + 151-170
 __device__ __forceinline__ EmitEval rect_sample(const vh_rect_light& l, f3 world, float u0, float u1, f3* direction) {
     f3 point = load3(l.position) + (load3(l.u) * (u0 - .5f) + load3(l.v) * (u1 - .5f));
-    float distance = length(point - world);
-    *direction = distance > 0.f ? (point - world) / distance : mk(0.f, 0.f, 0.f);
+    float distance = length_a(point - world);
+    *direction = distance > 0.f ? (point - world) * rcp_a(distance) : mk(0.f, 0.f, 0.f);
     EmitEval e = rect_eval(l, world, *direction);
     e.distance = distance;
     return e;
@@ -475,7 +479,7 @@ struct LightNode {
 // crates/vork-render/shaders/light_tree.slang:18-36
 __device__ __forceinline__ float tree_log_importance(const LightNode& n, f3 point) {
     if (!(n.lower_power.w > 0.f)) return -FLT_MAX;
-    float power = log2f(n.lower_power.w);
+    float power = log2_a(n.lower_power.w);
     if (n.upper_infinite.w != 0.f) return power;
     f3 lo = mk(n.lower_power.x, n.lower_power.y, n.lower_power.z);
     f3 hi = mk(n.upper_infinite.x, n.upper_infinite.y, n.upper_infinite.z);
@@ -484,16 +488,16 @@ __device__ __forceinline__ float tree_log_importance(const LightNode& n, f3 poin
     f3 delta = point - center;
     float distance = fmaxf(kLightTreeMinDistanceSquared, fmaxf(dot(delta, delta), dot(radius, radius)));
     if (!isfinite(distance)) return -FLT_MAX;
-    return power - log2f(distance);
+    return power - log2_a(distance);
 }
 __device__ __forceinline__ float tree_branch(const LightNode& a, const LightNode& b, f3 point) {
     float x = tree_log_importance(a, point), y = tree_log_importance(b, point);
     float scale = fmaxf(x, y);
     float uniform = (float)a.links.w / ((float)a.links.w + (float)b.links.w);
     if (scale == -FLT_MAX) return uniform;
-    x = exp2f(x - scale);
-    y = exp2f(y - scale);
-    return x / (x + y);
+    x = exp2_a(x - scale);
+    y = exp2_a(y - scale);
+    return div_a(x, x + y);
 }
 // light_tree.slang:58-84
 __device__ __forceinline__ uint32_t light_pick(const LightNode* nodes, uint32_t node_count, f3 point, float ux, float uy,
@@ -517,7 +521,7 @@ __device__ __forceinline__ uint32_t light_pick(const LightNode* nodes, uint32_t 
             if (!left) rank -= cnt;
         } else {
             left = choice < p;
-            choice = left ? choice / p : (choice - p) / (1.f - p);
+            choice = left ? div_a(choice, p) : div_a(choice - p, 1.f - p);
             choice = fminf(choice, one);
         }
         prob *= left ? p : 1.f - p;
@@ -554,16 +558,16 @@ __device__ __forceinline__ void surface_ray_connection(f3 origin, f3 world, f3 d
         return;
     }
     f3 delta = direction * distance - (origin - world);
-    float remaining = length(delta);
-    *dir_out = remaining > 0.f ? delta / remaining : direction;
+    float remaining = length_a(delta);
+    *dir_out = remaining > 0.f ? delta * rcp_a(remaining) : direction;
     *dist_out = remaining;
 }
 // stream.slang:416-427
 __device__ __forceinline__ f3 finite_unit_normal(f3 normal, f3 fallback) {
     float scale = fmaxf(fabsf(normal.x), fmaxf(fabsf(normal.y), fabsf(normal.z)));
-    if (finite3(normal) && scale > 0.f) return normalize(normal / scale);
+    if (finite3(normal) && scale > 0.f) return normalize_a(normal * rcp_a(scale));
     float fs = fmaxf(fabsf(fallback.x), fmaxf(fabsf(fallback.y), fabsf(fallback.z)));
-    if (finite3(fallback) && fs > 0.f) return normalize(fallback / fs);
+    if (finite3(fallback) && fs > 0.f) return normalize_a(fallback * rcp_a(fs));
     return mk(0.f, 0.f, 1.f);
 }
 // sample_clamp.slang:7-16
@@ -574,16 +578,18 @@ __device__ __forceinline__ f3 clamp_sample(f3 value, float limit, uint32_t& drop
         value = mk(0.f, 0.f, 0.f);
         brightness = 0.f;
     }
-    return value * ((limit > 0.f && brightness > limit) ? limit / fmaxf(brightness, 1e-20f) : 1.f);
+    return value * ((limit > 0.f && brightness > limit) ? div_a(limit, fmaxf(brightness, 1e-20f)) : 1.f);
 }
 // Escaping rays: constant or procedural sky.
 __device__ __forceinline__ f3 sky_radiance(const vh_sky& k, f3 d) {
     if (k.kind == VH_SKY_CONSTANT) return load3(k.color);
     float y = d.y;
-    f3 base = y >= 0.f ? lerp3(load3(k.color), load3(k.zenith), sqrtf(y))
+    f3 base = y >= 0.f ? lerp3(load3(k.color), load3(k.zenith), sqrt_a(y))
                        : lerp3(load3(k.color), load3(k.ground), fminf(1.f, -4.f * y));
     float cs = fmaxf(dot(d, load3(k.sun_direction)), 0.f);
-    return base + load3(k.sun_radiance) * powf(cs, k.sun_exponent);
+    // cs^e as 2^(e log2 cs); 0^0 is 1 as in powf
+    const float sun = k.sun_exponent == 0.f ? 1.f : exp2_a(k.sun_exponent * log2_a(cs));
+    return base + load3(k.sun_radiance) * sun;
 }
 
 }  // namespace vh
