@@ -77,6 +77,7 @@ struct Consts {
     float clamp_direct, clamp_indirect, light_threshold;
     uint32_t light_count, light_nodes, material_count;
     uint32_t batch_first, batch_samples, pixels;
+    uint32_t group;             // samples of one pixel in consecutive path slots (see path_slot)
     uint32_t minimum, decide;   // adaptive: first tested sample index; batch follows a decide pass
     float threshold;
     uint32_t guides;            // write the denoiser guides
@@ -84,14 +85,39 @@ struct Consts {
     vh_sky sky;
 };
 
-// Path slot t of a sample -> pixel: column-major inside bands of 8 rows, so
-// that 64 consecutive slots (one wave) cover an 8x8 tile.  A bijection on
-// [0, width*height) for any image size (the last band may be shorter).
+// Pixel slot t -> pixel: column-major inside bands of 8 rows, so that
+// consecutive slots are neighbouring pixels (64 of them an 8x8 tile).  A
+// bijection on [0, width*height) for any image size (the last band may be
+// shorter).
 __device__ __forceinline__ uint32_t slot_to_pixel(uint32_t t, uint32_t w, uint32_t h) {
     const uint32_t band = t / (8u * w), r = t % (8u * w);
     const uint32_t rows = min(8u, h - band * 8u);
     const uint32_t x = r / rows, y = band * 8u + r % rows;
     return y * w + x;
+}
+
+// Samples of one pixel that share consecutive path slots: the largest power of
+// two dividing the batch's sample count, at most VH_WAVE_SAMPLES.  A wave of
+// camera rays then holds the jittered samples of a few neighbouring pixels,
+// which walk nearly the same BVH nodes, so its node fetches touch few cache
+// lines and its lanes stay in step.
+#ifndef VH_WAVE_SAMPLES
+#define VH_WAVE_SAMPLES 64
+#endif
+inline uint32_t sample_group(uint32_t batch_samples) {
+    const uint32_t g = batch_samples & (0u - batch_samples);
+    return g < VH_WAVE_SAMPLES ? g : VH_WAVE_SAMPLES;
+}
+// Path slot of sample s (of the batch) of pixel slot t: groups of `group`
+// consecutive slots hold `group` samples of one pixel slot, and a whole
+// group of samples covers every pixel slot before the next one starts.
+__device__ __forceinline__ uint32_t path_slot(uint32_t s, uint32_t t, uint32_t group, uint32_t pixels) {
+    return (s / group) * group * pixels + t * group + s % group;
+}
+__device__ __forceinline__ void slot_sample(uint32_t p, uint32_t group, uint32_t pixels, uint32_t& s, uint32_t& t) {
+    const uint32_t span = group * pixels, r = p % span;
+    s = (p / span) * group + r % group;
+    t = r / group;
 }
 
 // Per-pixel adaptive moment (PathMoment, path.slang:57): running mean and
@@ -196,7 +222,9 @@ __global__ void k_camera(Paths P, Consts c, Scene S, uint32_t n, float4* rays, u
         reset_claims(counters);
     }
     if (p >= n) return;
-    uint32_t s_local = p / c.pixels, pixel = slot_to_pixel(p % c.pixels, c.width, c.height);
+    uint32_t s_local, t;
+    slot_sample(p, c.group, c.pixels, s_local, t);
+    const uint32_t pixel = slot_to_pixel(t, c.width, c.height);
     if (kAdaptive && c.decide && M[pixel].stopped) return;
     uint32_t px = pixel % c.width, py = pixel / c.width;
     uint32_t index = c.batch_first + s_local;
@@ -606,30 +634,54 @@ __global__ void k_resolve(Paths P, uint32_t* counters) {
 
 // One thread per pixel: adds the batch's samples, in sample order, to the
 // film (resolve.slang:228-238).  A sample is (radiance, 1), or zero when it
-// was non-finite (path.slang:1948-1952).  Threads walk the path slots in order
-// (consecutive slots hold an 8x8 pixel tile), so the path-state reads are
-// contiguous.  Adaptive batches also update the pixel's moment, add a stopped
+// was non-finite (path.slang:1948-1952).  When a pixel's samples sit in
+// groups of 8 or more consecutive path slots, the block stages 8 samples of
+// its 256 pixels at a time in LDS with loads that read whole cache lines (a
+// thread per pixel would read one 16-byte value per line per sample); the
+// per-pixel sums keep sample order.  Adaptive batches also update the pixel's moment, add a stopped
 // pixel's mean in place of a sample, and record the settled test of sample
 // indices up to `minimum` (later ones are the decide pass's).  With `guides`
 // it also sums the camera hits' guides (path.slang:1956-1961): albedo and
 // coverage at [2 pixel], normal and depth at [2 pixel + 1]; a dropped sample
 // adds none, and a stopped pixel's sums grow by (i + 1) / i so they stay
 // sums over every sample (path.slang:1806-1809).
+constexpr uint32_t kFilmStage = 8, kFilmPad = kFilmStage + 1;
 template <bool kAdaptive>
-__global__ void k_film(Paths P, Consts c, float4* film, float4* guides, Moment* M, uint32_t* counters) {
+__global__ __launch_bounds__(kBlock) void k_film(Paths P, Consts c, float4* film, float4* guides, Moment* M,
+                                                 uint32_t* counters) {
+    __shared__ float4 s_rad[kBlock * kFilmPad];
+    __shared__ uint32_t s_flags[kBlock * kFilmStage];
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot >= c.pixels) return;
-    const uint32_t pixel = slot_to_pixel(slot, c.width, c.height);
-    float4 acc = film[pixel];
-    float4 g0 = make_float4(0.f, 0.f, 0.f, 0.f), g1 = g0;
-    if (guides) {
-        g0 = guides[2 * pixel];
-        g1 = guides[2 * pixel + 1];
-    }
+    const bool valid = slot < c.pixels;
+    const bool staged = c.group >= kFilmStage;  // the same for the whole launch
+    const uint32_t pixel = valid ? slot_to_pixel(slot, c.width, c.height) : 0u;
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 g0 = acc, g1 = acc;
     Moment m;
-    if (kAdaptive) m = M[pixel];
+    if (valid) {
+        acc = film[pixel];
+        if (guides) {
+            g0 = guides[2 * pixel];
+            g1 = guides[2 * pixel + 1];
+        }
+        if (kAdaptive) m = M[pixel];
+    }
     uint32_t bad_count = 0, stopped_count = 0;
     for (uint32_t s = 0; s < c.batch_samples; s++) {
+        if (staged && s % kFilmStage == 0) {
+            __syncthreads();
+            const uint32_t t0 = blockIdx.x * blockDim.x;
+            for (uint32_t j = 0; j < kFilmStage; j++) {
+                const uint32_t e = threadIdx.x + kBlock * j, ts = t0 + e / kFilmStage, ss = s + e % kFilmStage;
+                if (ts < c.pixels && ss < c.batch_samples) {
+                    const uint32_t q = path_slot(ss, ts, c.group, c.pixels);
+                    s_rad[(e / kFilmStage) * kFilmPad + e % kFilmStage] = P.rad[q];
+                    s_flags[e] = P.info[q].w;
+                }
+            }
+            __syncthreads();
+        }
+        if (!valid) continue;
         const uint32_t index = c.batch_first + s;
         if (kAdaptive) {
             if (index == 0) {
@@ -653,9 +705,10 @@ __global__ void k_film(Paths P, Consts c, float4* film, float4* guides, Moment* 
             if (index <= c.minimum)
                 m.settled[index & 1u] = index >= c.minimum && moment_settled(m, index, c.threshold) ? 1u : 0u;
         }
-        const uint32_t p = s * c.pixels + slot;
-        const float4 r = P.rad[p];
-        const bool bad = (P.info[p].w & kNonFinite) || !isfinite(r.x) || !isfinite(r.y) || !isfinite(r.z);
+        const uint32_t p = path_slot(s, slot, c.group, c.pixels);
+        const float4 r = staged ? s_rad[threadIdx.x * kFilmPad + s % kFilmStage] : P.rad[p];
+        const uint32_t flags = staged ? s_flags[threadIdx.x * kFilmStage + s % kFilmStage] : P.info[p].w;
+        const bool bad = (flags & kNonFinite) || !isfinite(r.x) || !isfinite(r.y) || !isfinite(r.z);
         const float4 v = bad ? make_float4(0.f, 0.f, 0.f, 0.f) : make_float4(r.x, r.y, r.z, 1.f);
         bad_count += bad ? 1u : 0u;
         acc.x += v.x;
@@ -669,6 +722,7 @@ __global__ void k_film(Paths P, Consts c, float4* film, float4* guides, Moment* 
         }
         if (kAdaptive) moment_add(m, v, index);
     }
+    if (!valid) return;
     film[pixel] = acc;
     if (guides) {
         guides[2 * pixel] = g0;
@@ -918,6 +972,7 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, float* g
         }
         c.batch_first = first;
         c.batch_samples = nb;
+        c.group = sample_group(nb);
         const uint32_t n = pixels * nb;
         float4* qa = st->rays_a.as<float4>();
         float4* qb = st->rays_b.as<float4>();
