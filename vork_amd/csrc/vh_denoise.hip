@@ -4,12 +4,16 @@
 //
 // Activations are NHWC bf16 with a multiple of 32 channels, so one K step of
 // v_mfma_f32_16x16x32_bf16 is one tap over 32 input channels.  A workgroup
-// (4 waves) owns an 8 x 16 pixel tile and every output channel; for each
+// (4 waves) owns an 8 x 16 pixel tile and its output channels (all of them,
+// or a 32-channel slice per blockIdx.y on the coarse levels, whose tiles alone
+// would leave most CUs idle); for each
 // block of 32 input channels it stages the (8+2) x (16+2) halo of the tile in
 // LDS (fusing the nearest upsample and the skip concatenation of the decoder),
 // then each wave runs 9 taps of MFMAs over its share of the tile's 8 pixel
 // rows and of the output channels, with weights read as 16-byte fragments.
-// The epilogue adds the bias, applies ReLU and rounds to bf16.
+// The epilogue adds the bias, applies ReLU and rounds to bf16; on the encoder
+// it also writes the 2x2 average pool of the rounded output (the next level's
+// input), whose 2x2 groups never straddle a tile or a wave's rows.
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -41,6 +45,8 @@ struct ConvArgs {
     const float* bias;   // [cout]
     uint16_t* y;         // [h][w][cout]
     int h, w, c1, c2, up1, relu;
+    int cout;            // output channels of the layer (the block computes kCout of them)
+    uint16_t* pool;      // [(h + 1) / 2][(w + 1) / 2][cout] 2x2 average of y, or null
 };
 
 // One 16-byte chunk of channels [ch, ch + 8) of halo pixel (hy, hx) of the tile
@@ -55,7 +61,8 @@ __device__ __forceinline__ uint4 fetch_chunk(const ConvArgs& a, int y, int x, in
     return *reinterpret_cast<const uint4*>(a.x2 + ((size_t)y * a.w + x) * a.c2 + (ch - a.c1));
 }
 
-// kCout output channels; waves split as kMW (pixel rows) x kNW (channel blocks).
+// kCout output channels from blockIdx.y * kCout; waves split as kMW (pixel
+// rows) x kNW (channel blocks).
 template <int kCout, int kMW>
 __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
     constexpr int kNB = kCout / 16, kNW = 4 / kMW, kNBW = kNB / kNW, kRows = kTileH / kMW;
@@ -66,6 +73,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
     const int tiles_x = (a.w + kTileW - 1) / kTileW;
     const int y0 = (blockIdx.x / tiles_x) * kTileH, x0 = (blockIdx.x % tiles_x) * kTileW;
     const int cin = a.c1 + a.c2;
+    const int co0 = blockIdx.y * kCout;
     f32x4 acc[kRows][kNBW];
 #pragma unroll
     for (int r = 0; r < kRows; r++)
@@ -86,9 +94,9 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
             bf16x8 b[kNBW];
 #pragma unroll
             for (int n = 0; n < kNBW; n++) {
-                const int co = (wn * kNBW + n) * 16 + col;
+                const int co = co0 + (wn * kNBW + n) * 16 + col;
                 b[n] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(
-                                                      a.wt + ((size_t)tap * kCout + co) * cin + cb + kg * 8));
+                                                      a.wt + ((size_t)tap * a.cout + co) * cin + cb + kg * 8));
             }
 #pragma unroll
             for (int r = 0; r < kRows; r++) {
@@ -101,46 +109,53 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
         }
     }
     // D fragment: lane holds output channel (block + col) of pixels 4 kg + j of the row
+    static_assert(kRows % 2 == 0, "pooling pairs rows within a wave");
 #pragma unroll
     for (int r = 0; r < kRows; r++) {
         const int y = y0 + wm * kRows + r;
-        if (y >= a.h) continue;
 #pragma unroll
         for (int n = 0; n < kNBW; n++) {
-            const int co = (wn * kNBW + n) * 16 + col;
+            const int co = co0 + (wn * kNBW + n) * 16 + col;
             const float bias = a.bias[co];
 #pragma unroll
             for (int j = 0; j < 4; j++) {
                 const int x = x0 + kg * 4 + j;
-                if (x >= a.w) continue;
                 float v = acc[r][n][j] + bias;
                 if (a.relu) v = fmaxf(v, 0.f);
-                a.y[((size_t)y * a.w + x) * kCout + co] = f2bf(v);
+                const uint16_t h16 = f2bf(v);
+                acc[r][n][j] = bf2f(h16);  // the stored value, for the pool below
+                if (y < a.h && x < a.w) a.y[((size_t)y * a.w + x) * a.cout + co] = h16;
             }
         }
     }
-}
-
-// 2x2 average pool, NHWC bf16; the last row / column of an odd size averages
-// what exists.
-__global__ void k_pool2(const uint16_t* x, uint16_t* y, int h, int w, int c) {
-    const int oh = (h + 1) / 2, ow = (w + 1) / 2;
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)oh * ow * c) return;
-    const int ch = (int)(i % c);
-    const size_t p = i / c;
-    const int oy = (int)(p / ow), ox = (int)(p % ow);
-    float s = 0.f;
-    int n = 0;
-    for (int dy = 0; dy < 2; dy++)
-        for (int dx = 0; dx < 2; dx++) {
-            const int yy = 2 * oy + dy, xx = 2 * ox + dx;
-            if (yy < h && xx < w) {
-                s += bf2f(x[((size_t)yy * w + xx) * c + ch]);
-                n++;
+    if (a.pool) {
+        const int ow = (a.w + 1) / 2;
+#pragma unroll
+        for (int r = 0; r < kRows; r += 2) {
+            const int y = y0 + wm * kRows + r;
+            if (y >= a.h) continue;
+#pragma unroll
+            for (int n = 0; n < kNBW; n++) {
+                const int co = co0 + (wn * kNBW + n) * 16 + col;
+#pragma unroll
+                for (int j = 0; j < 4; j += 2) {
+                    const int x = x0 + kg * 4 + j;
+                    if (x >= a.w) continue;
+                    // sum in the order (0,0), (0,1), (1,0), (1,1); the last row / column of
+                    // an odd size averages what exists
+                    float s = acc[r][n][j];
+                    int cnt = 1;
+                    if (x + 1 < a.w) { s += acc[r][n][j + 1]; cnt++; }
+                    if (y + 1 < a.h) {
+                        s += acc[r + 1][n][j];
+                        cnt++;
+                        if (x + 1 < a.w) { s += acc[r + 1][n][j + 1]; cnt++; }
+                    }
+                    a.pool[((size_t)(y / 2) * ow + x / 2) * a.cout + co] = f2bf(s / (float)cnt);
+                }
             }
         }
-    y[i] = f2bf(s / (float)n);
+    }
 }
 
 // Input features from the film and guide sums of `samples` samples: log1p
@@ -203,14 +218,22 @@ struct vh_denoiser {
 namespace vh {
 namespace {
 
-int launch_conv(const ConvArgs& a, int cout, hipStream_t s) {
-    const uint32_t grid = nblocks(a.w, kTileW) * nblocks(a.h, kTileH);
-    switch (cout) {
-        case 16: hipLaunchKernelGGL((k_conv3x3<16, 4>), dim3(grid), dim3(256), 0, s, a); break;
-        case 32: hipLaunchKernelGGL((k_conv3x3<32, 2>), dim3(grid), dim3(256), 0, s, a); break;
-        case 64: hipLaunchKernelGGL((k_conv3x3<64, 1>), dim3(grid), dim3(256), 0, s, a); break;
-        case 96: hipLaunchKernelGGL((k_conv3x3<96, 2>), dim3(grid), dim3(256), 0, s, a); break;
-        case 128: hipLaunchKernelGGL((k_conv3x3<128, 1>), dim3(grid), dim3(256), 0, s, a); break;
+// Layers whose tiles give fewer than kSplitBlocks blocks per CU split their
+// output channels into 32-channel slices (more blocks, each staging the same
+// halo).
+#ifndef VH_DENOISE_SPLIT_BLOCKS
+#define VH_DENOISE_SPLIT_BLOCKS 4
+#endif
+int launch_conv(const ConvArgs& a, uint32_t cu_count, hipStream_t s) {
+    const uint32_t tiles = nblocks(a.w, kTileW) * nblocks(a.h, kTileH);
+    const bool split = a.cout > 32 && tiles < VH_DENOISE_SPLIT_BLOCKS * cu_count;
+    const dim3 grid(tiles, split ? a.cout / 32 : 1);
+    switch (split ? 32 : a.cout) {
+        case 16: hipLaunchKernelGGL((k_conv3x3<16, 4>), grid, dim3(256), 0, s, a); break;
+        case 32: hipLaunchKernelGGL((k_conv3x3<32, 2>), grid, dim3(256), 0, s, a); break;
+        case 64: hipLaunchKernelGGL((k_conv3x3<64, 1>), grid, dim3(256), 0, s, a); break;
+        case 96: hipLaunchKernelGGL((k_conv3x3<96, 2>), grid, dim3(256), 0, s, a); break;
+        case 128: hipLaunchKernelGGL((k_conv3x3<128, 1>), grid, dim3(256), 0, s, a); break;
         default: return fail(VH_ERR_USAGE, "denoiser: unsupported channel count");
     }
     return VH_OK;
@@ -243,14 +266,15 @@ int denoise_run(vh_denoiser* dn, int W, int H, float samples, const float* film,
     const uint16_t* wts = dn->weights.as<uint16_t>();
     const float* bs = dn->biases.as<float>();
     int layer = 0, rc;
-    auto conv = [&](int l, const uint16_t* x1, int c1, int up1, const uint16_t* x2, int c2, uint16_t* y, bool relu) {
+    auto conv = [&](int l, const uint16_t* x1, int c1, int up1, const uint16_t* x2, int c2, uint16_t* y, bool relu,
+                    uint16_t* pool = nullptr) {
         ConvArgs a;
-        a.x1 = x1; a.x2 = x2; a.c1 = c1; a.c2 = c2; a.up1 = up1;
+        a.x1 = x1; a.x2 = x2; a.c1 = c1; a.c2 = c2; a.up1 = up1; a.pool = pool;
         a.wt = wts + dn->w_off[layer]; a.bias = bs + dn->b_off[layer];
         a.y = y; a.h = h[l]; a.w = w[l]; a.relu = relu ? 1 : 0;
-        const int cout = kShape[layer][1];
+        a.cout = kShape[layer][1];
         layer++;
-        return launch_conv(a, cout, s);
+        return launch_conv(a, (uint32_t)dn->device->cu_count, s);
     };
     hipLaunchKernelGGL(k_features, dim3(nblocks(px(0), 256)), dim3(256), 0, s, reinterpret_cast<const float4*>(film),
                        reinterpret_cast<const float4*>(guides), samples, (int)px(0), at(o_feat));
@@ -258,13 +282,10 @@ int denoise_run(vh_denoiser* dn, int W, int H, float samples, const float* film,
     const uint16_t* in = at(o_feat);
     int cin = 32;
     for (int l = 0; l < 5; l++) {
-        if (l > 0) {
-            hipLaunchKernelGGL(k_pool2, dim3(nblocks(px(l) * cin, 256)), dim3(256), 0, s, at(o_enc[l - 1]), at(o_pool),
-                               h[l - 1], w[l - 1], cin);
-            in = at(o_pool);
-        }
         if ((rc = conv(l, in, cin, 0, nullptr, 0, at(o_tmp), true))) return rc;
-        if ((rc = conv(l, at(o_tmp), C[l], 0, nullptr, 0, at(o_enc[l]), true))) return rc;
+        // levels 0-3 also write the next level's input, the 2x2 pool of their output
+        if ((rc = conv(l, at(o_tmp), C[l], 0, nullptr, 0, at(o_enc[l]), true, l < 4 ? at(o_pool) : nullptr))) return rc;
+        in = at(o_pool);
         cin = C[l];
     }
     // decoder: nearest upsample of the coarser level fused with the skip
