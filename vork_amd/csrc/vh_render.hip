@@ -388,22 +388,39 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VH_SHADE
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, threads = gridDim.x * blockDim.x;
     const uint32_t n = counters[0];
     uint32_t dropped = 0;
-    for (uint32_t i = gid; i < n; i += threads) {
-        const float4 rd4 = rays[2 * i + 1];
-        const uint32_t p = f2u(rd4.w) & kPathMask;
-        uint4 info = P.info[p];
+    uint32_t i = gid;
+    while (true) {
+        // A miss only adds the sky.  Each lane walks its entries (i, i + threads,
+        // ...) past misses until it holds a hit, so the wave's shading steps run
+        // on hits: on the later bounces most extension rays miss, and a step
+        // costs the same however few of its lanes hold a hit.
+        float4 rd4, t4, r4, h;
+        uint4 info;
+        uint32_t p = 0;
+        bool held = false;
+        while (i < n) {
+            rd4 = rays[2 * i + 1];
+            i += threads;
+            p = f2u(rd4.w) & kPathMask;
+            info = P.info[p];
+            t4 = P.thr[p];
+            r4 = P.rad[p];
+            h = P.hit[p];
+            if (h.x >= 0.f) {
+                held = true;
+                break;
+            }
+            const float sky_limit = ((info.w >> kBounceShift) & 0xffu) == 0 ? c.clamp_direct : c.clamp_indirect;
+            const f3 sky = clamp_sample(mk(t4.x, t4.y, t4.z) * sky_radiance(c.sky, mk(rd4.x, rd4.y, rd4.z)),
+                                        sky_limit, dropped);
+            P.rad[p] = make_float4(r4.x + sky.x, r4.y + sky.y, r4.z + sky.z, r4.w);
+        }
+        if (!held) break;
         uint32_t flags = info.w;
         const uint32_t bounce = (flags >> kBounceShift) & 0xffu;
         uint32_t diffuse = (flags >> kDiffuseShift) & 0xfu, glossy = (flags >> kGlossyShift) & 0xfu;
-        float4 t4 = P.thr[p], r4 = P.rad[p];
-        f3 throughput = mk(t4.x, t4.y, t4.z), radiance = mk(r4.x, r4.y, r4.z), direction = mk(rd4.x, rd4.y, rd4.z);
+        f3 throughput = mk(t4.x, t4.y, t4.z), direction = mk(rd4.x, rd4.y, rd4.z);
         const float limit = bounce == 0 ? c.clamp_direct : c.clamp_indirect;
-        float4 h = P.hit[p];
-        if (h.x < 0.f) {
-            radiance = radiance + clamp_sample(throughput * sky_radiance(c.sky, direction), limit, dropped);
-            P.rad[p] = make_float4(radiance.x, radiance.y, radiance.z, r4.w);
-            continue;
-        }
         const uint32_t inst = P.hit_inst[p], prim = f2u(h.w);
         const InstanceRecord& ir = S.tv.instances[inst];
         const BlasRecord& br = S.tv.blases[ir.blas];
