@@ -86,6 +86,23 @@ def test_seed_determinism_and_sample_range_linearity(renderer):
     np.testing.assert_allclose(inst.render(p3), a, rtol=2e-6, atol=1e-6)
 
 
+def test_batch_layouts_give_the_same_film(renderer):
+    # a batch of B samples keeps the largest power of two dividing B of a
+    # pixel's samples in consecutive path slots (1, 2 and 16 here; 16 also
+    # takes the film kernel's staged path); every layout sums a pixel's samples
+    # in sample order, so the films are bit-identical
+    sc = S.config0(width=40, height=24)
+    inst = renderer.install(sc)
+    films = []
+    for batch in (1, 2, 6, 16, 0):
+        p = sc.render_params(samples=16, seed=11, adaptive=False)
+        p.batch_samples = batch
+        films.append(inst.render(p))
+    for f in films[1:]:
+        assert np.array_equal(f, films[0])
+    assert films[0][..., 3].min() > 0
+
+
 def test_device_film_accumulates(renderer):
     torch = pytest.importorskip("torch")
     sc = S.config0(width=32, height=32)
