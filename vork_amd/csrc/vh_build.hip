@@ -725,7 +725,8 @@ __global__ void k_tri_bounds(const float* pos, const uint32_t* idx, uint32_t n, 
         hi[3 * i + k] = fmaxf(x, fmaxf(y, z));
     }
 }
-__global__ void k_gather_triangles(const float* pos, const uint32_t* idx, const uint32_t* order, uint32_t n, float* tris,
+__global__ void k_gather_triangles(const float* pos, const float* nrm, const uint32_t* mat, const uint32_t* idx,
+                                   const uint32_t* order, uint32_t n, float* tris, float* leaf_nrm, uint32_t* leaf_mat,
                                    uint32_t* tri_ids) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -733,7 +734,10 @@ __global__ void k_gather_triangles(const float* pos, const uint32_t* idx, const 
     for (int v = 0; v < 3; v++) {
         uint32_t vi = idx[3 * prim + v];
         for (int k = 0; k < 3; k++) tris[9 * (size_t)i + 3 * v + k] = pos[3 * vi + k];
+        if (nrm)
+            for (int k = 0; k < 3; k++) leaf_nrm[9 * (size_t)i + 3 * v + k] = nrm[3 * vi + k];
     }
+    if (mat) leaf_mat[i] = mat[prim];
     tri_ids[i] = prim;
 }
 __global__ void k_instance_bounds(const InstanceRecord* inst, const BlasRecord* blases, uint32_t n, float* lo, float* hi) {
@@ -853,10 +857,11 @@ int launch_triangle_bounds(hipStream_t s, BuildScratch& B, const float* position
     return VH_OK;
 }
 
-int launch_gather_triangles(hipStream_t s, const float* positions, const uint32_t* indices, const uint32_t* order,
-                            uint32_t n, float* tris, uint32_t* tri_ids) {
-    hipLaunchKernelGGL(k_gather_triangles, dim3(blocks(n, 256)), dim3(256), 0, s, positions, indices, order, n, tris,
-                       tri_ids);
+int launch_gather_triangles(hipStream_t s, const float* positions, const float* normals, const uint32_t* materials,
+                            const uint32_t* indices, const uint32_t* order, uint32_t n, float* tris,
+                            float* leaf_normals, uint32_t* leaf_materials, uint32_t* tri_ids) {
+    hipLaunchKernelGGL(k_gather_triangles, dim3(blocks(n, 256)), dim3(256), 0, s, positions, normals, materials,
+                       indices, order, n, tris, leaf_normals, leaf_materials, tri_ids);
     VH_HIP(hipGetLastError());
     return VH_OK;
 }

@@ -61,6 +61,8 @@ struct BlasRecord {
     const float* normals;     // 3 per vertex or null
     const uint32_t* indices;  // 3 per triangle
     const uint32_t* materials;  // per triangle or null
+    const float* leaf_normals;      // 9 floats (the vertex normals) per leaf-order triangle, or null
+    const uint32_t* leaf_materials; // material per leaf-order triangle, or null
     float lo[3], hi[3];       // local bounds
     uint32_t triangle_count, node_count;
 };

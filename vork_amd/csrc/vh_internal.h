@@ -113,7 +113,7 @@ class DeviceRef {
 
 struct vh_blas {
     vh::DeviceRef device;  // first: dropped after the buffers below are freed
-    vh::DeviceBuffer nodes, tris, tri_ids, positions, normals, indices, materials;
+    vh::DeviceBuffer nodes, tris, tri_ids, positions, normals, indices, materials, leaf_normals, leaf_materials;
     vh::BlasRecord record{};
     vh_blas_info info{};
 };
@@ -181,8 +181,11 @@ int build_wide_bvh(vh_device* dev, hipStream_t stream, BuildScratch& scratch, co
 // Kernels of the BLAS build that depend on the mesh layout.
 int launch_triangle_bounds(hipStream_t s, BuildScratch& scratch, const float* positions, uint32_t stride_floats, const uint32_t* indices,
                            uint32_t n, float* lo, float* hi, float* compact_positions, uint32_t vertex_count);
-int launch_gather_triangles(hipStream_t s, const float* positions, const uint32_t* indices, const uint32_t* order,
-                            uint32_t n, float* tris, uint32_t* tri_ids);
+// Leaf-order copies of the triangles: vertex positions, vertex normals and
+// materials (when the mesh has them) and the original primitive index.
+int launch_gather_triangles(hipStream_t s, const float* positions, const float* normals, const uint32_t* materials,
+                            const uint32_t* indices, const uint32_t* order, uint32_t n, float* tris,
+                            float* leaf_normals, uint32_t* leaf_materials, uint32_t* tri_ids);
 int launch_instance_bounds(hipStream_t s, const InstanceRecord* inst, const BlasRecord* blases, uint32_t n, float* lo,
                            float* hi);
 int launch_compact3(hipStream_t s, const float* src, uint32_t stride_floats, uint32_t count, float* dst);

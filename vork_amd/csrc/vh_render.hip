@@ -42,7 +42,7 @@ struct Paths {
     float4* thr;      // throughput
     float4* rad;      // radiance
     uint4* info;      // pixel, sample index, rng, flags
-    float4* hit;      // t, u, v, primitive bits
+    float4* hit;      // t, u, v, leaf-order triangle bits
     uint32_t* hit_inst;
     float4* pend;     // per pending next event q: throughput x estimator 0 (rgb), clamp limit
     float4* pend1;    //   throughput x estimator 1 (rgb), path index bits
@@ -288,7 +288,7 @@ struct ExtendSource {
         key = id & kPathMask;
     }
     __device__ void store(uint32_t p, const TraceHit& h, const TraceCounts& c) const {
-        P.hit[p] = make_float4(h.found ? h.t : -1.f, h.u, h.v, u2f(h.prim));
+        P.hit[p] = make_float4(h.found ? h.t : -1.f, h.u, h.v, u2f(h.item));
         P.hit_inst[p] = h.inst;
         if (kCount) { *nodes += c.nodes; *tris += c.tris; }
     }
@@ -335,19 +335,23 @@ struct Surface {
     f3 world, face, smooth;
     uint32_t material;
 };
-// hit_geometry.slang:120-135 and the double-sided facing of the hit
+// hit_geometry.slang:120-135 and the double-sided facing of the hit.  The
+// vertex positions, normals and material come from the BLAS's leaf-order
+// copies of the hit triangle (`item`), one contiguous record each, not
+// through the index triple.
 __device__ __forceinline__ Surface reconstruct(const Scene& S, uint32_t material_count, const InstanceRecord& r,
-                                               const BlasRecord& b, uint32_t prim, float u, float v, f3 direction) {
+                                               const BlasRecord& b, uint32_t item, float u, float v, f3 direction) {
     Surface sf;
-    const uint32_t i0 = b.indices[3 * prim], i1 = b.indices[3 * prim + 1], i2 = b.indices[3 * prim + 2];
-    f3 p0 = load3(b.positions + 3 * i0), p1 = load3(b.positions + 3 * i1), p2 = load3(b.positions + 3 * i2);
+    const float* tp = b.tris + 9 * (size_t)item;
+    f3 p0 = load3(tp), p1 = load3(tp + 3), p2 = load3(tp + 6);
     float b1 = u, b2 = v, b0 = 1.f - b1 - b2;
     f3 p = p0 * b0 + p1 * b1 + p2 * b2;
     sf.world = xform_point(r.o2w, p);
     f3 face_o = cross(p1 - p0, p2 - p0);
-    f3 n0 = b.normals ? load3(b.normals + 3 * i0) : face_o;
-    f3 n1 = b.normals ? load3(b.normals + 3 * i1) : face_o;
-    f3 n2 = b.normals ? load3(b.normals + 3 * i2) : face_o;
+    const float* np = b.leaf_normals + 9 * (size_t)item;
+    f3 n0 = b.leaf_normals ? load3(np) : face_o;
+    f3 n1 = b.leaf_normals ? load3(np + 3) : face_o;
+    f3 n2 = b.leaf_normals ? load3(np + 6) : face_o;
     face_o = finite_unit_normal(face_o, n0);
     // normal matrix rows = columns of world-to-object
     const float* w = r.w2o;
@@ -366,7 +370,7 @@ __device__ __forceinline__ Surface reconstruct(const Scene& S, uint32_t material
     }
     sf.face = face;
     sf.smooth = ns;
-    uint32_t local = b.materials ? b.materials[prim] : 0u;
+    uint32_t local = b.leaf_materials ? b.leaf_materials[item] : 0u;
     uint32_t m = r.material + local;
     sf.material = m < material_count ? m : material_count - 1;
     return sf;
@@ -421,10 +425,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VH_SHADE
         uint32_t diffuse = (flags >> kDiffuseShift) & 0xfu, glossy = (flags >> kGlossyShift) & 0xfu;
         f3 throughput = mk(t4.x, t4.y, t4.z), direction = mk(rd4.x, rd4.y, rd4.z);
         const float limit = bounce == 0 ? c.clamp_direct : c.clamp_indirect;
-        const uint32_t inst = P.hit_inst[p], prim = f2u(h.w);
+        const uint32_t inst = P.hit_inst[p], item = f2u(h.w);
         const InstanceRecord& ir = S.tv.instances[inst];
         const BlasRecord& br = S.tv.blases[ir.blas];
-        Surface sf = reconstruct(S, c.material_count, ir, br, prim, h.y, h.z, direction);
+        Surface sf = reconstruct(S, c.material_count, ir, br, item, h.y, h.z, direction);
         const vh_material& mat = S.materials[sf.material];
         Closure cl;
         cl.base_color = load3(mat.base_color);

@@ -15,6 +15,7 @@ struct ArraySource {
     uint32_t n, mask;
     void* out;
     const InstanceRecord* instances;
+    const BlasRecord* blases;
     __device__ uint32_t count() const { return n; }
     __device__ void fetch(uint32_t i, RayData& r) const {
         const float4* p = reinterpret_cast<const float4*>(rays + i);
@@ -35,7 +36,9 @@ struct ArraySource {
         if (kMode == 0) {
             vh_hit hit;
             if (h.found) {
-                hit.t = h.t; hit.instance = h.inst; hit.primitive = h.prim;
+                // the hit's leaf-order triangle back to the mesh's primitive index
+                hit.t = h.t; hit.instance = h.inst;
+                hit.primitive = blases[instances[h.inst].blas].tri_ids[h.item];
                 hit.custom_index = instances[h.inst].custom_index & 0xffffffu;
                 hit.u = h.u; hit.v = h.v;
             } else {
@@ -91,13 +94,13 @@ int launch_trace(vh_device* dev, hipStream_t s, const TlasView& tv, const void* 
     uint32_t* of = st->overflow.as<uint32_t>();
     if (mode == 0)
         hipLaunchKernelGGL(k_trace<0>, dim3(grid), dim3(kTraceBlock), 0, s, tv,
-                           ArraySource<0>{r, count, mask, out, tv.instances}, of, errors);
+                           ArraySource<0>{r, count, mask, out, tv.instances, tv.blases}, of, errors);
     else if (mode == 1)
         hipLaunchKernelGGL(k_trace<1>, dim3(grid), dim3(kTraceBlock), 0, s, tv,
-                           ArraySource<1>{r, count, mask, out, tv.instances}, of, errors);
+                           ArraySource<1>{r, count, mask, out, tv.instances, tv.blases}, of, errors);
     else
         hipLaunchKernelGGL(k_trace<2>, dim3(grid), dim3(kTraceBlock), 0, s, tv,
-                           ArraySource<2>{r, count, mask, out, tv.instances}, of, errors);
+                           ArraySource<2>{r, count, mask, out, tv.instances, tv.blases}, of, errors);
     VH_HIP(hipGetLastError());
     uint32_t h = 0;
     VH_HIP(hipMemcpyAsync(&h, errors, sizeof(uint32_t), hipMemcpyDeviceToHost, s));

@@ -107,7 +107,7 @@ __device__ __forceinline__ Stack make_stack(uint64_t* lds, uint64_t* overflow) {
 
 struct TraceHit {
     float t, u, v;
-    uint32_t inst, prim;
+    uint32_t inst, item;  // instance and leaf-order triangle (the BLAS's tri_ids map it to the primitive)
     bool found;
 };
 
@@ -237,7 +237,7 @@ struct Tracer {
         cur_inst = 0; cull = false; in_blas = false; blas_sp = 0;
         bnodes = nullptr; btris = nullptr; bids = nullptr;
         set_root();
-        hit.found = false; hit.t = tmax_; hit.u = hit.v = 0.f; hit.inst = hit.prim = 0xffffffffu;
+        hit.found = false; hit.t = tmax_; hit.u = hit.v = 0.f; hit.inst = hit.item = 0xffffffffu;
         cnt.nodes = cnt.tris = 0;
         st.sp = 0;
         done = tv.node_count == 0;
@@ -293,9 +293,14 @@ struct Tracer {
                 done = true;
                 return;
             }
-            const uint32_t prim = as_global(bids)[item];
-            if (!hit.found || better_hit(t, cur_inst, prim, hit.t, hit.inst, hit.prim)) {
-                hit.found = true; hit.t = t; hit.u = u; hit.v = v; hit.inst = cur_inst; hit.prim = prim;
+            // (t, instance, primitive) order (better_hit); t never exceeds the held hit here
+            // (lim), and an instance is entered once per ray, so an equal t in the same
+            // instance compares two primitives of the current BLAS
+            bool take = !hit.found || t < hit.t;
+            if (!take && t == hit.t)
+                take = cur_inst != hit.inst ? cur_inst < hit.inst : as_global(bids)[item] < as_global(bids)[hit.item];
+            if (take) {
+                hit.found = true; hit.t = t; hit.u = u; hit.v = v; hit.inst = cur_inst; hit.item = item;
             }
         }
     }
