@@ -728,14 +728,18 @@ __global__ void k_tri_bounds(const float* pos, const uint32_t* idx, uint32_t n, 
 __global__ void k_gather_triangles(const float* pos, const float* nrm, const uint32_t* mat, const uint32_t* idx,
                                    const uint32_t* order, uint32_t n, float* tris, float* leaf_nrm, uint32_t* leaf_mat,
                                    uint32_t* tri_ids) {
+    // 12-byte records move as single dwordx3 accesses
+    struct alignas(4) V3 { float x, y, z; };
+    struct alignas(4) I3 { uint32_t v[3]; };
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint32_t prim = order[i];
-    for (int v = 0; v < 3; v++) {
-        uint32_t vi = idx[3 * prim + v];
-        for (int k = 0; k < 3; k++) tris[9 * (size_t)i + 3 * v + k] = pos[3 * vi + k];
-        if (nrm)
-            for (int k = 0; k < 3; k++) leaf_nrm[9 * (size_t)i + 3 * v + k] = nrm[3 * vi + k];
+    const I3 t = *reinterpret_cast<const I3*>(idx + 3 * (size_t)prim);
+    V3* out = reinterpret_cast<V3*>(tris + 9 * (size_t)i);
+    for (int v = 0; v < 3; v++) out[v] = *reinterpret_cast<const V3*>(pos + 3 * (size_t)t.v[v]);
+    if (nrm) {
+        V3* on = reinterpret_cast<V3*>(leaf_nrm + 9 * (size_t)i);
+        for (int v = 0; v < 3; v++) on[v] = *reinterpret_cast<const V3*>(nrm + 3 * (size_t)t.v[v]);
     }
     if (mat) leaf_mat[i] = mat[prim];
     tri_ids[i] = prim;
