@@ -210,7 +210,7 @@ def main():
         per_launch = algorithmic / launches
         avg_ms = prof["extend_ms"] / launches
         achieved = per_launch / (avg_ms * 1e-3) / 1e9
-        roofline = {"kernel": "k_extend (closest-hit traversal of the extension rays)", "bound": "hbm",
+        roofline = {"kernel": "extend stage: k_extend_packet (camera rays) + k_extend (later bounces), closest-hit traversal", "bound": "hbm",
                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
                     "bytes_per_launch": int(per_launch), "avg_launch_ms": round(avg_ms, 4), "launches": launches,

@@ -309,6 +309,45 @@ VH_TRACE_KERNEL void k_extend(Paths P, Scene S, const float4* rays, uint32_t* co
     flush_counts<kCount>(nodes, tris, (unsigned long long*)&counters[16]);
 }
 
+// Camera rays whose waves hold the jittered samples of one pixel: a wave
+// traces its 64 queue entries as one packet (trace_packet), with the hits
+// the per-lane traversal gives.
+#ifndef VH_PACKET_CAMERA
+#define VH_PACKET_CAMERA 1
+#endif
+VH_TRACE_KERNEL void k_extend_packet(Paths P, Scene S, const float4* rays, uint32_t* counters) {
+    __shared__ uint64_t stacks[kBlock / 64][kPacketStack];
+    const uint32_t wave = threadIdx.x >> 6, lane = __lane_id();
+    const uint32_t waves = gridDim.x * (kBlock / 64);
+    const uint32_t n = counters[0];
+    bool ok = true;
+    for (uint32_t chunk = blockIdx.x * (kBlock / 64) + wave; (size_t)chunk * 64 < n; chunk += waves) {
+        const uint32_t i = chunk * 64 + lane;
+        PacketRay r;
+        r.live = i < n;
+        const float4 a = r.live ? rays[2 * (size_t)i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float4 b = r.live ? rays[2 * (size_t)i + 1] : make_float4(0.f, 0.f, 1.f, 0.f);
+        const uint32_t id = f2u(b.w);
+        r.wo = r.o = mk(a.x, a.y, a.z);
+        r.wd = r.ld = mk(b.x, b.y, b.z);
+        r.inv = box_inv(r.ld);
+        r.oct = octant(r.inv);
+        r.lsh = make_shear(r.ld);
+        r.tmin = 0.f;
+        r.tmax = a.w;
+        r.mask = code_mask(id >> kPathBits);
+        r.hit.found = false; r.hit.t = a.w; r.hit.u = r.hit.v = 0.f; r.hit.inst = r.hit.item = 0xffffffffu;
+        if (!trace_packet(S.tv, r, stacks[wave])) ok = false;
+        if (r.live) {
+            const uint32_t p = id & kPathMask;
+            P.hit[p] = make_float4(r.hit.found ? r.hit.t : -1.f, r.hit.u, r.hit.v, u2f(r.hit.item));
+            P.hit_inst[p] = r.hit.inst;
+        }
+    }
+    if (!ok && lane == 0) atomicOr(&counters[8], kStackOverflow);
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd((unsigned long long*)&counters[4], (unsigned long long)n);
+}
+
 // material.slang:69-79 with the instance's local bounds
 __device__ __forceinline__ f3 surface_position_error(const InstanceRecord& r, const BlasRecord& b, f3 world) {
     const float* model = r.o2w;
@@ -1011,6 +1050,8 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, float* g
         for (uint32_t bounce = 0; bounce <= c.max_bounces; bounce++) {
             int rc2 = timed(0, [&] {
                 if (count) hipLaunchKernelGGL(k_extend<true>, dim3(grid), dim3(kBlock), 0, s, P, S, qa, counters, overflow);
+                else if (VH_PACKET_CAMERA && bounce == 0 && c.group >= 16)
+                    hipLaunchKernelGGL(k_extend_packet, dim3(grid), dim3(kBlock), 0, s, P, S, qa, counters);
                 else hipLaunchKernelGGL(k_extend<false>, dim3(grid), dim3(kBlock), 0, s, P, S, qa, counters, overflow);
             });
             if (!rc2) rc2 = timed(1, [&] {

@@ -448,4 +448,200 @@ __device__ bool trace_persistent(const TlasView& tv, const Src& src, uint32_t* n
     return ok;
 }
 
+
+// ---------------------------------------------------------------- packets
+// Packet traversal (closest hit) for waves whose rays are coherent, such as
+// the jittered camera samples of one pixel: the wave walks one stack, visits a
+// node once for all its lanes, and each lane tests its own ray against it.
+// A child is taken when any lane's box test hits it (each lane culls with its
+// own closest hit), and every lane tests every triangle of the union of hit
+// leaves.  The box test is conservative, so a triangle outside a lane's boxes
+// cannot hit that lane's ray, and the closest hit by (t, instance, primitive)
+// does not depend on the visiting order: each lane ends with the hit of the
+// per-lane traversal.  The wave's stack lives in LDS and holds as many
+// entries as a lane's stack in trace_persistent (kLdsStack + kGlobalStack), so
+// both overflow at the same tree depth.
+constexpr int kPacketStack = kLdsStack + kGlobalStack;
+
+struct PacketRay {
+    f3 wo, wd;       // world ray
+    f3 o, inv, ld;   // ray of the current space
+    Shear lsh;
+    uint32_t oct, mask;
+    float tmin, tmax;
+    bool live, in_inst;  // holds a ray; takes part in the current instance
+    TraceHit hit;
+};
+
+// Box tests of a node's children for every lane; the union over the wave of
+// the hit internal children (bits at slot ^ order_oct) and of the hit leaf
+// items.
+__device__ __forceinline__ void packet_children(const NodeBits& nb, const PacketRay& r, bool take_part,
+                                                uint32_t order_oct, uint32_t& hits, uint32_t& items) {
+    const WideNode& n = nb.n;
+    const f3 o = r.o, inv = r.inv;
+    const float tmax = r.hit.found ? r.hit.t * kCullScale : r.tmax;
+    const float sx = exp_scale(n.ex), sy = exp_scale(n.ey), sz = exp_scale(n.ez);
+    const float ax = sx * inv.x, ay = sy * inv.y, az = sz * inv.z;
+    const float bx = (n.origin[0] - o.x) * inv.x, by = (n.origin[1] - o.y) * inv.y, bz = (n.origin[2] - o.z) * inv.z;
+    const float ex = ((255.f * sx + fabsf(n.origin[0])) + fabsf(o.x)) * fabsf(inv.x) * kBoxMargin;
+    const float ey = ((255.f * sy + fabsf(n.origin[1])) + fabsf(o.y)) * fabsf(inv.y) * kBoxMargin;
+    const float ez = ((255.f * sz + fabsf(n.origin[2])) + fabsf(o.z)) * fabsf(inv.z) * kBoxMargin;
+    const float bnx = bx - ex, bfx = bx + ex, bny = by - ey, bfy = by + ey, bnz = bz - ez, bfz = bz + ez;
+    const uint32_t oct = r.oct;
+    const bool px = !(oct & 1u), py = !(oct & 2u), pz = !(oct & 4u);
+    const uint32_t nx0 = px ? nb.v[2].x : nb.v[3].z, nx1 = px ? nb.v[2].y : nb.v[3].w;
+    const uint32_t fx0 = px ? nb.v[3].z : nb.v[2].x, fx1 = px ? nb.v[3].w : nb.v[2].y;
+    const uint32_t ny0 = py ? nb.v[2].z : nb.v[4].x, ny1 = py ? nb.v[2].w : nb.v[4].y;
+    const uint32_t fy0 = py ? nb.v[4].x : nb.v[2].z, fy1 = py ? nb.v[4].y : nb.v[2].w;
+    const uint32_t nz0 = pz ? nb.v[3].x : nb.v[4].z, nz1 = pz ? nb.v[3].y : nb.v[4].w;
+    const uint32_t fz0 = pz ? nb.v[4].z : nb.v[3].x, fz1 = pz ? nb.v[4].w : nb.v[3].y;
+    const uint32_t imask = n.imask;
+    uint32_t h = 0, it = 0;
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        const uint32_t sh = (c & 3) * 8;
+        const uint32_t nxw = c < 4 ? nx0 : nx1, fxw = c < 4 ? fx0 : fx1;
+        const uint32_t nyw = c < 4 ? ny0 : ny1, fyw = c < 4 ? fy0 : fy1;
+        const uint32_t nzw = c < 4 ? nz0 : nz1, fzw = c < 4 ? fz0 : fz1;
+        const float tnx = __builtin_fmaf((float)((nxw >> sh) & 0xffu), ax, bnx);
+        const float tfx = __builtin_fmaf((float)((fxw >> sh) & 0xffu), ax, bfx);
+        const float tny = __builtin_fmaf((float)((nyw >> sh) & 0xffu), ay, bny);
+        const float tfy = __builtin_fmaf((float)((fyw >> sh) & 0xffu), ay, bfy);
+        const float tnz = __builtin_fmaf((float)((nzw >> sh) & 0xffu), az, bnz);
+        const float tfz = __builtin_fmaf((float)((fzw >> sh) & 0xffu), az, bfz);
+        const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, r.tmin));
+        const float tf = fminf(fminf(tfx, tfy), fminf(tfz, tmax));
+        if (__ballot(take_part && tn <= tf)) {
+            const uint32_t meta = ((c < 4 ? nb.v[1].z : nb.v[1].w) >> sh) & 0xffu;
+            if ((imask >> c) & 1u) {
+                h |= 1u << ((uint32_t)c ^ order_oct);
+            } else if (!(meta & kMetaNoItems)) {
+                it |= ((1u << ((meta >> 5) + 1u)) - 1u) << (meta & 31u);
+            }
+        }
+    }
+    hits = h;
+    items = it;
+}
+
+// Traces the wave's rays (one per lane, `live` marks the lanes holding one).
+// Returns false when the wave's stack overflowed (the rays' hits are then
+// incomplete).
+__device__ inline bool trace_packet(const TlasView& tv, PacketRay& r, uint64_t* stack) {
+    const uint32_t lane = __lane_id();
+    bool in_blas = false, cull = false;
+    uint32_t cur_inst = 0;
+    const WideNode* bnodes = nullptr;
+    const float* btris = nullptr;
+    const uint32_t* bids = nullptr;
+    int sp = 0, blas_sp = 0;
+    uint32_t nbase = 0, ibase = 0, items = 0;
+    uint32_t order = (uint32_t)__builtin_amdgcn_readfirstlane((int)r.oct);
+    uint32_t ngroup = tv.node_count ? (1u << order) | (1u << 8) : 0u;  // the root as slot 0
+    r.in_inst = r.live;
+    while (true) {
+        if (items) {
+            const uint32_t item = ibase + (uint32_t)__builtin_ctz(items);
+            items &= items - 1u;
+            if (in_blas) {
+                const VH_GLOBAL float* tp = as_global(btris) + 9 * (size_t)item;
+                const f3 a = mk(tp[0], tp[1], tp[2]), b = mk(tp[3], tp[4], tp[5]), c = mk(tp[6], tp[7], tp[8]);
+                if (r.in_inst) {
+                    const float lim = r.hit.found ? next_up(r.hit.t) : r.tmax;
+                    float t, u, v;
+                    if (intersect_triangle(a, b, c, r.o, r.ld, r.lsh, r.tmin, lim, cull, &t, &u, &v) && t < r.tmax) {
+                        bool take = !r.hit.found || t < r.hit.t;
+                        if (!take && t == r.hit.t)
+                            take = cur_inst != r.hit.inst ? cur_inst < r.hit.inst
+                                                          : as_global(bids)[item] < as_global(bids)[r.hit.item];
+                        if (take) {
+                            r.hit.found = true; r.hit.t = t; r.hit.u = u; r.hit.v = v;
+                            r.hit.inst = cur_inst; r.hit.item = item;
+                        }
+                    }
+                }
+                continue;
+            }
+            // an instance of the TLAS: enter it if some lane's mask takes it
+            const uint32_t inst = as_global(tv.inst_ids)[item];
+            const VH_GLOBAL InstanceRecord& rec = as_global(tv.instances)[inst];
+            const uint32_t mf = rec.mask_flags;
+            const bool enter = r.live && (mf & (1u << 9)) && (mf & r.mask & 0xffu);
+            if (!__ballot(enter)) continue;
+            if (sp + 2 > kPacketStack) return false;
+            if (ngroup & 0xffu) {
+                if (lane == 0) stack[sp] = (uint64_t)nbase | ((uint64_t)ngroup << 32);
+                sp++;
+            }
+            if (items) {
+                if (lane == 0) stack[sp] = (uint64_t)(ibase | kItemGroup) | ((uint64_t)items << 32);
+                sp++;
+            }
+            blas_sp = sp;
+            float w2o[12];
+#pragma unroll
+            for (int k = 0; k < 12; k++) w2o[k] = rec.w2o[k];
+            r.o = xform_point(w2o, r.wo);
+            r.ld = xform_dir(w2o, r.wd);
+            r.inv = box_inv(r.ld);
+            r.oct = octant(r.inv);
+            r.lsh = make_shear(r.ld);
+            r.in_inst = enter;
+            cull = (mf >> 8) & 1u;
+            const VH_GLOBAL BlasRecord& br = as_global(tv.blases)[rec.blas];
+            bnodes = br.nodes;
+            btris = br.tris;
+            bids = br.tri_ids;
+            cur_inst = inst;
+            in_blas = true;
+            order = (uint32_t)__builtin_amdgcn_readfirstlane((int)r.oct);
+            nbase = 0;
+            ngroup = (1u << order) | (1u << 8);
+            items = 0;
+            continue;
+        }
+        if (ngroup & 0xffu) {
+            const uint32_t hits = ngroup & 0xffu, imask = ngroup >> 8;
+            const uint32_t slot = (uint32_t)__builtin_ctz(hits) ^ order;
+            const uint32_t rest = hits & (hits - 1u);
+            const uint32_t node = nbase + (uint32_t)__popc(imask & ((1u << slot) - 1u));
+            if (rest) {
+                if (sp >= kPacketStack) return false;
+                if (lane == 0) stack[sp] = (uint64_t)nbase | ((uint64_t)(rest | (imask << 8)) << 32);
+                sp++;
+            }
+            NodeBits nb;
+            load_node(in_blas ? bnodes : tv.nodes, node, nb);
+            uint32_t h, it;
+            packet_children(nb, r, in_blas ? r.in_inst : r.live, order, h, it);
+            nbase = nb.n.child_base;
+            ngroup = h | ((uint32_t)nb.n.imask << 8);
+            ibase = nb.n.item_base;
+            items = it;
+            continue;
+        }
+        // the group is used up: leave the instance, take the next group, or stop
+        if (in_blas && sp == blas_sp) {
+            r.o = r.wo;
+            r.inv = box_inv(r.wd);
+            r.oct = octant(r.inv);
+            in_blas = false;
+            order = (uint32_t)__builtin_amdgcn_readfirstlane((int)r.oct);
+        }
+        if (sp == 0) break;
+        sp--;
+        const uint64_t e = stack[sp];
+        const uint32_t e0 = (uint32_t)e, e1 = (uint32_t)(e >> 32);
+        if (e0 & kItemGroup) {
+            ibase = e0 & ~kItemGroup;
+            items = e1;
+        } else {
+            nbase = e0;
+            ngroup = e1;
+        }
+    }
+    return true;
+}
+
 }  // namespace vh
