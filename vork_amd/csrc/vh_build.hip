@@ -342,6 +342,98 @@ __global__ void k_ploc_merge(const uint32_t* clusters, const uint32_t* nn, uint3
     }
 }
 
+// The last rounds, once at most kTailMax clusters are left, in one block of
+// kTailMax threads: the clusters' ids and boxes stay in LDS between rounds,
+// and each round is the rounds above (k_ploc_nn, k_ploc_flags, the scan,
+// k_ploc_merge, k_ploc_advance) with the same neighbours and node ids, with
+// barriers for launches.  Merged nodes' boxes, sizes and tables go to global
+// memory as k_ploc_merge writes them; later rounds of the block read them
+// back after a barrier (one block shares its CU's caches).
+constexpr uint32_t kTailMax = 1024;
+__global__ __launch_bounds__(kTailMax) void k_ploc_tail(const uint32_t* clusters, uint32_t* state, uint32_t n,
+                                                       float* nb, uint32_t* left, uint32_t* right, uint32_t* size,
+                                                       float* dp_cost, uint8_t* dp_dec, uint32_t p_max,
+                                                       uint32_t n_nodes) {
+    __shared__ float box[6][kTailMax];
+    __shared__ uint32_t cl[kTailMax], nnb[kTailMax];
+    __shared__ uint64_t sh[kTailMax];
+    const uint32_t tid = threadIdx.x;
+    uint32_t count = state[0], node_base = state[1];
+    if (count <= 1 || count > kTailMax) return;
+    if (tid < count) {
+        const uint32_t c = clusters[tid];
+        cl[tid] = c;
+        for (int q = 0; q < 6; q++) box[q][tid] = nb[6 * (size_t)c + q];
+    }
+    __syncthreads();
+    bool failed = false;
+    while (count > 1) {
+        if (tid < count) {
+            const float lo0 = box[0][tid], lo1 = box[1][tid], lo2 = box[2][tid];
+            const float hi0 = box[3][tid], hi1 = box[4][tid], hi2 = box[5][tid];
+            float best = INFINITY;
+            uint32_t bj = tid;
+            for (int d = -kPlocRadius; d <= kPlocRadius; d++) {
+                const int64_t j = (int64_t)tid + d;
+                if (d == 0 || j < 0 || j >= (int64_t)count) continue;
+                float lo[3] = {fminf(lo0, box[0][j]), fminf(lo1, box[1][j]), fminf(lo2, box[2][j])};
+                float hi[3] = {fmaxf(hi0, box[3][j]), fmaxf(hi1, box[4][j]), fmaxf(hi2, box[5][j])};
+                const float a = half_area(lo, hi);
+                if (a < best) { best = a; bj = (uint32_t)j; }
+            }
+            nnb[tid] = bj;
+        }
+        __syncthreads();
+        uint64_t f = 0;
+        if (tid < count) {
+            const uint32_t j = nnb[tid];
+            const bool mutual = j != tid && nnb[j] == tid;
+            f = (uint64_t)((!mutual || tid < j) ? 1u : 0u) | ((uint64_t)((mutual && tid < j) ? 1u : 0u) << 32);
+        }
+        const uint64_t o = block_inclusive_scan(f, sh) - f;
+        const uint64_t total = sh[kTailMax - 1];
+        uint32_t id = 0;
+        float lo[3] = {0.f, 0.f, 0.f}, hi[3] = {0.f, 0.f, 0.f};
+        if (f & 1u) {
+            id = cl[tid];
+            for (int k = 0; k < 3; k++) { lo[k] = box[k][tid]; hi[k] = box[3 + k][tid]; }
+            if (f >> 32) {
+                const uint32_t bp = nnb[tid], a = id, b = cl[bp];
+                id = node_base + (uint32_t)(o >> 32);
+                for (int k = 0; k < 3; k++) {
+                    lo[k] = fminf(lo[k], box[k][bp]);
+                    hi[k] = fmaxf(hi[k], box[3 + k][bp]);
+                    nb[6 * (size_t)id + k] = lo[k];
+                    nb[6 * (size_t)id + 3 + k] = hi[k];
+                }
+                left[id - n] = a;
+                right[id - n] = b;
+                const uint32_t prims = size[a] + size[b];
+                size[id] = prims;
+                dp_table(nb, dp_cost, n, a, b, id, half_area(lo, hi), prims, p_max, dp_cost, dp_dec);
+            }
+        }
+        __syncthreads();  // every read of this round's LDS is done
+        if (f & 1u) {
+            cl[(uint32_t)o] = id;
+            for (int k = 0; k < 3; k++) { box[k][(uint32_t)o] = lo[k]; box[3 + k][(uint32_t)o] = hi[k]; }
+        }
+        const uint32_t survivors = (uint32_t)total, merged = (uint32_t)(total >> 32);
+        if (survivors >= count || merged == 0 || node_base + merged > n_nodes) {
+            failed = true;
+            break;
+        }
+        count = survivors;
+        node_base += merged;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        state[0] = failed ? 1u : count;
+        state[1] = node_base;
+        if (failed) state[4] = 1u;
+    }
+}
+
 // Next round: survivors become the clusters, merged nodes advance the node
 // base; a round that merged nothing sets the error flag and stops the rounds.
 __global__ void k_ploc_advance(uint32_t* state, uint32_t n_nodes) {
@@ -616,7 +708,8 @@ int build_wide_bvh(vh_device* dev, hipStream_t s, BuildScratch& B, const float* 
     // Clustering rounds run from device-resident state ([0] clusters, [1]
     // next node id, [2..3] the round's totals, [4] no-progress flag); the host
     // reads it back every kRounds rounds and sizes the grids by the last count
-    // it saw (later rounds have fewer clusters and exit early).
+    // it saw (later rounds have fewer clusters and exit early); once it sees at
+    // most kTailMax clusters, one k_ploc_tail launch runs the remaining rounds.
     constexpr int kRounds = 6;
     VH_HIP(tile_sums.reserve(sizeof(uint64_t) * ((n + kScanTile - 1) / kScanTile + 1)));
     uint32_t* state = counters.as<uint32_t>();
@@ -625,7 +718,11 @@ int build_wide_bvh(vh_device* dev, hipStream_t s, BuildScratch& B, const float* 
     uint32_t bound = n;
     uint32_t *cl_in = ca.as<uint32_t>(), *cl_out = cb.as<uint32_t>();
     while (bound > 1) {
-        for (int r = 0; r < kRounds; r++) {
+        if (bound <= kTailMax) {
+            hipLaunchKernelGGL(k_ploc_tail, dim3(1), dim3(kTailMax), 0, s, cl_in, state, n, nb.as<float>(),
+                               left.as<uint32_t>(), right.as<uint32_t>(), size.as<uint32_t>(), B.dp_cost.as<float>(),
+                               B.dp_dec.as<uint8_t>(), leaf_max, (uint32_t)n_nodes);
+        } else for (int r = 0; r < kRounds; r++) {
             hipLaunchKernelGGL(k_ploc_nn, dim3(blocks(bound, kPlocBlock)), dim3(kPlocBlock), 0, s, cl_in, state,
                                nb.as<float>(), nn.as<uint32_t>());
             hipLaunchKernelGGL(k_ploc_flags, dim3(blocks(bound, 256)), dim3(256), 0, s, nn.as<uint32_t>(), state,
