@@ -181,14 +181,11 @@ int vh_blas_build(vh_device* dev, const vh_geometry* g, uint32_t flags, void* st
     if ((rc = build_wide_bvh(dev, s, B, lo, hi, n, kLeafMax, &wb))) return bail(rc);
     VH_HIP(b->tris.allocate(sizeof(float) * 9 * (size_t)n));
     VH_HIP(b->tri_ids.allocate(sizeof(uint32_t) * (size_t)n));
-    if (g->normals) VH_HIP(b->leaf_normals.allocate(sizeof(float) * 9 * (size_t)n));
-    if (g->triangle_materials) VH_HIP(b->leaf_materials.allocate(sizeof(uint32_t) * (size_t)n));
+    VH_HIP(b->surf.allocate(sizeof(float) * kSurfFloats * (size_t)n));
     if ((rc = launch_gather_triangles(s, b->positions.as<float>(), g->normals ? b->normals.as<float>() : nullptr,
                                       g->triangle_materials ? b->materials.as<uint32_t>() : nullptr,
                                       b->indices.as<uint32_t>(), wb.order, n, b->tris.as<float>(),
-                                      g->normals ? b->leaf_normals.as<float>() : nullptr,
-                                      g->triangle_materials ? b->leaf_materials.as<uint32_t>() : nullptr,
-                                      b->tri_ids.as<uint32_t>())))
+                                      b->surf.as<float>(), b->tri_ids.as<uint32_t>())))
         return bail(rc);
     // the node array leaves the scratch at its exact size
     VH_HIP(b->nodes.allocate(sizeof(WideNode) * (size_t)wb.node_count));
@@ -209,8 +206,8 @@ int vh_blas_build(vh_device* dev, const vh_geometry* g, uint32_t flags, void* st
     r.normals = g->normals ? b->normals.as<float>() : nullptr;
     r.indices = b->indices.as<uint32_t>();
     r.materials = g->triangle_materials ? b->materials.as<uint32_t>() : nullptr;
-    r.leaf_normals = g->normals ? b->leaf_normals.as<float>() : nullptr;
-    r.leaf_materials = g->triangle_materials ? b->leaf_materials.as<uint32_t>() : nullptr;
+    r.surf = b->surf.as<float>();
+    r.surf_normals = g->normals ? 1u : 0u;
     for (int k = 0; k < 3; k++) { r.lo[k] = bounds[k]; r.hi[k] = bounds[3 + k]; }
     r.triangle_count = n;
     r.node_count = wb.node_count;
@@ -221,7 +218,7 @@ int vh_blas_build(vh_device* dev, const vh_geometry* g, uint32_t flags, void* st
     b->info.sah_cost = wb.sah_cost;
     b->info.build_ms = ms;
     b->info.device_bytes = b->nodes.bytes + b->tris.bytes + b->tri_ids.bytes + b->positions.bytes + b->normals.bytes +
-                           b->indices.bytes + b->materials.bytes + b->leaf_normals.bytes + b->leaf_materials.bytes;
+                           b->indices.bytes + b->materials.bytes + b->surf.bytes;
     *out = b;
     return VH_OK;
 }

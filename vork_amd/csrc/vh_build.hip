@@ -823,8 +823,7 @@ __global__ void k_tri_bounds(const float* pos, const uint32_t* idx, uint32_t n, 
     }
 }
 __global__ void k_gather_triangles(const float* pos, const float* nrm, const uint32_t* mat, const uint32_t* idx,
-                                   const uint32_t* order, uint32_t n, float* tris, float* leaf_nrm, uint32_t* leaf_mat,
-                                   uint32_t* tri_ids) {
+                                   const uint32_t* order, uint32_t n, float* tris, float* surf, uint32_t* tri_ids) {
     // 12-byte records move as single dwordx3 accesses
     struct alignas(4) V3 { float x, y, z; };
     struct alignas(4) I3 { uint32_t v[3]; };
@@ -833,12 +832,22 @@ __global__ void k_gather_triangles(const float* pos, const float* nrm, const uin
     uint32_t prim = order[i];
     const I3 t = *reinterpret_cast<const I3*>(idx + 3 * (size_t)prim);
     V3* out = reinterpret_cast<V3*>(tris + 9 * (size_t)i);
-    for (int v = 0; v < 3; v++) out[v] = *reinterpret_cast<const V3*>(pos + 3 * (size_t)t.v[v]);
-    if (nrm) {
-        V3* on = reinterpret_cast<V3*>(leaf_nrm + 9 * (size_t)i);
-        for (int v = 0; v < 3; v++) on[v] = *reinterpret_cast<const V3*>(nrm + 3 * (size_t)t.v[v]);
+    float r[kSurfFloats] = {};
+    for (int v = 0; v < 3; v++) {
+        const V3 p = *reinterpret_cast<const V3*>(pos + 3 * (size_t)t.v[v]);
+        out[v] = p;
+        r[3 * v] = p.x; r[3 * v + 1] = p.y; r[3 * v + 2] = p.z;
     }
-    if (mat) leaf_mat[i] = mat[prim];
+    if (nrm) {
+        for (int v = 0; v < 3; v++) {
+            const V3 q = *reinterpret_cast<const V3*>(nrm + 3 * (size_t)t.v[v]);
+            r[9 + 3 * v] = q.x; r[10 + 3 * v] = q.y; r[11 + 3 * v] = q.z;
+        }
+    }
+    if (mat) r[18] = __uint_as_float(mat[prim]);
+    // the whole line, padding included, so that no store is partial
+    float4* os = reinterpret_cast<float4*>(surf + kSurfFloats * (size_t)i);
+    for (int k = 0; k < (int)kSurfFloats / 4; k++) os[k] = make_float4(r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]);
     tri_ids[i] = prim;
 }
 __global__ void k_instance_bounds(const InstanceRecord* inst, const BlasRecord* blases, uint32_t n, float* lo, float* hi) {
@@ -960,9 +969,9 @@ int launch_triangle_bounds(hipStream_t s, BuildScratch& B, const float* position
 
 int launch_gather_triangles(hipStream_t s, const float* positions, const float* normals, const uint32_t* materials,
                             const uint32_t* indices, const uint32_t* order, uint32_t n, float* tris,
-                            float* leaf_normals, uint32_t* leaf_materials, uint32_t* tri_ids) {
+                            float* surf, uint32_t* tri_ids) {
     hipLaunchKernelGGL(k_gather_triangles, dim3(blocks(n, 256)), dim3(256), 0, s, positions, normals, materials,
-                       indices, order, n, tris, leaf_normals, leaf_materials, tri_ids);
+                       indices, order, n, tris, surf, tri_ids);
     VH_HIP(hipGetLastError());
     return VH_OK;
 }

@@ -53,6 +53,8 @@ VH_HD float decode(float origin, uint8_t q, float scale) { return __builtin_fmaf
 // ------------------------------------------------------------ records
 // One BLAS on the device: its node and triangle arrays plus the shading data
 // of the mesh (original indexed vertices, normals and triangle materials).
+constexpr uint32_t kSurfFloats = 32;
+
 struct BlasRecord {
     const WideNode* nodes;
     const float* tris;        // 9 floats per triangle, leaf order
@@ -61,10 +63,13 @@ struct BlasRecord {
     const float* normals;     // 3 per vertex or null
     const uint32_t* indices;  // 3 per triangle
     const uint32_t* materials;  // per triangle or null
-    const float* leaf_normals;      // 9 floats (the vertex normals) per leaf-order triangle, or null
-    const uint32_t* leaf_materials; // material per leaf-order triangle, or null
+    // per leaf-order triangle one 128-byte record (kSurfFloats floats, one cache
+    // line): vertex positions (9 floats), vertex normals (9, zero when the mesh
+    // has none), the material (bits of float 18, zero when none), padding
+    const float* surf;
     float lo[3], hi[3];       // local bounds
     uint32_t triangle_count, node_count;
+    uint32_t surf_normals;    // 1 when the records hold vertex normals
 };
 
 // One TLAS instance.  w2o are the world-to-object rows computed on the host in

@@ -113,7 +113,7 @@ class DeviceRef {
 
 struct vh_blas {
     vh::DeviceRef device;  // first: dropped after the buffers below are freed
-    vh::DeviceBuffer nodes, tris, tri_ids, positions, normals, indices, materials, leaf_normals, leaf_materials;
+    vh::DeviceBuffer nodes, tris, tri_ids, positions, normals, indices, materials, surf;
     vh::BlasRecord record{};
     vh_blas_info info{};
 };
@@ -185,7 +185,7 @@ int launch_triangle_bounds(hipStream_t s, BuildScratch& scratch, const float* po
 // materials (when the mesh has them) and the original primitive index.
 int launch_gather_triangles(hipStream_t s, const float* positions, const float* normals, const uint32_t* materials,
                             const uint32_t* indices, const uint32_t* order, uint32_t n, float* tris,
-                            float* leaf_normals, uint32_t* leaf_materials, uint32_t* tri_ids);
+                            float* surf, uint32_t* tri_ids);
 int launch_instance_bounds(hipStream_t s, const InstanceRecord* inst, const BlasRecord* blases, uint32_t n, float* lo,
                            float* hi);
 int launch_compact3(hipStream_t s, const float* src, uint32_t stride_floats, uint32_t count, float* dst);

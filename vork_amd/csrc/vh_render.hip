@@ -375,22 +375,22 @@ struct Surface {
     uint32_t material;
 };
 // hit_geometry.slang:120-135 and the double-sided facing of the hit.  The
-// vertex positions, normals and material come from the BLAS's leaf-order
-// copies of the hit triangle (`item`), one contiguous record each, not
-// through the index triple.
+// vertex positions, normals and material come from the hit triangle's
+// leaf-order record (`item`): one 128-byte line, not the index triple.
 __device__ __forceinline__ Surface reconstruct(const Scene& S, uint32_t material_count, const InstanceRecord& r,
                                                const BlasRecord& b, uint32_t item, float u, float v, f3 direction) {
     Surface sf;
-    const float* tp = b.tris + 9 * (size_t)item;
-    f3 p0 = load3(tp), p1 = load3(tp + 3), p2 = load3(tp + 6);
+    const float4* rec = reinterpret_cast<const float4*>(b.surf + kSurfFloats * (size_t)item);
+    const float4 q0 = rec[0], q1 = rec[1], q2 = rec[2], q3 = rec[3], q4 = rec[4];
+    f3 p0 = mk(q0.x, q0.y, q0.z), p1 = mk(q0.w, q1.x, q1.y), p2 = mk(q1.z, q1.w, q2.x);
     float b1 = u, b2 = v, b0 = 1.f - b1 - b2;
     f3 p = p0 * b0 + p1 * b1 + p2 * b2;
     sf.world = xform_point(r.o2w, p);
     f3 face_o = cross(p1 - p0, p2 - p0);
-    const float* np = b.leaf_normals + 9 * (size_t)item;
-    f3 n0 = b.leaf_normals ? load3(np) : face_o;
-    f3 n1 = b.leaf_normals ? load3(np + 3) : face_o;
-    f3 n2 = b.leaf_normals ? load3(np + 6) : face_o;
+    const bool vn = b.surf_normals != 0u;
+    f3 n0 = vn ? mk(q2.y, q2.z, q2.w) : face_o;
+    f3 n1 = vn ? mk(q3.x, q3.y, q3.z) : face_o;
+    f3 n2 = vn ? mk(q3.w, q4.x, q4.y) : face_o;
     face_o = finite_unit_normal(face_o, n0);
     // normal matrix rows = columns of world-to-object
     const float* w = r.w2o;
@@ -409,7 +409,7 @@ __device__ __forceinline__ Surface reconstruct(const Scene& S, uint32_t material
     }
     sf.face = face;
     sf.smooth = ns;
-    uint32_t local = b.leaf_materials ? b.leaf_materials[item] : 0u;
+    uint32_t local = f2u(q4.z);
     uint32_t m = r.material + local;
     sf.material = m < material_count ? m : material_count - 1;
     return sf;
