@@ -135,6 +135,38 @@ def test_blas_from_device_pointers_matches_host(renderer):
     _compare(t1.trace_closest(rays), t2.trace_closest(rays))
 
 
+def test_concurrent_builds_on_two_streams_match_serial(renderer):
+    """Two BLASes built at once from two threads on two streams (each stream
+    has its own build scratch) equal the same builds made one after the
+    other on the device's stream: same trees, same hits."""
+    torch = pytest.importorskip("torch")
+    import concurrent.futures
+    meshes = [S.uv_sphere(1.0, 256, 128), S.uv_sphere(0.5, 96, 48)]
+    dev_meshes = [(torch.from_numpy(m.positions).cuda(), torch.from_numpy(m.indices.astype(np.int32)).cuda(),
+                   torch.from_numpy(m.normals).cuda()) for m in meshes]
+    torch.cuda.synchronize()
+    serial = [native.Blas(renderer.device, *d, device_pointers=True) for d in dev_meshes]
+    streams = [torch.cuda.Stream() for _ in meshes]
+    with concurrent.futures.ThreadPoolExecutor(2) as pool:
+        for _ in range(2):  # the second round reuses the streams' scratch
+            both = list(pool.map(lambda k: native.Blas(renderer.device, *dev_meshes[k], device_pointers=True,
+                                                        stream=streams[k].cuda_stream), range(2)))
+            for a, b in zip(serial, both):
+                ia, ib = a.info(), b.info()
+                for key in ("triangle_count", "node_count", "binary_node_count", "max_depth"):
+                    assert ia[key] == ib[key], key
+                # summed with device atomics in any order
+                assert ib["sah_cost"] == pytest.approx(ia["sah_cost"], rel=1e-12)
+            insts = [dict(transform=S.translate(0, 0, 0), blas=0, material=0),
+                     dict(transform=S.translate(1.2, 0.3, 0), blas=1, material=0)]
+            sc = S.Scene(meshes, insts, [dict(base_color=(1, 1, 1))], [],
+                         dict(eye=(0, 0, 5), target=(0, 0, 0)), dict(kind=0, color=(0, 0, 0)))
+            rays = random_rays(sc, 4000, seed=3)
+            t1 = native.Tlas(renderer.device, serial, sc.instance_array())
+            t2 = native.Tlas(renderer.device, both, sc.instance_array())
+            _compare(t1.trace_closest(rays), t2.trace_closest(rays))
+
+
 def test_invalid_inputs_raise(renderer):
     with pytest.raises(native.VorkError) as e:
         native.Blas(renderer.device, np.zeros((3, 3), np.float32), np.array([[0, 1, 5]], np.uint32))

@@ -133,9 +133,13 @@ class Device:
 
 
 class Blas:
-    def __init__(self, device, positions, indices, normals=None, triangle_materials=None, device_pointers=False):
+    def __init__(self, device, positions, indices, normals=None, triangle_materials=None, device_pointers=False,
+                 stream=None):
         """positions (V,3) float32, indices (T,3) uint32; numpy arrays, or torch
-        CUDA tensors when device_pointers is True."""
+        CUDA tensors when device_pointers is True.  `stream` (a HIP stream
+        handle, e.g. torch.cuda.Stream().cuda_stream) orders the build on that
+        stream; builds on different streams run concurrently from several
+        threads.  The call returns with the BLAS complete."""
         self.device = device
         g = abi.Geometry()
         if device_pointers:
@@ -167,7 +171,8 @@ class Blas:
         g.triangle_count = nt
         self.handle = C.c_void_p()
         flags = abi.VH_DEVICE_POINTERS if device_pointers else 0
-        check(lib().vh_blas_build(device.handle, C.byref(g), flags, None, C.byref(self.handle)))
+        check(lib().vh_blas_build(device.handle, C.byref(g), flags, C.c_void_p(stream) if stream else None,
+                                  C.byref(self.handle)))
         self._keep = None
 
     def info(self):
