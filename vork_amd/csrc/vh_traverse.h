@@ -458,10 +458,26 @@ __device__ bool trace_persistent(const TlasView& tv, const Src& src, uint32_t* n
 // leaves.  The box test is conservative, so a triangle outside a lane's boxes
 // cannot hit that lane's ray, and the closest hit by (t, instance, primitive)
 // does not depend on the visiting order: each lane ends with the hit of the
-// per-lane traversal.  The wave's stack lives in LDS and holds as many
+// per-lane traversal.  Nodes and triangles, read at wave-uniform addresses,
+// come in through the scalar unit into SGPRs.  The wave's stack lives in LDS and holds as many
 // entries as a lane's stack in trace_persistent (kLdsStack + kGlobalStack), so
 // both overflow at the same tree depth.
 constexpr int kPacketStack = kLdsStack + kGlobalStack;
+
+// Wave-uniform reads through the scalar unit: the address is made uniform
+// with readfirstlane and read through the constant address space.
+#define VH_CONST __attribute__((address_space(4)))
+__device__ __forceinline__ uint64_t uniform_address(const void* p, uint64_t offset) {
+    const uint64_t a = (uint64_t)p + offset;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)a);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(a >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ void load_node_scalar(const WideNode* base, uint32_t i, NodeBits& nb) {
+    const VH_CONST vu4* p = (const VH_CONST vu4*)uniform_address(base, (uint64_t)i * sizeof(WideNode));
+#pragma unroll
+    for (int k = 0; k < 5; k++) nb.v[k] = p[k];
+}
 
 struct PacketRay {
     f3 wo, wd;       // world ray
@@ -545,7 +561,7 @@ __device__ inline bool trace_packet(const TlasView& tv, PacketRay& r, uint64_t* 
             const uint32_t item = ibase + (uint32_t)__builtin_ctz(items);
             items &= items - 1u;
             if (in_blas) {
-                const VH_GLOBAL float* tp = as_global(btris) + 9 * (size_t)item;
+                const VH_CONST float* tp = (const VH_CONST float*)uniform_address(btris, 36ull * item);
                 const f3 a = mk(tp[0], tp[1], tp[2]), b = mk(tp[3], tp[4], tp[5]), c = mk(tp[6], tp[7], tp[8]);
                 if (r.in_inst) {
                     const float lim = r.hit.found ? next_up(r.hit.t) : r.tmax;
@@ -612,7 +628,7 @@ __device__ inline bool trace_packet(const TlasView& tv, PacketRay& r, uint64_t* 
                 sp++;
             }
             NodeBits nb;
-            load_node(in_blas ? bnodes : tv.nodes, node, nb);
+            load_node_scalar(in_blas ? bnodes : tv.nodes, node, nb);
             uint32_t h, it;
             packet_children(nb, r, in_blas ? r.in_inst : r.live, order, h, it);
             nbase = nb.n.child_base;
