@@ -41,7 +41,7 @@
 extern "C" {
 #endif
 
-#define VH_ABI_VERSION 1u
+#define VH_ABI_VERSION 2u
 
 /* Status codes (crates/vork-render/src/failure.rs, session.rs:101-104). */
 enum {
@@ -185,10 +185,29 @@ int vh_trace_closest(const vh_tlas* tlas, const vh_ray* rays, uint32_t count, ui
 int vh_trace_any(const vh_tlas* tlas, const vh_ray* rays, uint32_t count, uint32_t mask,
                  uint32_t flags, void* stream, uint8_t* occluded);
 
-/* Per-ray traversal work of the closest-hit kernel: [nodes visited, triangles
- * tested] for each ray.  Used to state algorithmic bytes. */
+/* Per-ray traversal work of the closest-hit kernel, four uint32 per ray:
+ * [nodes visited, triangles tested, deepest traversal-stack entry, 0].  Used
+ * to state algorithmic bytes and to show how deep traversals go (entries
+ * beyond the 12 a lane keeps in LDS spill to global memory). */
 int vh_trace_closest_counts(const vh_tlas* tlas, const vh_ray* rays, uint32_t count, uint32_t mask,
                             uint32_t flags, void* stream, uint32_t* counts);
+
+/* --------------------------------------------------------------- sampler */
+
+/* The sampler draws the path kernels make, for checking them bit for bit
+ * against a CPU restatement, as the reference checks its compiled shader in
+ * shader_matches_cpu_reference (crates/vork-render/src/path_sampler.rs:262-318).
+ * `queries` holds four uint32 per query: px, py, sample index, group.  For a
+ * group, `bits` receives the group's four scrambled 32-bit Sobol values of
+ * path_sampler(pixel, index, samples, seed) (path_sampler.slang:54-93; a
+ * kernel's float is ((bits >> 9) + 0.5) / 2^23, path_sampler.slang:70).  For
+ * group VH_SAMPLER_PATH_RANDOM it receives the white-noise stream instead:
+ * screen_ray_hash(px, py, index, seed) (vork_sampling.slang:8-15) and the
+ * states after 1, 2 and 3 path_random draws (path_sampler.slang:72).  With
+ * VH_DEVICE_POINTERS both arrays are 16-byte aligned device memory. */
+#define VH_SAMPLER_PATH_RANDOM 0xffffffffu
+int vh_sampler_bits(vh_device* device, const uint32_t* queries, uint32_t count, uint32_t samples, uint32_t seed,
+                    uint32_t flags, void* stream, uint32_t* bits);
 
 /* ----------------------------------------------------------------- scene */
 

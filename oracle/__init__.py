@@ -31,6 +31,7 @@ def lib():
     u32, f32, vp = C.c_uint32, C.c_float, C.c_void_p
     L.vo_pixel_hash.restype = u32
     L.vo_pixel_hash.argtypes = [u32]
+    L.vo_pixel_hash_many.argtypes = [vp, u32, vp]
     L.vo_screen_ray_hash.restype = u32
     L.vo_screen_ray_hash.argtypes = [u32, u32, u32, u32]
     L.vo_sobol.argtypes = [u32, vp]
@@ -80,6 +81,9 @@ def lib():
     L.vo_render_guides.argtypes = [vp, vp, vp, vp, vp, C.c_int]
     L.vo_camera_ray.argtypes = [vp, vp, u32, u32, u32, vp]
     L.vo_render_stats.argtypes = [vp, vp]
+    L.vo_camera_rays.argtypes = [vp, vp, vp, vp, u32, vp]
+    L.vo_render_pixels.argtypes = [vp, vp, vp, u32, vp, C.c_int]
+    L.vo_sample_radiance_many.argtypes = [vp, vp, vp, vp, u32, vp, C.c_int]
     _lib = L
     return L
 
@@ -168,6 +172,31 @@ class OracleScene:
         lib().vo_sample_radiance(self.handle, C.byref(params), px, py, index, ptr(out))
         return out
 
+    def camera_rays(self, params, pixels, indices):
+        """(n, 8) float32 rays (origin, tmin 0, direction, tmax) of the camera
+        samples (row-major pixel, sample index)."""
+        pix = np.ascontiguousarray(pixels, np.uint32)
+        idx = np.ascontiguousarray(np.broadcast_to(np.asarray(indices, np.uint32), pix.shape), np.uint32)
+        out = np.zeros((pix.size, 8), np.float32)
+        lib().vo_camera_rays(self.handle, C.byref(params), ptr(pix), ptr(idx), pix.size, ptr(out))
+        return out
+
+    def render_pixels(self, params, pixels, threads=0):
+        """Film sums (n, 4) float32 over the params' sample range for the
+        row-major pixel indices `pixels`."""
+        pix = np.ascontiguousarray(pixels, np.uint32)
+        out = np.zeros((pix.size, 4), np.float32)
+        lib().vo_render_pixels(self.handle, C.byref(params), ptr(pix), pix.size, ptr(out), threads)
+        return out
+
+    def sample_radiance_many(self, params, pixels, indices, threads=0):
+        """One sample's RGBA value (n, 4) per (row-major pixel, sample index)."""
+        pix = np.ascontiguousarray(pixels, np.uint32)
+        idx = np.ascontiguousarray(np.broadcast_to(np.asarray(indices, np.uint32), pix.shape), np.uint32)
+        out = np.zeros((pix.size, 4), np.float32)
+        lib().vo_sample_radiance_many(self.handle, C.byref(params), ptr(pix), ptr(idx), pix.size, ptr(out), threads)
+        return out
+
     def camera_ray(self, params, px, py, index):
         out = np.zeros(8, np.float32)
         lib().vo_camera_ray(self.handle, C.byref(params), px, py, index, ptr(out))
@@ -195,6 +224,13 @@ def sample_group_bits_many(px, py, index, count, stream, group):
                             for a in (px, py, index, group))
     out = np.zeros((index.size, 4), np.uint32)
     lib().vo_sample_group_bits_many(ptr(px), ptr(py), ptr(index), count, stream, ptr(group), index.size, ptr(out))
+    return out
+
+
+def pixel_hash_many(values):
+    v = np.ascontiguousarray(values, np.uint32)
+    out = np.zeros_like(v)
+    lib().vo_pixel_hash_many(ptr(v), v.size, ptr(out))
     return out
 
 

@@ -92,6 +92,9 @@ uint32_t vo_pixel_hash(uint32_t seed) {
     value = (value ^ (value >> 16)) * SAMPLE_HASH_MIX;
     return value ^ (value >> 16);
 }
+void vo_pixel_hash_many(const uint32_t* in, uint32_t n, uint32_t* out) {
+    for (uint32_t i = 0; i < n; i++) out[i] = vo_pixel_hash(in[i]);
+}
 /* vork_sampling.slang:8-15 */
 uint32_t vo_screen_ray_hash(uint32_t px, uint32_t py, uint32_t index, uint32_t stream) {
     uint32_t spatial = vo_pixel_hash((px * SAMPLE_HASH_MIX) ^ py);
@@ -1649,5 +1652,63 @@ void vo_render_guides(const vo_scene* s, const vh_render_params* p, float* film3
     }
     ms->stats[0] = (uint64_t)pixels * (p->sample_end - p->sample_begin);
     ms->stats[1] = ext; ms->stats[2] = shadow; ms->stats[3] = dropped;
+}
+/* vo_render over a list of pixels: out[4 i..4 i+3] = the sum of samples
+ * [sample_begin, sample_end) of pixel pixels[i] (row-major index), each
+ * sample (radiance, 1) or zero when non-finite, summed in sample order in
+ * float as the film is. */
+/* Camera rays as vh_ray records (origin, tmin 0, direction, tmax) for a list
+ * of (row-major pixel, sample index) pairs. */
+void vo_camera_rays(const vo_scene* s, const vh_render_params* p, const uint32_t* pixels, const uint32_t* indices,
+                    uint32_t n, vh_ray* out) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; i++) {
+        float r[8];
+        vo_camera_ray(s, p, pixels[i] % p->width, pixels[i] / p->width, indices[i], r);
+        out[i].origin[0] = r[0]; out[i].origin[1] = r[1]; out[i].origin[2] = r[2]; out[i].tmin = 0.f;
+        out[i].direction[0] = r[4]; out[i].direction[1] = r[5]; out[i].direction[2] = r[6]; out[i].tmax = r[3];
+    }
+}
+
+void vo_render_pixels(const vo_scene* s, const vh_render_params* p, const uint32_t* pixels, uint32_t n, float* out,
+                      int threads) {
+    if (vo_dielectric_albedo(0.5f, 0.5f, 1.5f) < 0.f) return; /* warm the tables outside the loop */
+    ensure_moments();
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#else
+    (void)threads;
+#endif
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = 0; i < (int64_t)n; i++) {
+        uint32_t px = pixels[i] % p->width, py = pixels[i] / p->width;
+        counters_t cnt = {0, 0, 0};
+        float acc[4] = {0.f, 0.f, 0.f, 0.f};
+        for (uint32_t k = p->sample_begin; k < p->sample_end; k++) {
+            float alpha;
+            v3 r = sample_radiance(s, p, px, py, k, &cnt, &alpha, NULL);
+            acc[0] += r.x; acc[1] += r.y; acc[2] += r.z; acc[3] += alpha;
+        }
+        memcpy(out + 4 * i, acc, sizeof(acc));
+    }
+}
+
+/* One sample value per (pixel, sample index) pair: out[4 i..4 i+3]. */
+void vo_sample_radiance_many(const vo_scene* s, const vh_render_params* p, const uint32_t* pixels,
+                             const uint32_t* indices, uint32_t n, float* out, int threads) {
+    if (vo_dielectric_albedo(0.5f, 0.5f, 1.5f) < 0.f) return;
+    ensure_moments();
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#else
+    (void)threads;
+#endif
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < (int64_t)n; i++) {
+        counters_t cnt = {0, 0, 0};
+        float alpha;
+        v3 r = sample_radiance(s, p, pixels[i] % p->width, pixels[i] / p->width, indices[i], &cnt, &alpha, NULL);
+        out[4 * i] = r.x; out[4 * i + 1] = r.y; out[4 * i + 2] = r.z; out[4 * i + 3] = alpha;
+    }
 }
 void vo_render_stats(const vo_scene* s, uint64_t out[4]) { memcpy(out, s->stats, sizeof(s->stats)); }

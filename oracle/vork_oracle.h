@@ -22,6 +22,7 @@ extern "C" {
 
 /* ------------------------------------------------------------ sampler */
 uint32_t vo_pixel_hash(uint32_t seed);
+void vo_pixel_hash_many(const uint32_t* in, uint32_t n, uint32_t* out);
 uint32_t vo_screen_ray_hash(uint32_t px, uint32_t py, uint32_t index, uint32_t stream);
 void vo_sobol(uint32_t index, uint32_t out[4]);
 void vo_sobol_directions(uint32_t out[3 * 32]);
@@ -112,6 +113,14 @@ void vo_render_guides(const vo_scene* scene, const vh_render_params* params, flo
 /* Camera ray of one sample: out = [origin xyz, tmax, dir xyz, 0]. */
 void vo_camera_ray(const vo_scene* scene, const vh_render_params* params, uint32_t px, uint32_t py,
                    uint32_t index, float out[8]);
+void vo_camera_rays(const vo_scene* scene, const vh_render_params* params, const uint32_t* pixels,
+                    const uint32_t* indices, uint32_t n, vh_ray* out);
+/* Film sums over [sample_begin, sample_end) for a list of row-major pixel
+ * indices (4 floats each), and single sample values per (pixel, index). */
+void vo_render_pixels(const vo_scene* scene, const vh_render_params* params, const uint32_t* pixels, uint32_t n,
+                      float* out, int threads);
+void vo_sample_radiance_many(const vo_scene* scene, const vh_render_params* params, const uint32_t* pixels,
+                             const uint32_t* indices, uint32_t n, float* out, int threads);
 /* Path statistics of the last vo_render call: [paths, extension rays, shadow rays, dropped]. */
 void vo_render_stats(const vo_scene* scene, uint64_t out[4]);
 

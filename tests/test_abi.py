@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_defaults():
     L = native.lib()
-    assert L.vh_abi_version() == 1
+    assert L.vh_abi_version() == abi.VH_ABI_VERSION == 2
     p = abi.RenderParams()
     L.vh_render_params_default(ctypes.byref(p), 1920, 1080, 64)
     assert (p.width, p.height, p.samples, p.sample_end) == (1920, 1080, 64, 64)

@@ -67,3 +67,24 @@ def test_bad_weight_blobs_are_refused():
         with pytest.raises(native.VorkError) as e:
             D.Denoiser(r.device, bad)
         assert e.value.status == 2
+
+
+def test_shipped_weights_at_1080p():
+    """The denoiser at the size it serves, 1920x1080: the coarse levels take
+    the channel split the kernel picks for this size.  Input: a 4-spp render
+    with guides of a 1.8M-triangle GGX scene."""
+    sc = S.config1(seed=2, width=1920, height=1080, sphere_level=8, terrain=512, materials=16)
+    r = native.Renderer(0)
+    inst = r.install(sc)
+    film, g = inst.render_guides(sc.render_params(samples=4, seed=9, adaptive=False))
+    den = D.Denoiser(r.device)
+    tf, tg = torch.from_numpy(film).cuda(), torch.from_numpy(g).cuda()
+    out = den(tf, tg, 4).cpu().numpy()
+    torch.backends.cudnn.allow_tf32 = False
+    torch.backends.cuda.matmul.allow_tf32 = False
+    model = D.load_model().eval().cuda()
+    with torch.no_grad():
+        x = D.features(tf[None], tg[None], 4)
+        ref = D.radiance(model(x, bf16=True))[0].permute(1, 2, 0).cpu().numpy()
+    compare(out, ref)
+    den.close()

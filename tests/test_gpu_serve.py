@@ -61,3 +61,34 @@ def test_results_are_the_same_alone_and_in_the_mix():
             assert np.array_equal(a, m)
     finally:
         backend.close()
+
+
+def test_a_1080p_64spp_request_on_a_large_scene_beside_small_ones():
+    """Config 3 at its largest request: a 1920x1080 x 64-spp render of a
+    1,835,008-triangle scene admitted with small requests of other scenes;
+    every film is the same alone (one slot, in order) and in the mix (four
+    slots at once)."""
+    big = S.config1(seed=3, sphere_level=8, terrain=512, materials=16)
+    assert big.triangle_count >= 1_800_000
+    scenes = {"big": big, "c0": S.config0(width=64, height=48, spp=4), "g": S.small_config1(width=96, height=54, spp=4)}
+    mix = [("c0", 256, 256, 4, 1), ("big", 1920, 1080, 64, 2), ("g", 512, 512, 1, 3), ("c0", 64, 48, 16, 4),
+           ("g", 96, 54, 64, 5), ("big", 256, 256, 16, 6), ("c0", 1280, 720, 1, 7)]
+    backend = serve.GpuBackend(0, slots=4)
+    try:
+        results = []
+        for slots in (1, 4):
+            s = serve.Server(backend, slots=slots)
+            try:
+                reqs = [s.submit(k, scenes[k], w, h, spp, seed, False, False) for k, w, h, spp, seed in mix]
+                s.wait_all(300)
+                results.append([r.result() for r in reqs])
+                assert s.report()["failed"] == 0
+            finally:
+                s.close()
+        alone, mixed = results
+        for (k, w, h, spp, _), a, m in zip(mix, alone, mixed):
+            assert a.shape == (h, w, 4) and np.isfinite(a).all()
+            assert np.array_equal(a[..., 3], np.full((h, w), float(spp), np.float32))
+            assert np.array_equal(a, m), (k, w, h, spp)
+    finally:
+        backend.close()

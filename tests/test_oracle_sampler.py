@@ -136,14 +136,42 @@ def test_light_point_is_independent_of_light_pick():
             assert abs(p - 0.5) < 5 * np.sqrt(0.25 / n), (count, c, p)
 
 
-def test_path_random_stream():
-    # path_sampler.slang:72: state advances by the golden-ratio step, then hashes
-    st = np.array([123], np.uint32)
-    v = oracle.lib().vo_path_random(oracle.ptr(st))
-    with np.errstate(over="ignore"):
-        expect = _hash(np.uint32(123) + U32(0x9E3779B9))[0]
-    assert st[0] == expect
-    assert v == ((float(expect >> 9) + 0.5) / 8388608.0)
+def test_path_random_stream_properties():
+    """path_sampler.slang:72: the hashed white-noise stream behind the
+    data-dependent draws (light-sample roulette), started from
+    screen_ray_hash(pixel, index, seed).  The reference holds no vectors for
+    it, so its properties are pinned: draws are bucket centres in (0, 1),
+    uniform over 256 buckets; successive draws, neighbouring pixels and
+    neighbouring sample indices are uncorrelated; and the state update is
+    injective (no collision among 2^22 states; a random map would give about
+    2,000), so streams never merge."""
+    import ctypes
+    L = oracle.lib()
+    n = 20000
+    rng = np.random.default_rng(5)
+    px, py, idx = rng.integers(0, 1920, n), rng.integers(0, 1080, n), rng.integers(0, 4096, n)
+    draws = np.zeros((n, 4))
+    nbr_x, nbr_i = np.zeros(n), np.zeros(n)
+    for k in range(n):
+        st = ctypes.c_uint32(L.vo_screen_ray_hash(int(px[k]), int(py[k]), int(idx[k]), 7))
+        for j in range(4):
+            draws[k, j] = L.vo_path_random(ctypes.byref(st))
+        st = ctypes.c_uint32(L.vo_screen_ray_hash(int(px[k]) + 1, int(py[k]), int(idx[k]), 7))
+        nbr_x[k] = L.vo_path_random(ctypes.byref(st))
+        st = ctypes.c_uint32(L.vo_screen_ray_hash(int(px[k]), int(py[k]), int(idx[k]) + 1, 7))
+        nbr_i[k] = L.vo_path_random(ctypes.byref(st))
+    assert (draws > 0).all() and (draws < 1).all()
+    assert np.all(((draws * 8388608.0) - 0.5) % 1.0 == 0)
+    for j in range(4):
+        obs = np.bincount((draws[:, j] * 256).astype(int), minlength=256)
+        e = n / 256
+        assert ((obs - e) ** 2 / e).sum() < 255 + 6 * 22.6, j
+    bound = 5 / np.sqrt(n)
+    for a, b in ((draws[:, 0], draws[:, 1]), (draws[:, 1], draws[:, 2]), (draws[:, 0], nbr_x), (draws[:, 0], nbr_i)):
+        assert abs(np.corrcoef(a, b)[0, 1]) < bound
+    x = np.arange(1 << 22, dtype=np.uint32) * np.uint32(2654435761)
+    h = oracle.pixel_hash_many(x)
+    assert np.unique(h).size == x.size
 
 
 def test_sample_values_are_bucket_centres_in_open_interval():

@@ -14,6 +14,8 @@ VH_RAY_CAMERA, VH_RAY_SHADOW, VH_RAY_DIFFUSE, VH_RAY_SPECULAR, VH_RAY_TRANSMISSI
 VH_INSTANCE_CULL_BACK_FACES = 1
 VH_TLAS_REBUILD, VH_TLAS_REFIT = 0, 1
 VH_SKY_CONSTANT, VH_SKY_PROCEDURAL = 0, 1
+VH_SAMPLER_PATH_RANDOM = 0xFFFFFFFF
+VH_ABI_VERSION = 2
 
 F3 = C.c_float * 3
 

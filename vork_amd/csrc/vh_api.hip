@@ -353,7 +353,35 @@ int vh_trace_any(const vh_tlas* t, const vh_ray* rays, uint32_t count, uint32_t 
 }
 int vh_trace_closest_counts(const vh_tlas* t, const vh_ray* rays, uint32_t count, uint32_t mask, uint32_t flags,
                             void* stream, uint32_t* counts) {
-    return trace_common(t, rays, count, mask, flags, stream, counts, 2 * sizeof(uint32_t), 2);
+    return trace_common(t, rays, count, mask, flags, stream, counts, 4 * sizeof(uint32_t), 2);
+}
+
+// ---------------------------------------------------------------- sampler
+int vh_sampler_bits(vh_device* dev, const uint32_t* queries, uint32_t count, uint32_t samples, uint32_t seed,
+                    uint32_t flags, void* stream, uint32_t* bits) {
+    clear();
+    if (!dev || (count && (!queries || !bits))) return fail(VH_ERR_USAGE, "vh_sampler_bits: null argument");
+    if (count == 0) return VH_OK;
+    VH_HIP(hipSetDevice(dev->ordinal));
+    hipStream_t s = pick(dev, stream);
+    const size_t bytes = 4 * sizeof(uint32_t) * (size_t)count;
+    if (flags & VH_DEVICE_POINTERS) {
+        if (reinterpret_cast<uintptr_t>(queries) % 16u || reinterpret_cast<uintptr_t>(bits) % 16u)
+            return fail(VH_ERR_USAGE, "vh_sampler_bits: device arrays must be 16-byte aligned");
+        int rc = launch_sampler_bits(s, queries, count, samples, seed, bits);
+        if (rc) return rc;
+        VH_HIP(hipStreamSynchronize(s));
+        return VH_OK;
+    }
+    DeviceBuffer dq, dout;
+    VH_HIP(dq.allocate(bytes));
+    VH_HIP(dout.allocate(bytes));
+    VH_HIP(hipMemcpyAsync(dq.ptr, queries, bytes, hipMemcpyHostToDevice, s));
+    int rc = launch_sampler_bits(s, dq.as<uint32_t>(), count, samples, seed, dout.as<uint32_t>());
+    if (rc) return rc;
+    VH_HIP(hipMemcpyAsync(bits, dout.ptr, bytes, hipMemcpyDeviceToHost, s));
+    VH_HIP(hipStreamSynchronize(s));
+    return VH_OK;
 }
 
 // ------------------------------------------------------------------ scene

@@ -196,7 +196,12 @@ int launch_instance_records(hipStream_t s, const vh_instance* inst, uint32_t n, 
 
 // ---------------------------------------------------------------- trace
 int launch_trace(vh_device* dev, hipStream_t s, const TlasView& tv, const void* rays, uint32_t count, uint32_t mask,
-                 void* out, int mode);  // mode 0 closest -> vh_hit, 1 any -> uint8, 2 counts -> uint32 pairs
+                 void* out, int mode);  // mode 0 closest -> vh_hit, 1 any -> uint8, 2 counts -> 4 uint32 per ray
+
+// -------------------------------------------------------------- sampler
+// queries and out: 4 uint32 per query, device memory, 16-byte aligned
+int launch_sampler_bits(hipStream_t s, const uint32_t* queries, uint32_t n, uint32_t samples, uint32_t seed,
+                        uint32_t* out);
 
 // ------------------------------------------------------------------ LUT
 int bake_luts(vh_device* dev);

@@ -880,7 +880,35 @@ __global__ void k_bake_dielectric(float* out) {
 
 inline uint32_t blocks(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
 
+// Sampler draws for a list of (px, py, index, group) queries, through the
+// same group_bits / path_random the path kernels call.
+__global__ void k_sampler_bits(const uint4* queries, uint32_t n, uint32_t samples, uint32_t seed, uint4* out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint4 q = queries[i];
+    uint32_t b[4];
+    if (q.w == VH_SAMPLER_PATH_RANDOM) {
+        uint32_t state = screen_ray_hash(q.x, q.y, q.z, seed);
+        b[0] = state;
+        for (int k = 1; k < 4; k++) {
+            (void)path_random(state);
+            b[k] = state;
+        }
+    } else {
+        group_bits(make_sampler(q.x, q.y, q.z, samples, seed), q.w, b);
+    }
+    out[i] = make_uint4(b[0], b[1], b[2], b[3]);
+}
+
 }  // namespace
+
+int launch_sampler_bits(hipStream_t s, const uint32_t* queries, uint32_t n, uint32_t samples, uint32_t seed,
+                        uint32_t* out) {
+    hipLaunchKernelGGL(k_sampler_bits, dim3(blocks(n, kBlock)), dim3(kBlock), 0, s,
+                       reinterpret_cast<const uint4*>(queries), n, samples, seed, reinterpret_cast<uint4*>(out));
+    VH_HIP(hipGetLastError());
+    return VH_OK;
+}
 
 int bake_luts(vh_device* dev) {
     hipStream_t s = dev->stream;

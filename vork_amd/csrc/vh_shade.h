@@ -60,7 +60,9 @@ __device__ __forceinline__ Sampler make_sampler(uint32_t px, uint32_t py, uint32
 }
 // path_sampler.slang:74-93: Sobol dimensions 1-3 from their generator
 // recurrences (degree 1, 2, 3 primitive polynomials), one column per bit.
-__device__ __forceinline__ void group_values(const Sampler& s, uint32_t group, float out[4]) {
+// The four scrambled 32-bit values of a group; group_values maps them to
+// (0, 1) with 2^23 buckets.
+__device__ __forceinline__ void group_bits(const Sampler& s, uint32_t group, uint32_t out[4]) {
     uint32_t seed = pixel_hash(s.seed ^ pixel_hash(group + 1u));
     uint32_t index = rev(owen_reversed(rev(s.index), seed ^ 0x6c8e9cf5u)) & s.mask;
     uint32_t x0 = 0, x1 = 0, x2 = 0;
@@ -74,10 +76,16 @@ __device__ __forceinline__ void group_values(const Sampler& s, uint32_t group, f
         uint32_t nr = ry ^ rx ^ (rx >> 3);
         rx = ry; ry = rz; rz = nr;
     }
-    out[0] = sample_unit(rev(owen_reversed(index, seed ^ 0xa511e9b3u)));
-    out[1] = sample_unit(rev(owen_reversed(rev(x0), seed ^ 0x63d83595u)));
-    out[2] = sample_unit(rev(owen_reversed(rev(x1), seed ^ 0x2f7c4a1bu)));
-    out[3] = sample_unit(rev(owen_reversed(rev(x2), seed ^ kStreamStep)));
+    out[0] = rev(owen_reversed(index, seed ^ 0xa511e9b3u));
+    out[1] = rev(owen_reversed(rev(x0), seed ^ 0x63d83595u));
+    out[2] = rev(owen_reversed(rev(x1), seed ^ 0x2f7c4a1bu));
+    out[3] = rev(owen_reversed(rev(x2), seed ^ kStreamStep));
+}
+__device__ __forceinline__ void group_values(const Sampler& s, uint32_t group, float out[4]) {
+    uint32_t b[4];
+    group_bits(s, group, b);
+#pragma unroll
+    for (int k = 0; k < 4; k++) out[k] = sample_unit(b[k]);
 }
 // path_sampler.slang:72
 __device__ __forceinline__ float path_random(uint32_t& state) {

@@ -8,7 +8,8 @@ namespace {
 
 constexpr int kTraceBlock = 256;
 
-// Rays from a vh_ray array; results to vh_hit / occlusion bytes / visit counts.
+// Rays from a vh_ray array; results to vh_hit / occlusion bytes / visit counts
+// (nodes, triangles, deepest stack entry, 0).
 template <int kMode>
 struct ArraySource {
     const vh_ray* rays;
@@ -48,8 +49,11 @@ struct ArraySource {
         } else if (kMode == 1) {
             static_cast<uint8_t*>(out)[i] = h.found ? 1 : 0;
         } else {
-            static_cast<uint32_t*>(out)[2 * i] = c.nodes;
-            static_cast<uint32_t*>(out)[2 * i + 1] = c.tris;
+            uint32_t* q = static_cast<uint32_t*>(out) + 4 * (size_t)i;  // 4-byte aligned caller arrays
+            q[0] = c.nodes;
+            q[1] = c.tris;
+            q[2] = c.depth;
+            q[3] = 0u;
         }
     }
 };

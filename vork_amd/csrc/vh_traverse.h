@@ -113,6 +113,7 @@ struct TraceHit {
 
 struct TraceCounts {
     uint32_t nodes, tris;
+    uint32_t depth;  // deepest stack entry the ray used (entries beyond kLdsStack live in global memory)
 };
 
 union NodeBits {
@@ -238,7 +239,7 @@ struct Tracer {
         bnodes = nullptr; btris = nullptr; bids = nullptr;
         set_root();
         hit.found = false; hit.t = tmax_; hit.u = hit.v = 0.f; hit.inst = hit.item = 0xffffffffu;
-        cnt.nodes = cnt.tris = 0;
+        cnt.nodes = cnt.tris = cnt.depth = 0;
         st.sp = 0;
         done = tv.node_count == 0;
     }
@@ -436,6 +437,9 @@ __device__ bool trace_persistent(const TlasView& tv, const Src& src, uint32_t* n
                     ok = false;
                     tr.done = true;
                 }
+                // the phases only push (pops happen in pop_phase), so the
+                // stack is at its deepest for this step here
+                if (kCount) tr.cnt.depth = max(tr.cnt.depth, (uint32_t)st.sp);
                 if (tr.done) {
                     src.store(key, tr.hit, tr.cnt);
                     active = false;

@@ -26,7 +26,7 @@ EXPORTS = [
     "vh_blas_build", "vh_blas_get_info", "vh_blas_release", "vh_tlas_build", "vh_tlas_release",
     "vh_trace_closest", "vh_trace_any", "vh_trace_closest_counts", "vh_scene_create", "vh_scene_set_camera",
     "vh_scene_release", "vh_render_params_default", "vh_render", "vh_render_guides", "vh_render_get_stats",
-    "vh_denoiser_create", "vh_denoise", "vh_denoiser_release",
+    "vh_denoiser_create", "vh_denoise", "vh_denoiser_release", "vh_sampler_bits",
 ]
 
 _lib = None
@@ -90,6 +90,8 @@ def lib():
     L.vh_denoise.argtypes = [vp, u32, u32, C.c_float, vp, vp, u32, vp, vp]
     L.vh_denoiser_release.restype = None
     L.vh_denoiser_release.argtypes = [vp]
+    L.vh_sampler_bits.restype = i32
+    L.vh_sampler_bits.argtypes = [vp, vp, u32, u32, u32, u32, vp, vp]
     L.vh_render_get_stats.restype = i32
     L.vh_render_get_stats.argtypes = [vp, C.POINTER(abi.RenderStats)]
     _lib = L
@@ -130,6 +132,15 @@ class Device:
             self.close()
         except Exception:
             pass
+
+
+    def sampler_bits(self, queries, samples, seed):
+        """queries (n, 4) uint32: px, py, sample index, group (or
+        abi.VH_SAMPLER_PATH_RANDOM).  Returns (n, 4) uint32 drawn on the GPU."""
+        q = np.ascontiguousarray(queries, np.uint32).reshape(-1, 4)
+        out = np.zeros_like(q)
+        check(lib().vh_sampler_bits(self.handle, _ptr(q), q.shape[0], samples, seed, 0, None, _ptr(out)))
+        return out
 
 
 class Blas:
@@ -222,8 +233,10 @@ class Tlas:
         return occ
 
     def trace_counts(self, rays, mask=0xFF):
+        """(n, 4) uint32 per ray: nodes visited, triangles tested, deepest
+        traversal-stack entry, 0."""
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
-        counts = np.zeros((rays.shape[0], 2), np.uint32)
+        counts = np.zeros((rays.shape[0], 4), np.uint32)
         check(lib().vh_trace_closest_counts(self.handle, _ptr(rays), rays.shape[0], mask, 0, None, _ptr(counts)))
         return counts
 
