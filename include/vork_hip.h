@@ -185,6 +185,20 @@ int vh_trace_closest(const vh_tlas* tlas, const vh_ray* rays, uint32_t count, ui
 int vh_trace_any(const vh_tlas* tlas, const vh_ray* rays, uint32_t count, uint32_t mask,
                  uint32_t flags, void* stream, uint8_t* occluded);
 
+/* vh_trace_closest for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * no counterpart in the reference.  `rays` (16-byte aligned) and `hits`
+ * (4-byte aligned) are device memory on the TLAS's device; at most 2^31 rays
+ * per call, else VH_ERR_USAGE.  Gives the hits vh_trace_closest gives, with
+ * the same kernel, and allocates nothing: its counters and traversal
+ * overflow stacks live in a static device array of the library (64 MB per
+ * GPU, looked up at vh_device_create, held until the process exits; at most
+ * 128 workgroups of it).  Every call on a GPU uses that GPU's one array, so
+ * the library serializes these calls per GPU and each returns once its work
+ * is done; they cannot overlap one another. */
+int vh_trace_closest_static(const vh_tlas* tlas, const vh_ray* rays, uint32_t count, uint32_t mask, void* stream,
+                            vh_hit* hits);
+
 /* Per-ray traversal work of the closest-hit kernel, four uint32 per ray:
  * [nodes visited, triangles tested, deepest traversal-stack entry, 0].  Used
  * to state algorithmic bytes and to show how deep traversals go (entries

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -34,6 +35,7 @@ void device_drop(vh_device* d) {
     d->lut_moments.release();
     d->lut_dielectric.release();
     d->streams.clear();
+    if (d->static_host) (void)hipHostFree(d->static_host);
     if (d->stream) (void)hipStreamDestroy(d->stream);
     delete d;
 }
@@ -99,6 +101,16 @@ int vh_device_create(int ordinal, vh_device** out) {
         return hip_fail(e, "hipStreamCreate");
     }
     int rc = bake_luts(d);
+    // set-up of vh_trace_closest_static, with this device current: its
+    // static scratch's address here and a pinned word for its error flags
+    if (!rc) rc = static_scratch_address(&d->static_scratch);
+    if (!rc) {
+        e = hipHostMalloc(reinterpret_cast<void**>(&d->static_host), sizeof(uint32_t), hipHostMallocDefault);
+        if (e != hipSuccess) {
+            d->static_host = nullptr;
+            rc = hip_fail(e, "hipHostMalloc");
+        }
+    }
     if (rc) {
         vh_device_release(d);
         return rc;
@@ -351,6 +363,29 @@ int vh_trace_any(const vh_tlas* t, const vh_ray* rays, uint32_t count, uint32_t 
                  uint8_t* occluded) {
     return trace_common(t, rays, count, mask, flags, stream, occluded, 1, 1);
 }
+// Calls on one GPU share that GPU's static scratch; a lock per GPU ordinal
+// keeps them from overlapping, whichever vh_device handle they come through.
+static std::mutex g_static_trace_locks[64];
+
+int vh_trace_closest_static(const vh_tlas* t, const vh_ray* rays, uint32_t count, uint32_t mask, void* stream,
+                            vh_hit* hits) {
+    clear();
+    if (!t || (count && (!rays || !hits))) return fail(VH_ERR_USAGE, "vh_trace_closest_static: null argument");
+    if (count > kStaticTraceMaxRays)
+        return fail(VH_ERR_USAGE, "vh_trace_closest_static: more than 2^31 rays in one call");
+    if (reinterpret_cast<uintptr_t>(rays) % 16u)
+        return fail(VH_ERR_USAGE, "vh_trace_closest_static: the ray array must be 16-byte aligned device memory");
+    if (reinterpret_cast<uintptr_t>(hits) % 4u)
+        return fail(VH_ERR_USAGE, "vh_trace_closest_static: the hit array must be 4-byte aligned device memory");
+    if (count == 0) return VH_OK;
+    vh_device* dev = t->device;
+    if (dev->ordinal < 0 || dev->ordinal >= 64 || !dev->static_scratch || !dev->static_host)
+        return fail(VH_ERR_DEVICE, "vh_trace_closest_static: no static scratch on this device");
+    std::lock_guard<std::mutex> hold(g_static_trace_locks[dev->ordinal]);
+    VH_HIP(hipSetDevice(dev->ordinal));
+    return launch_trace_static(dev, pick(dev, stream), t->view(), rays, count, mask, hits);
+}
+
 int vh_trace_closest_counts(const vh_tlas* t, const vh_ray* rays, uint32_t count, uint32_t mask, uint32_t flags,
                             void* stream, uint32_t* counts) {
     return trace_common(t, rays, count, mask, flags, stream, counts, 4 * sizeof(uint32_t), 2);

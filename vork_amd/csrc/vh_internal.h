@@ -84,6 +84,11 @@ struct vh_device {
     // and kept (grown to the largest call) until the device is released
     std::mutex streams_lock;
     std::map<hipStream_t, std::unique_ptr<vh::StreamState>> streams;
+    // vh_trace_closest_static: this GPU's address of the library's static
+    // scratch array (looked up once, at device creation) and a pinned host
+    // word its error flags are read back into
+    void* static_scratch = nullptr;
+    uint32_t* static_host = nullptr;
 };
 
 namespace vh {
@@ -195,6 +200,15 @@ int launch_instance_records(hipStream_t s, const vh_instance* inst, uint32_t n, 
                             InstanceRecord* out);
 
 // ---------------------------------------------------------------- trace
+// The device address of the static scratch of the allocation-free closest-hit
+// entry point on the current device (hipGetSymbolAddress).
+int static_scratch_address(void** out);
+// Closest hit of `count` device-resident rays into device-resident hits with
+// the static scratch only; `dev->static_scratch` must be set.
+int launch_trace_static(vh_device* dev, hipStream_t s, const TlasView& tv, const vh_ray* rays, uint32_t count,
+                        uint32_t mask, vh_hit* hits);
+// Largest ray count one vh_trace_closest_static call takes.
+constexpr uint32_t kStaticTraceMaxRays = 1u << 31;
 int launch_trace(vh_device* dev, hipStream_t s, const TlasView& tv, const void* rays, uint32_t count, uint32_t mask,
                  void* out, int mode);  // mode 0 closest -> vh_hit, 1 any -> uint8, 2 counts -> 4 uint32 per ray
 

@@ -67,7 +67,55 @@ VH_TRACE_KERNEL void k_trace(TlasView tv, ArraySource<kMode> src, uint32_t* over
     if (!ok) atomicOr(&counters[0], kStackOverflow);
 }
 
+// Zeroes `n` words (n <= blockDim.x): a fill kernel, so that the clear is an
+// ordinary kernel node wherever the launch sequence is recorded.
+__global__ void k_clear_words(uint32_t* w, uint32_t n) {
+    if (threadIdx.x < n) w[threadIdx.x] = 0u;
+}
+
+// Scratch of vh_trace_closest_static, the closest-hit entry point that
+// allocates nothing: 256 bytes of counters ([0] error flags, [1] ray claims),
+// then the traversal overflow stacks of a grid of at most kStaticBlocks
+// workgroups (kGlobalStack 8-byte entries per thread, the same depth as the
+// allocating entry point, so both overflow on the same rays).  128
+// workgroups x 256 threads x 244 entries x 8 B = 64 MB per GPU, held by every
+// process that loads the library until it exits.  Host code never uses this
+// name as an address: each device's copy is looked up once with
+// hipGetSymbolAddress and the kernels receive that pointer.
+constexpr uint32_t kStaticBlocks = 128;
+constexpr size_t kStaticCounterBytes = 256;
+constexpr size_t kStaticWords =
+    kStaticCounterBytes / sizeof(uint64_t) + (size_t)kStaticBlocks * kTraceBlock * kGlobalStack;
+static __device__ __align__(256) uint64_t scratch[kStaticWords];
+
 }  // namespace
+
+int static_scratch_address(void** out) {
+    void* p = nullptr;
+    VH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(VH_ERR_DEVICE, "hipGetSymbolAddress(scratch) gave no address");
+    *out = p;
+    return VH_OK;
+}
+
+int launch_trace_static(vh_device* dev, hipStream_t s, const TlasView& tv, const vh_ray* rays, uint32_t count,
+                        uint32_t mask, vh_hit* hits) {
+    if (count == 0) return VH_OK;
+    char* base = static_cast<char*>(dev->static_scratch);
+    uint32_t* counters = reinterpret_cast<uint32_t*>(base);
+    uint32_t* of = reinterpret_cast<uint32_t*>(base + kStaticCounterBytes);
+    uint32_t grid = (count + kTraceBlock - 1) / kTraceBlock;
+    if (grid > kStaticBlocks) grid = kStaticBlocks;
+    // the counters keep the last call's values: clear them first
+    hipLaunchKernelGGL(k_clear_words, dim3(1), dim3(64), 0, s, counters, 2u);
+    hipLaunchKernelGGL(k_trace<0>, dim3(grid), dim3(kTraceBlock), 0, s, tv,
+                       ArraySource<0>{rays, count, mask, hits, tv.instances, tv.blases}, of, counters);
+    VH_HIP(hipGetLastError());
+    VH_HIP(hipMemcpyAsync(dev->static_host, counters, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    VH_HIP(hipStreamSynchronize(s));
+    if (*dev->static_host & kStackOverflow) return fail(VH_ERR_DEVICE, "traversal stack overflow");
+    return VH_OK;
+}
 
 // Workgroups resident at once for the persistent trace kernels.
 uint32_t trace_grid(vh_device* dev, uint32_t n) {

@@ -26,7 +26,7 @@ EXPORTS = [
     "vh_blas_build", "vh_blas_get_info", "vh_blas_release", "vh_tlas_build", "vh_tlas_release",
     "vh_trace_closest", "vh_trace_any", "vh_trace_closest_counts", "vh_scene_create", "vh_scene_set_camera",
     "vh_scene_release", "vh_render_params_default", "vh_render", "vh_render_guides", "vh_render_get_stats",
-    "vh_denoiser_create", "vh_denoise", "vh_denoiser_release", "vh_sampler_bits",
+    "vh_denoiser_create", "vh_denoise", "vh_denoiser_release", "vh_sampler_bits", "vh_trace_closest_static",
 ]
 
 _lib = None
@@ -74,6 +74,8 @@ def lib():
         f = getattr(L, name)
         f.restype = i32
         f.argtypes = [vp, vp, u32, u32, u32, vp, vp]
+    L.vh_trace_closest_static.restype = i32
+    L.vh_trace_closest_static.argtypes = [vp, vp, u32, u32, vp, vp]
     L.vh_scene_create.restype = i32
     L.vh_scene_create.argtypes = [vp, vp, C.POINTER(abi.SceneDesc), C.POINTER(vp)]
     L.vh_scene_set_camera.restype = i32
