@@ -44,9 +44,12 @@ sys.path.insert(0, ROOT)
 
 S_NODE, S_TRI, S_RAYIN, S_HITOUT = 80, 36, 32, 16   # bytes: wide node, triangle record, ray in, hit out
 HBM_PEAK_GBPS = 8000.0
-# HBM-side bytes per k_extend launch of the default config-1 step, measured
-# with rocprofv3 FETCH_SIZE / WRITE_SIZE and scaled by a plain copy kernel
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_extend_config1_1080p_64spp.traffic.json")
+# HBM-side traffic of the traced kernels, measured by this script's own
+# rocprofv3 children (FETCH_SIZE, then WRITE_SIZE, each over one render step
+# and over a plain device copy of known size that scales the counter to bytes)
+TRAFFIC_KERNELS = {"k_extend_packet": "k_extend_packet", "k_extend": "k_extend<false>", "k_shade": "k_shade",
+                   "k_shadow": "k_shadow<false>"}
+COPY_MIB = 1024
 
 
 def log(*a):
@@ -77,6 +80,10 @@ def main():
     ap.add_argument("--batch", type=int, default=0, help="exposure samples per wavefront batch (0 = auto)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 FETCH_SIZE / WRITE_SIZE passes that measure HBM traffic")
+    ap.add_argument("--traffic-pass", action="store_true",
+                    help="(run under rocprofv3 by this script) one untimed render step, no report")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read + write) three times, for scaling "
                          "rocprof FETCH_SIZE / WRITE_SIZE to bytes")
@@ -178,6 +185,13 @@ def main():
         dev.synchronize()
         return inst, blases
 
+    if args.traffic_pass:
+        film.zero_()
+        torch.cuda.synchronize()
+        step()
+        log("traffic pass: one render step done")
+        return
+
     for w in range(args.warmup):
         film.zero_()
         torch.cuda.synchronize()
@@ -221,21 +235,49 @@ def main():
         per_launch = algorithmic / launches
         avg_ms = prof["extend_ms"] / launches
         achieved = per_launch / (avg_ms * 1e-3) / 1e9
+        # the camera bounce (k_extend_packet: coherent camera rays as wave
+        # packets) and the later bounces (k_extend: incoherent rays, per lane)
+        # priced apart; both by the per-ray visits of the per-lane traversal
+        cam_bytes = (cnt["extend_camera_node_visits"] * S_NODE + cnt["extend_camera_triangle_tests"] * S_TRI
+                     + cnt["extend_camera_rays"] * (S_RAYIN + S_HITOUT))
+        parts = {}
+        for name, nbytes, ms, n, rays, nodes, tris in (
+                ("k_extend_packet", cam_bytes, prof["extend_camera_ms"], prof["extend_camera_launches"],
+                 cnt["extend_camera_rays"], cnt["extend_camera_node_visits"], cnt["extend_camera_triangle_tests"]),
+                ("k_extend", algorithmic - cam_bytes, prof["extend_ms"] - prof["extend_camera_ms"],
+                 launches - prof["extend_camera_launches"], cnt["extension_rays"] - cnt["extend_camera_rays"],
+                 cnt["extend_node_visits"] - cnt["extend_camera_node_visits"],
+                 cnt["extend_triangle_tests"] - cnt["extend_camera_triangle_tests"])):
+            n = max(n, 1)
+            gbps = nbytes / n / (ms / n * 1e-3) / 1e9 if ms > 0 else None
+            parts[name] = {"launches": n, "avg_launch_ms": round(ms / n, 4), "bytes_per_launch": int(nbytes / n),
+                           "rays": rays, "node_visits_per_ray": round(nodes / max(rays, 1), 2),
+                           "triangle_tests_per_ray": round(tris / max(rays, 1), 2),
+                           "achieved": round(gbps, 1) if gbps else None,
+                           "frac": round(gbps / HBM_PEAK_GBPS, 4) if gbps else None}
         roofline = {"kernel": "extend stage: k_extend_packet (camera rays) + k_extend (later bounces), closest-hit traversal", "bound": "hbm",
                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
                     "bytes_per_launch": int(per_launch), "avg_launch_ms": round(avg_ms, 4), "launches": launches,
                     "node_visits_per_ray": round(cnt["extend_node_visits"] / max(cnt["extension_rays"], 1), 2),
                     "triangle_tests_per_ray": round(cnt["extend_triangle_tests"] / max(cnt["extension_rays"], 1), 2),
+                    "per_launch": parts,
                     "stage_ms": {"extend": round(prof["extend_ms"], 1), "shade": round(prof["shade_ms"], 1),
                                  "shadow": round(prof["shadow_ms"], 1), "other": round(prof["other_ms"], 1)}}
-
-    default_c1 = (args.workload == "config1" and (args.width, args.height, sc.spp, args.sphere_level, args.terrain)
-                  == (1920, 1080, 64, 9, 1024))
-    if roofline is not None and default_c1 and os.path.exists(TRAFFIC_FILE):
-        with open(TRAFFIC_FILE) as f:
-            roofline["traffic"] = json.load(f)["traffic_per_launch"]
-        roofline["traffic_source"] = "rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, copy-calibrated (profiles/)"
+        if not args.no_traffic:
+            tr = measure_traffic(args)
+            roofline["traffic_measured"] = tr
+            ext = tr.get("kernels", {})
+            if "k_extend_packet" in ext and "k_extend" in ext:
+                a, b = ext["k_extend_packet"], ext["k_extend"]
+                stage_bytes = a["bytes_per_launch"] * a["launches"] + b["bytes_per_launch"] * b["launches"]
+                roofline["traffic"] = int(stage_bytes / (a["launches"] + b["launches"]))
+                roofline["traffic_source"] = ("rocprofv3 FETCH_SIZE + WRITE_SIZE of this run's own children, "
+                                              "per launch of the extend stage, copy-calibrated")
+                for k in ("k_extend_packet", "k_extend"):
+                    t = ext[k]["bytes_per_launch"]
+                    parts[k]["traffic_bytes_per_launch"] = t
+                    parts[k]["traffic_gbps"] = round(t / (parts[k]["avg_launch_ms"] * 1e-3) / 1e9, 1)
 
     cpu = None
     if not args.no_cpu_baseline and args.workload == "config1":
@@ -425,24 +467,108 @@ def serve_bench(args):
         "evictions": server.evictions}
 
 
-def cpu_baseline(sc):
+CPU_SHARE = 16   # host cores a one-GPU box gives its jobs
+
+
+def cpu_baseline(sc, samples=8):
     """The CPU oracle (a restatement of the estimator) on the host cores, over a
-    bounded sample of the same workload: exposure sample 0 of every pixel."""
+    bounded sample of the same workload: exposure samples [0, samples) of
+    every pixel, about 10 s of CPU work."""
     import oracle
     from vork_amd import native  # noqa: F401  (the oracle shares the ABI structs)
-    threads = min(os.cpu_count() or 1, 16)
+    visible = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    threads = min(visible, CPU_SHARE)
     t = time.time()
     o = oracle.OracleScene(sc)
     build_s = time.time() - t
-    p = sc.render_params()
-    p.sample_begin, p.sample_end = 0, 1
+    p = sc.render_params(adaptive=False)
+    p.sample_begin, p.sample_end = 0, samples
     t = time.perf_counter()
     o.render(p, threads=threads, film32=False)
     dt = time.perf_counter() - t
-    n = sc.width * sc.height
+    n = sc.width * sc.height * samples
     return {"value": round(n / dt / 1e6, 4), "unit": "Msamples/s", "cores": threads, "kind": "port",
-            "sample": f"exposure sample 0 of 64 for all {sc.width}x{sc.height} pixels ({n} samples) "
-                      f"in {dt:.1f}s; oracle BVH build {build_s:.1f}s untimed"}
+            "host_cpus_visible": visible,
+            "sample": f"exposure samples 0-{samples - 1} of {sc.spp} for all {sc.width}x{sc.height} pixels "
+                      f"({n} samples) in {dt:.1f}s on {threads} threads (all host cores of the run, at most the "
+                      f"one-GPU box's share of {CPU_SHARE}); oracle BVH build {build_s:.1f}s untimed"}
+
+
+def _rocprof_counter(counter, argv, outdir, timeout=300):
+    """Runs `argv` (a python command) under rocprofv3 --pmc <counter> and
+    returns {kernel name: [value per dispatch]} from its CSV."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        raise RuntimeError("rocprofv3 not found")
+    if os.path.isdir(outdir):
+        shutil.rmtree(outdir)
+    cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", outdir, "-o", "run", "--", *argv]
+    env = dict(os.environ, TMPDIR=os.environ.get("TMPDIR", "/tmp"))
+    r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=timeout)
+    files = glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise RuntimeError(f"rocprofv3 --pmc {counter} wrote no counter CSV (rc {r.returncode}): "
+                           + r.stdout[-400:].replace("\n", " | "))
+    values = {}
+    for f in files:
+        with open(f) as fh:
+            rows = list(csv.DictReader(fh))
+        per = {}
+        for row in rows:
+            if row.get("Counter_Name") != counter:
+                continue
+            key = (row.get("Dispatch_Id"), row.get("Kernel_Name", ""))
+            per[key] = per.get(key, 0.0) + float(row["Counter_Value"])
+        for (_, name), v in per.items():
+            values.setdefault(name, []).append(v)
+    return values
+
+
+def measure_traffic(args):
+    """HBM bytes per launch of the traced kernels, from rocprofv3 FETCH_SIZE and
+    WRITE_SIZE (KiB per dispatch) over one render step of the same workload,
+    each scaled by what the counter reads for a plain device copy of known
+    size.  Runs in child processes; returns the figures or the error."""
+    out = {}
+    base = os.path.join(ROOT, "gpurun_out", "bench_traffic")
+    me = [sys.executable, os.path.abspath(__file__)]
+    step = me + ["--traffic-pass", "--workload", args.workload, "--width", str(args.width), "--height",
+                 str(args.height), "--spp", str(args.spp), "--sphere-level", str(args.sphere_level), "--terrain",
+                 str(args.terrain), "--batch", str(args.batch)]
+    copy = me + ["--copy-calibration", str(COPY_MIB)]
+    try:
+        t = time.time()
+        scale = {}
+        for counter, truth in (("FETCH_SIZE", COPY_MIB << 20), ("WRITE_SIZE", COPY_MIB << 20)):
+            vals = _rocprof_counter(counter, copy, os.path.join(base, "copy_" + counter))
+            kib = [v for name, vs in vals.items() if "copy" in name.lower() or "add" in name.lower() for v in vs]
+            if not kib:
+                raise RuntimeError(f"no copy kernel in the {counter} calibration run: {sorted(vals)[:4]}")
+            scale[counter] = truth / (max(kib) * 1024.0)
+            log(f"traffic: {counter} copy calibration {max(kib):.0f} KiB for {truth >> 20} MiB")
+        kernels = {}
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            vals = _rocprof_counter(counter, step, os.path.join(base, "step_" + counter))
+            for key, pattern in TRAFFIC_KERNELS.items():
+                got = [v for name, vs in vals.items() if pattern in name for v in vs]
+                if got:
+                    k = kernels.setdefault(key, {"launches": len(got)})
+                    k[counter] = sum(got) / len(got) * 1024.0 * scale[counter]
+            log(f"traffic: {counter} pass done")
+        for k in kernels.values():
+            k["bytes_per_launch"] = int(k.get("FETCH_SIZE", 0.0) + k.get("WRITE_SIZE", 0.0))
+            k["read_bytes_per_launch"] = int(k.pop("FETCH_SIZE", 0.0))
+            k["written_bytes_per_launch"] = int(k.pop("WRITE_SIZE", 0.0))
+        out = {"kernels": kernels, "fetch_scale": round(scale["FETCH_SIZE"], 4),
+               "write_scale": round(scale["WRITE_SIZE"], 4), "seconds": round(time.time() - t, 1)}
+    except Exception as e:  # reported, never required
+        out = {"error": f"{type(e).__name__}: {e}"}
+    return out
 
 
 if __name__ == "__main__":

@@ -402,6 +402,11 @@ typedef struct vh_render_stats {
     uint64_t shadow_node_visits, shadow_triangle_tests;
     /* with VH_RENDER_ADAPTIVE: samples of stopped pixels (their mean added, no path) */
     uint64_t stopped_samples;
+    /* the camera bounce's part of the extend stage (the first launch of each batch): its device time and
+     * launches with VH_RENDER_PROFILE, its rays, nodes and triangles with VH_RENDER_COUNT_VISITS */
+    double extend_camera_ms;
+    uint64_t extend_camera_launches, extend_camera_rays;
+    uint64_t extend_camera_node_visits, extend_camera_triangle_tests;
 } vh_render_stats;
 int vh_render_get_stats(const vh_scene* scene, vh_render_stats* stats);
 

@@ -281,3 +281,33 @@ def test_denoiser_guides_match_the_oracle(renderer, make, spp):
     assert np.array_equal(g[..., 0, 3], go[..., 0, 3])   # the same camera hits
     close = np.abs(g - go) <= 1e-4 + 1e-4 * np.abs(go)
     assert close.mean() > 0.999, close.mean()
+
+
+def test_graph_replays_equal_direct_launches(renderer):
+    """Every batch replays the stream's recorded graph of its launch sequence
+    (the kernels read the batch from a device-side launch block); with
+    VH_RENDER_PROFILE the same kernels are launched one by one instead.  The
+    films agree bit for bit across batch sizes, adaptive batches (a decide
+    pass before each late sample), two scenes alternating on one stream, and
+    replays separated by other device work."""
+    torch = pytest.importorskip("torch")
+    from vork_amd import abi
+    scenes = [renderer.install(S.config0(width=40, height=32)), renderer.install(S.small_config1(width=48, height=27))]
+    for rep in range(3):
+        for k, inst in enumerate(scenes):
+            for batch, adaptive in ((0, False), (1, False), (3, False), (0, True)):
+                p = inst.scene.render_params(samples=20, seed=5 + k, adaptive=adaptive)
+                p.batch_samples = batch
+                if adaptive:
+                    p.adaptive_threshold, p.adaptive_min_samples = 0.1, 6
+                graph = inst.render(p)
+                st_graph = inst.stats()
+                p.flags |= abi.VH_RENDER_PROFILE
+                direct = inst.render(p)
+                assert np.array_equal(graph, direct), (rep, k, batch, adaptive)
+                st = inst.stats()
+                assert st["paths"] == st_graph["paths"] and st["extension_rays"] == st_graph["extension_rays"]
+                assert st["extend_launches"] > 0 and st["extend_camera_launches"] >= 1
+            x = torch.randn(1 << 20, device="cuda")
+            torch.cuda.synchronize()
+            assert float((x * 0).sum()) == 0.0

@@ -96,7 +96,10 @@ class RenderStats(C.Structure):
                 ("other_ms", C.c_double), ("extend_launches", C.c_uint64), ("shade_launches", C.c_uint64),
                 ("shadow_launches", C.c_uint64), ("extend_node_visits", C.c_uint64),
                 ("extend_triangle_tests", C.c_uint64), ("shadow_node_visits", C.c_uint64),
-                ("shadow_triangle_tests", C.c_uint64), ("stopped_samples", C.c_uint64)]
+                ("shadow_triangle_tests", C.c_uint64), ("stopped_samples", C.c_uint64),
+                ("extend_camera_ms", C.c_double), ("extend_camera_launches", C.c_uint64),
+                ("extend_camera_rays", C.c_uint64), ("extend_camera_node_visits", C.c_uint64),
+                ("extend_camera_triangle_tests", C.c_uint64)]
 
 
 VH_RENDER_PROFILE, VH_RENDER_COUNT_VISITS, VH_RENDER_ADAPTIVE = 1, 2, 4
@@ -120,7 +123,7 @@ def default_params(width, height, samples, seed=1, max_bounces=12, adaptive=True
 
 def check_layout():
     sizes = {Geometry: 48, BlasInfo: 40, Instance: 64, Ray: 32, Hit: 24, Material: 32, RectLight: 80,
-             Camera: 64, Sky: 80, RenderParams: 76, RenderStats: 136}
+             Camera: 64, Sky: 80, RenderParams: 76, RenderStats: 176}
     for t, n in sizes.items():
         if C.sizeof(t) != n:
             raise AssertionError(f"{t.__name__} is {C.sizeof(t)} bytes, header layout says {n}")

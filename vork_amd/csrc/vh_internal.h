@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <array>
 #include <atomic>
 #include <map>
 #include <memory>
@@ -64,9 +65,21 @@ struct BuildScratch {
 // each other while calls on different streams overlap.
 struct StreamState {
     std::mutex lock;
-    DeviceBuffer paths, rays_a, rays_b, shadow_rays, counters, overflow, scratch_u32;
+    DeviceBuffer paths, rays_a, rays_b, shadow_rays, counters, overflow, scratch_u32, launch_args;
     size_t capacity = 0;  // paths the wavefront buffers hold
     BuildScratch build;
+    // instantiated launch sequences of render batches by shape (hipGraphExec_t),
+    // recorded on `capture` and replayed on the calling stream
+    using GraphKey = std::array<uint32_t, 8>;
+    std::vector<std::pair<GraphKey, void*>> graphs;
+    hipStream_t capture = nullptr;
+    StreamState() = default;
+    StreamState(const StreamState&) = delete;
+    StreamState& operator=(const StreamState&) = delete;
+    ~StreamState() {
+        for (auto& g : graphs) (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(g.second));
+        if (capture) (void)hipStreamDestroy(capture);
+    }
 };
 
 }  // namespace vh

@@ -32,7 +32,7 @@
 namespace vh {
 namespace {
 
-constexpr int kBlock = 256;
+constexpr int kBlock = VH_BLOCK;
 
 // flags word of a path
 constexpr uint32_t kEventShift = 0, kDiffuseShift = 8, kGlossyShift = 12, kBounceShift = 16;
@@ -164,17 +164,52 @@ struct Scene {
 // [3] dropped non-finite contributions, [4..5] extension rays (u64),
 // [6..7] shadow rays (u64), [8] errors, [9] non-finite samples zeroed,
 // [10..11] samples of stopped adaptive pixels (u64), [16..19] extension node /
-// triangle visits, [20..23] shadow node / triangle visits (u64), [24]
-// extension ray claims, [25] shadow ray claims, [28] live count out and [29]
-// pending next events, one u64 so that one atomic reserves both
+// triangle visits of the later bounces, [20..23] shadow node / triangle
+// visits (u64), [24] extension ray claims, [25] shadow ray claims, [28] live
+// count out and [29] pending next events, one u64 so that one atomic reserves
+// both, [32..35] the camera bounce's node / triangle visits and [36..37] its
+// rays (u64)
 constexpr int kExtendClaims = 24, kShadowClaims = 25, kLiveOut = 28, kPending = 29, kShadowB = 2;
+constexpr int kCameraVisits = 32, kCameraRays = 36, kCounterWords = 64;
+
+// Paths a wavefront batch holds at most when the caller leaves the batch size
+// to the library (the queue length of the wavefront).
+#ifndef VH_BATCH_PATHS_LOG2
+#define VH_BATCH_PATHS_LOG2 27
+#endif
+// Workgroups per CU of the persistent and grid-stride launches.
+#ifndef VH_GRID_BLOCKS_PER_CU
+#define VH_GRID_BLOCKS_PER_CU 8
+#endif
+
+// Arguments of one batch's kernels, in device memory.  Every kernel of a batch
+// takes a pointer to this block (and its bounce), so the launch sequence of a
+// batch does not depend on the scene, the buffers or the batch: it is recorded
+// once per stream into a graph and replayed for every batch, the block
+// rewritten before each replay.
+struct LaunchArgs {
+    Paths P;
+    Consts c;
+    Scene S;
+    float4* rays[2];   // extension queues: bounce b traces rays[b & 1], its shading appends to rays[(b + 1) & 1]
+    float4* shadow_a;  // shadow rays of estimator a (slot q = pending event q)
+    float4* shadow_b;  // shadow rays of estimator b
+    uint32_t* counters;
+    uint32_t* overflow;
+    float4* film;
+    float4* guides;    // null: no guides
+    Moment* moments;   // null unless adaptive
+    uint32_t n;        // paths of the batch
+};
+#define VH_ARGS const LaunchArgs* __restrict__
 
 __device__ __forceinline__ void reset_claims(uint32_t* counters) {
     counters[kExtendClaims] = 0;
     counters[kShadowClaims] = 0;
 }
 // Opens an adaptive batch: its camera pass appends to an empty queue.
-__global__ void k_open_batch(uint32_t* counters) {
+__global__ void k_open_batch(VH_ARGS A) {
+    uint32_t* counters = A->counters;
     if (threadIdx.x == 0) {
         counters[0] = 0;
         counters[kLiveOut] = 0;
@@ -188,9 +223,7 @@ __global__ void k_open_batch(uint32_t* counters) {
 // publish this sample's test in its parity slot, then stop if it and the
 // whole clamped 3x3 neighbourhood's test of the previous sample passed.  The
 // neighbours' slot is one no thread writes in this pass.
-__global__ void k_decide(Moment* M, Consts c, uint32_t index) {
-    const uint32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pixel >= c.pixels) return;
+__device__ __forceinline__ void decide_pixel(Moment* M, const Consts& c, uint32_t index, uint32_t pixel) {
     Moment& m = M[pixel];
     if (m.stopped) return;
     const uint32_t slot = index & 1u;
@@ -207,21 +240,20 @@ __global__ void k_decide(Moment* M, Consts c, uint32_t index) {
     }
     if (done) m.stopped = 1u;
 }
+__global__ void k_decide(VH_ARGS A) {
+    const Consts& c = A->c;
+    Moment* M = A->moments;
+    const uint32_t index = c.batch_first;
+    for (uint32_t pixel = blockIdx.x * blockDim.x + threadIdx.x; pixel < c.pixels; pixel += gridDim.x * blockDim.x)
+        decide_pixel(M, c, index, pixel);
+}
+
 
 // Camera rays of the batch's paths.  Plain batches write ray p for path p;
 // adaptive ones append the rays of the pixels that still run.
 template <bool kAdaptive>
-__global__ void k_camera(Paths P, Consts c, Scene S, uint32_t n, float4* rays, uint32_t* counters,
-                         const Moment* M) {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (!kAdaptive && p == 0) {
-        counters[0] = n;
-        counters[kLiveOut] = 0;
-        counters[kShadowB] = 0;
-        counters[kPending] = 0;
-        reset_claims(counters);
-    }
-    if (p >= n) return;
+__device__ __forceinline__ void camera_path(const Paths& P, const Consts& c, const Scene& S, float4* rays,
+                                            uint32_t* counters, const Moment* M, uint32_t p) {
     uint32_t s_local, t;
     slot_sample(p, c.group, c.pixels, s_local, t);
     const uint32_t pixel = slot_to_pixel(t, c.width, c.height);
@@ -249,6 +281,20 @@ __global__ void k_camera(Paths P, Consts c, Scene S, uint32_t n, float4* rays, u
         P.guide_a[p] = make_float4(0.f, 0.f, 0.f, 0.f);
         P.guide_n[p] = make_float4(0.f, 0.f, 0.f, 1.f);
     }
+}
+template <bool kAdaptive>
+__global__ void k_camera(VH_ARGS A) {
+    const uint32_t n = A->n;
+    uint32_t* counters = A->counters;
+    if (!kAdaptive && blockIdx.x == 0 && threadIdx.x == 0) {
+        counters[0] = n;
+        counters[kLiveOut] = 0;
+        counters[kShadowB] = 0;
+        counters[kPending] = 0;
+        reset_claims(counters);
+    }
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x)
+        camera_path<kAdaptive>(A->P, A->c, A->S, A->rays[0], counters, A->moments, p);
 }
 
 // Sums per-thread visit counts of a block into two u64 counters.
@@ -295,18 +341,23 @@ struct ExtendSource {
 };
 
 template <bool kCount>
-VH_TRACE_KERNEL void k_extend(Paths P, Scene S, const float4* rays, uint32_t* counters,
-                                                   uint32_t* overflow) {
+VH_TRACE_KERNEL void k_extend(VH_ARGS A, uint32_t bounce) {
     __shared__ uint64_t lds_stack[kLdsStack * kBlock];
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    Stack st = make_stack(lds_stack, reinterpret_cast<uint64_t*>(overflow));
+    uint32_t* counters = A->counters;
     const uint32_t n = counters[0];
+    if (n == 0) return;  // the paths ended at an earlier bounce
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    Stack st = make_stack(lds_stack, reinterpret_cast<uint64_t*>(A->overflow));
     uint64_t nodes = 0, tris = 0;
-    ExtendSource<kCount> src{P, rays, n, &nodes, &tris};
-    if (!trace_persistent<false, kCount>(S.tv, src, &counters[kExtendClaims], st))
+    const TlasView tv = A->S.tv;
+    ExtendSource<kCount> src{A->P, A->rays[bounce & 1u], n, &nodes, &tris};
+    if (!trace_persistent<false, kCount>(tv, src, &counters[kExtendClaims], st))
         atomicOr(&counters[8], kStackOverflow);
-    if (gid == 0) atomicAdd((unsigned long long*)&counters[4], (unsigned long long)n);
-    flush_counts<kCount>(nodes, tris, (unsigned long long*)&counters[16]);
+    if (gid == 0) {
+        atomicAdd((unsigned long long*)&counters[4], (unsigned long long)n);
+        if (kCount && bounce == 0) atomicAdd((unsigned long long*)&counters[kCameraRays], (unsigned long long)n);
+    }
+    flush_counts<kCount>(nodes, tris, (unsigned long long*)&counters[bounce == 0 ? kCameraVisits : 16]);
 }
 
 // Camera rays whose waves hold the jittered samples of one pixel: a wave
@@ -315,10 +366,20 @@ VH_TRACE_KERNEL void k_extend(Paths P, Scene S, const float4* rays, uint32_t* co
 #ifndef VH_PACKET_CAMERA
 #define VH_PACKET_CAMERA 1
 #endif
-VH_TRACE_KERNEL void k_extend_packet(Paths P, Scene S, const float4* rays, uint32_t* counters) {
+// Waves per SIMD the packet kernel is compiled for: its node and triangle
+// data sit in SGPRs, so it fits 80 VGPRs and 6 waves.
+#ifndef VH_PACKET_WAVES_PER_EU
+#define VH_PACKET_WAVES_PER_EU 6
+#endif
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VH_PACKET_WAVES_PER_EU))) void k_extend_packet(
+    VH_ARGS A) {
     __shared__ uint64_t stacks[kBlock / 64][kPacketStack];
     const uint32_t wave = threadIdx.x >> 6, lane = __lane_id();
     const uint32_t waves = gridDim.x * (kBlock / 64);
+    uint32_t* counters = A->counters;
+    const Paths P = A->P;
+    const Scene S = A->S;
+    const float4* rays = A->rays[0];
     const uint32_t n = counters[0];
     bool ok = true;
     for (uint32_t chunk = blockIdx.x * (kBlock / 64) + wave; (size_t)chunk * 64 < n; chunk += waves) {
@@ -426,9 +487,17 @@ __device__ __forceinline__ f3 path_origin(const Surface& sf, f3 error, f3 direct
 #ifndef VH_SHADE_WAVES_PER_EU
 #define VH_SHADE_WAVES_PER_EU 4
 #endif
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VH_SHADE_WAVES_PER_EU))) void k_shade(Paths P, Consts c, Scene S, const float4* rays, float4* next,
-                                                  float4* shadow_rays, float4* shadow_b, uint32_t* counters) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VH_SHADE_WAVES_PER_EU))) void k_shade(
+    VH_ARGS A, uint32_t bounce_parity) {
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, threads = gridDim.x * blockDim.x;
+    uint32_t* counters = A->counters;
+    const Paths& P = A->P;
+    const Consts& c = A->c;
+    const Scene& S = A->S;
+    const float4* rays = A->rays[bounce_parity & 1u];
+    float4* next = A->rays[(bounce_parity + 1u) & 1u];
+    float4* shadow_rays = A->shadow_a;
+    float4* shadow_b = A->shadow_b;
     const uint32_t n = counters[0];
     uint32_t dropped = 0;
     uint32_t i = gid;
@@ -660,15 +729,17 @@ struct ShadowSource {
 };
 
 template <bool kCount>
-VH_ANY_KERNEL void k_shadow(Paths P, Scene S, const float4* rays, const float4* rays_b, uint32_t* counters,
-                                                   uint32_t* overflow) {
+VH_ANY_KERNEL void k_shadow(VH_ARGS A) {
     __shared__ uint64_t lds_stack[kLdsStack * kBlock];
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    Stack st = make_stack(lds_stack, reinterpret_cast<uint64_t*>(overflow));
+    uint32_t* counters = A->counters;
     const uint32_t na = counters[kPending], n = na + counters[kShadowB];
+    if (n == 0) return;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    Stack st = make_stack(lds_stack, reinterpret_cast<uint64_t*>(A->overflow));
     uint64_t nodes = 0, tris = 0;
-    ShadowSource<kCount> src{P, rays, rays_b, na, n, &nodes, &tris};
-    if (!trace_persistent<true, kCount>(S.tv, src, &counters[kShadowClaims], st))
+    const TlasView tv = A->S.tv;
+    ShadowSource<kCount> src{A->P, A->shadow_a, A->shadow_b, na, n, &nodes, &tris};
+    if (!trace_persistent<true, kCount>(tv, src, &counters[kShadowClaims], st))
         atomicOr(&counters[8], kStackOverflow);
     if (gid == 0) atomicAdd((unsigned long long*)&counters[6], (unsigned long long)n);
     flush_counts<kCount>(nodes, tris, (unsigned long long*)&counters[20]);
@@ -676,7 +747,9 @@ VH_ANY_KERNEL void k_shadow(Paths P, Scene S, const float4* rays, const float4* 
 
 // Adds each pending next event: the clamped sum of its unoccluded estimators
 // (sample_clamp.slang:7-16 over path.slang:925-950's value).
-__global__ void k_resolve(Paths P, uint32_t* counters) {
+__global__ void k_resolve(VH_ARGS A) {
+    uint32_t* counters = A->counters;
+    const Paths& P = A->P;
     const uint32_t n = counters[kPending];
     uint32_t dropped = 0;
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
@@ -696,7 +769,7 @@ __global__ void k_resolve(Paths P, uint32_t* counters) {
 // film (resolve.slang:228-238).  A sample is (radiance, 1), or zero when it
 // was non-finite (path.slang:1948-1952).  When a pixel's samples sit in
 // groups of 8 or more consecutive path slots, the block stages 8 samples of
-// its 256 pixels at a time in LDS with loads that read whole cache lines (a
+// its block of pixels at a time in LDS with loads that read whole cache lines (a
 // thread per pixel would read one 16-byte value per line per sample); the
 // per-pixel sums keep sample order.  Adaptive batches also update the pixel's moment, add a stopped
 // pixel's mean in place of a sample, and record the settled test of sample
@@ -707,11 +780,9 @@ __global__ void k_resolve(Paths P, uint32_t* counters) {
 // sums over every sample (path.slang:1806-1809).
 constexpr uint32_t kFilmStage = 8, kFilmPad = kFilmStage + 1;
 template <bool kAdaptive>
-__global__ __launch_bounds__(kBlock) void k_film(Paths P, Consts c, float4* film, float4* guides, Moment* M,
-                                                 uint32_t* counters) {
-    __shared__ float4 s_rad[kBlock * kFilmPad];
-    __shared__ uint32_t s_flags[kBlock * kFilmStage];
-    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void film_tile(const Paths& P, const Consts& c, float4* film, float4* guides, Moment* M,
+                                          uint32_t* counters, uint32_t tile, float4* s_rad, uint32_t* s_flags) {
+    const uint32_t slot = tile * kBlock + threadIdx.x;
     const bool valid = slot < c.pixels;
     const bool staged = c.group >= kFilmStage;  // the same for the whole launch
     const uint32_t pixel = valid ? slot_to_pixel(slot, c.width, c.height) : 0u;
@@ -730,7 +801,7 @@ __global__ __launch_bounds__(kBlock) void k_film(Paths P, Consts c, float4* film
     for (uint32_t s = 0; s < c.batch_samples; s++) {
         if (staged && s % kFilmStage == 0) {
             __syncthreads();
-            const uint32_t t0 = blockIdx.x * blockDim.x;
+            const uint32_t t0 = tile * kBlock;
             for (uint32_t j = 0; j < kFilmStage; j++) {
                 const uint32_t e = threadIdx.x + kBlock * j, ts = t0 + e / kFilmStage, ss = s + e % kFilmStage;
                 if (ts < c.pixels && ss < c.batch_samples) {
@@ -792,8 +863,18 @@ __global__ __launch_bounds__(kBlock) void k_film(Paths P, Consts c, float4* film
     if (bad_count) atomicAdd(&counters[9], bad_count);
     if (stopped_count) atomicAdd((unsigned long long*)&counters[10], (unsigned long long)stopped_count);
 }
+// Tiles of kBlock pixel slots, strided over the grid (the barriers of a tile
+// are uniform: every thread of a block walks the same tiles).
+template <bool kAdaptive>
+__global__ __launch_bounds__(kBlock) void k_film(VH_ARGS A) {
+    __shared__ float4 s_rad[kBlock * kFilmPad];
+    __shared__ uint32_t s_flags[kBlock * kFilmStage];
+    for (uint32_t tile = blockIdx.x; tile * kBlock < A->c.pixels; tile += gridDim.x)
+        film_tile<kAdaptive>(A->P, A->c, A->film, A->guides, A->moments, A->counters, tile, s_rad, s_flags);
+}
 
-__global__ void k_swap_counts(uint32_t* counters) {
+__global__ void k_swap_counts(VH_ARGS A) {
+    uint32_t* counters = A->counters;
     if (threadIdx.x == 0) {
         counters[0] = counters[kLiveOut];
         counters[kLiveOut] = 0;
@@ -880,6 +961,16 @@ __global__ void k_bake_dielectric(float* out) {
 
 inline uint32_t blocks(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
 
+// Writes one batch's launch block (a kernel, so the write is ordered on the
+// stream with the batch before and after it, whatever the host does next).
+__global__ void k_store_args(LaunchArgs* dst, LaunchArgs value) {
+    constexpr uint32_t words = sizeof(LaunchArgs) / 4;
+    static_assert(sizeof(LaunchArgs) % 4 == 0, "launch block in words");
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(&value);
+    uint32_t* out = reinterpret_cast<uint32_t*>(dst);
+    for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) out[i] = src[i];
+}
+
 // Sampler draws for a list of (px, py, index, group) queries, through the
 // same group_bits / path_random the path kernels call.
 __global__ void k_sampler_bits(const uint4* queries, uint32_t n, uint32_t samples, uint32_t seed, uint4* out) {
@@ -925,6 +1016,91 @@ int bake_luts(vh_device* dev) {
 
 int ensure_overflow(DeviceBuffer& buf, uint32_t threads);
 
+namespace {
+// The launch sequence of one batch, fixed by these values (its kernels read
+// everything else from the launch block).
+struct BatchShape {
+    uint32_t bounces = 0, grid = 0;
+    uint32_t paths = 0, pixels = 0;  // one thread per path (camera) and per pixel (decide, film)
+    uint32_t mode = 0;  // 0 plain, 1 adaptive up to `minimum`, 2 adaptive after a decide pass
+    bool count = false, packet = false;
+    StreamState::GraphKey key() const {
+        return {bounces, grid, paths, pixels, mode, count ? 1u : 0u, packet ? 1u : 0u, 0u};
+    }
+};
+// Graphs a stream keeps (the oldest goes first): renders of a few image sizes
+// and batch layouts in turn each find theirs.
+constexpr size_t kGraphsPerStream = 8;
+
+// Issues one batch's kernels on `s`.  `mark(stage, begin)` is called around
+// the extend (0; 3 for the camera bounce), shade (1) and shadow (2) launches.
+template <class Mark>
+int launch_batch(const BatchShape& b, const LaunchArgs* A, hipStream_t s, Mark&& mark) {
+    const dim3 grid(b.grid), block(kBlock), one(1), wave(64);
+    const dim3 per_path(blocks(b.paths, kBlock)), per_pixel(blocks(b.pixels, kBlock));
+    if (b.mode == 2) hipLaunchKernelGGL(k_decide, per_pixel, block, 0, s, A);
+    if (b.mode != 0) {
+        hipLaunchKernelGGL(k_open_batch, one, wave, 0, s, A);
+        hipLaunchKernelGGL(k_camera<true>, per_path, block, 0, s, A);
+    } else {
+        hipLaunchKernelGGL(k_camera<false>, per_path, block, 0, s, A);
+    }
+    int rc;
+    for (uint32_t bounce = 0; bounce < b.bounces; bounce++) {
+        const int stage = bounce == 0 ? 3 : 0;
+        if ((rc = mark(stage, true))) return rc;
+        if (b.count) hipLaunchKernelGGL(k_extend<true>, grid, block, 0, s, A, bounce);
+        else if (b.packet && bounce == 0) hipLaunchKernelGGL(k_extend_packet, grid, block, 0, s, A);
+        else hipLaunchKernelGGL(k_extend<false>, grid, block, 0, s, A, bounce);
+        if ((rc = mark(stage, false))) return rc;
+        if ((rc = mark(1, true))) return rc;
+        hipLaunchKernelGGL(k_shade, grid, block, 0, s, A, bounce);
+        if ((rc = mark(1, false))) return rc;
+        if ((rc = mark(2, true))) return rc;
+        if (b.count) hipLaunchKernelGGL(k_shadow<true>, grid, block, 0, s, A);
+        else hipLaunchKernelGGL(k_shadow<false>, grid, block, 0, s, A);
+        if ((rc = mark(2, false))) return rc;
+        hipLaunchKernelGGL(k_resolve, grid, block, 0, s, A);
+        hipLaunchKernelGGL(k_swap_counts, one, wave, 0, s, A);
+    }
+    if (b.mode != 0) hipLaunchKernelGGL(k_film<true>, per_pixel, block, 0, s, A);
+    else hipLaunchKernelGGL(k_film<false>, per_pixel, block, 0, s, A);
+    VH_HIP(hipGetLastError());
+    return VH_OK;
+}
+
+// Replays the stream's graph of this batch shape on `s`, recording it first
+// on the stream's private capture stream if the stream has none yet.
+int launch_batch_graph(StreamState* st, const BatchShape& b, const LaunchArgs* A, hipStream_t s) {
+    const StreamState::GraphKey key = b.key();
+    hipGraphExec_t exec = nullptr;
+    for (const auto& g : st->graphs)
+        if (g.first == key) exec = static_cast<hipGraphExec_t>(g.second);
+    if (!exec) {
+        if (st->graphs.size() >= kGraphsPerStream) {
+            (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(st->graphs.front().second));
+            st->graphs.erase(st->graphs.begin());
+        }
+        if (!st->capture) VH_HIP(hipStreamCreateWithFlags(&st->capture, hipStreamNonBlocking));
+        VH_HIP(hipStreamBeginCapture(st->capture, hipStreamCaptureModeRelaxed));
+        int rc = launch_batch(b, A, st->capture, [](int, bool) { return VH_OK; });
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(st->capture, &graph);
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+        st->graphs.emplace_back(key, exec);
+    }
+    VH_HIP(hipGraphLaunch(exec, s));
+    return VH_OK;
+}
+}  // namespace
+
 int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, float* guides, hipStream_t s) {
     vh_device* dev = sc->device;
     const uint32_t pixels = rp->width * rp->height;
@@ -933,11 +1109,11 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, float* g
     if (total == 0) return VH_OK;
     uint32_t batch = rp->batch_samples;
     if (batch == 0) {
-        // Up to 2^27 paths in flight (262 B each, about 35 GB: a whole
-        // 1920x1080 x 64-spp exposure), whole exposure samples per batch.
-        // Every wavefront launch pays a tail while its slowest rays finish,
-        // so fewer, longer launches are faster.
-        batch = std::max<uint32_t>(1u, (1u << 27) / std::max<uint32_t>(pixels, 1u));
+        // Up to 2^VH_BATCH_PATHS_LOG2 paths in flight (262 B each; 2^27 is
+        // about 35 GB, a whole 1920x1080 x 64-spp exposure), whole exposure
+        // samples per batch.  Every wavefront launch pays a tail while its
+        // slowest rays finish, so fewer, longer launches are faster.
+        batch = std::max<uint32_t>(1u, (1u << VH_BATCH_PATHS_LOG2) / std::max<uint32_t>(pixels, 1u));
     }
     batch = std::min(batch, total);
     if (pixels > kPathMask) return fail(VH_ERR_USAGE, "vh_render: more than 2^30 - 1 pixels");
@@ -1025,28 +1201,33 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, float* g
     c.guides = guides != nullptr;
     c.camera = sc->camera;
     c.sky = sc->sky;
-    const uint32_t grid = std::min<uint32_t>(blocks(cap, kBlock), (uint32_t)dev->cu_count * 8u);
+    const uint32_t grid = std::min<uint32_t>(blocks(cap, kBlock), (uint32_t)dev->cu_count * VH_GRID_BLOCKS_PER_CU);
     int rc = ensure_overflow(st->overflow, grid * kBlock);
     if (rc) return rc;
+    VH_HIP(st->launch_args.reserve(sizeof(LaunchArgs)));
     uint32_t* counters = st->counters.as<uint32_t>();
-    uint32_t* overflow = st->overflow.as<uint32_t>();
-    VH_HIP(hipMemsetAsync(counters, 0, 64 * sizeof(uint32_t), s));
+    VH_HIP(hipMemsetAsync(counters, 0, kCounterWords * sizeof(uint32_t), s));
     const bool profile = (rp->flags & VH_RENDER_PROFILE) != 0;
     const bool count = (rp->flags & VH_RENDER_COUNT_VISITS) != 0;
+    LaunchArgs args;
+    std::memset(&args, 0, sizeof(args));
+    args.P = P;
+    args.S = S;
+    args.rays[0] = st->rays_a.as<float4>();
+    args.rays[1] = st->rays_b.as<float4>();
+    args.shadow_a = st->shadow_rays.as<float4>();
+    args.shadow_b = args.shadow_a + 2 * cap;
+    args.counters = counters;
+    args.overflow = st->overflow.as<uint32_t>();
+    args.film = reinterpret_cast<float4*>(film);
+    args.guides = reinterpret_cast<float4*>(guides);
+    args.moments = moments;
+    const LaunchArgs* dargs = st->launch_args.as<LaunchArgs>();
     // stage events: (stage, start, stop) triples recorded around each launch
+    // when profiling (stage 3: the camera bounce's extend launch, also
+    // counted in stage 0)
     struct Mark { int stage; hipEvent_t a, b; };
     std::vector<Mark> marks;
-    auto timed = [&](int stage, auto&& launch) -> int {
-        if (!profile) { launch(); return VH_OK; }
-        Mark m{stage, nullptr, nullptr};
-        VH_HIP(hipEventCreate(&m.a));
-        VH_HIP(hipEventCreate(&m.b));
-        VH_HIP(hipEventRecord(m.a, s));
-        launch();
-        VH_HIP(hipEventRecord(m.b, s));
-        marks.push_back(m);
-        return VH_OK;
-    };
     hipEvent_t e0, e1;
     VH_HIP(hipEventCreate(&e0));
     VH_HIP(hipEventCreate(&e1));
@@ -1061,72 +1242,73 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, float* g
         c.batch_first = first;
         c.batch_samples = nb;
         c.group = sample_group(nb);
-        const uint32_t n = pixels * nb;
-        float4* qa = st->rays_a.as<float4>();
-        float4* qb = st->rays_b.as<float4>();
-        float4* sq = st->shadow_rays.as<float4>();
-        float4* sqb = sq + 2 * cap;
-        if (adaptive) {
-            if (c.decide) hipLaunchKernelGGL(k_decide, dim3(blocks(pixels, kBlock)), dim3(kBlock), 0, s, moments, c, first);
-            hipLaunchKernelGGL(k_open_batch, dim3(1), dim3(64), 0, s, counters);
-            hipLaunchKernelGGL(k_camera<true>, dim3(blocks(n, kBlock)), dim3(kBlock), 0, s, P, c, S, n, qa, counters,
-                               moments);
+        args.c = c;
+        args.n = pixels * nb;
+        hipLaunchKernelGGL(k_store_args, dim3(1), dim3(kBlock), 0, s, const_cast<LaunchArgs*>(dargs), args);
+        BatchShape shape;
+        shape.bounces = c.max_bounces + 1;
+        shape.grid = grid;
+        shape.paths = args.n;
+        shape.pixels = pixels;
+        shape.mode = !adaptive ? 0u : c.decide ? 2u : 1u;
+        shape.count = count;
+        shape.packet = VH_PACKET_CAMERA && !count && c.group >= 16;
+        if (profile) {
+            // launched one by one, with events around the stage launches
+            rc = launch_batch(shape, dargs, s, [&](int stage, bool begin) -> int {
+                if (begin) {
+                    marks.push_back(Mark{stage, nullptr, nullptr});
+                    VH_HIP(hipEventCreate(&marks.back().a));
+                    VH_HIP(hipEventCreate(&marks.back().b));
+                    VH_HIP(hipEventRecord(marks.back().a, s));
+                } else {
+                    VH_HIP(hipEventRecord(marks.back().b, s));
+                }
+                return VH_OK;
+            });
         } else {
-            hipLaunchKernelGGL(k_camera<false>, dim3(blocks(n, kBlock)), dim3(kBlock), 0, s, P, c, S, n, qa, counters,
-                               moments);
+            rc = launch_batch_graph(st, shape, dargs, s);
         }
-        for (uint32_t bounce = 0; bounce <= c.max_bounces; bounce++) {
-            int rc2 = timed(0, [&] {
-                if (count) hipLaunchKernelGGL(k_extend<true>, dim3(grid), dim3(kBlock), 0, s, P, S, qa, counters, overflow);
-                else if (VH_PACKET_CAMERA && bounce == 0 && c.group >= 16)
-                    hipLaunchKernelGGL(k_extend_packet, dim3(grid), dim3(kBlock), 0, s, P, S, qa, counters);
-                else hipLaunchKernelGGL(k_extend<false>, dim3(grid), dim3(kBlock), 0, s, P, S, qa, counters, overflow);
-            });
-            if (!rc2) rc2 = timed(1, [&] {
-                hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kBlock), 0, s, P, c, S, qa, qb, sq, sqb, counters);
-            });
-            if (!rc2) rc2 = timed(2, [&] {
-                if (count) hipLaunchKernelGGL(k_shadow<true>, dim3(grid), dim3(kBlock), 0, s, P, S, sq, sqb, counters, overflow);
-                else hipLaunchKernelGGL(k_shadow<false>, dim3(grid), dim3(kBlock), 0, s, P, S, sq, sqb, counters, overflow);
-            });
-            if (rc2) return rc2;
-            hipLaunchKernelGGL(k_resolve, dim3(grid), dim3(kBlock), 0, s, P, counters);
-            hipLaunchKernelGGL(k_swap_counts, dim3(1), dim3(64), 0, s, counters);
-            std::swap(qa, qb);
-        }
-        float4* gf = reinterpret_cast<float4*>(guides);
-        if (adaptive)
-            hipLaunchKernelGGL(k_film<true>, dim3(blocks(pixels, kBlock)), dim3(kBlock), 0, s, P, c,
-                               reinterpret_cast<float4*>(film), gf, moments, counters);
-        else
-            hipLaunchKernelGGL(k_film<false>, dim3(blocks(pixels, kBlock)), dim3(kBlock), 0, s, P, c,
-                               reinterpret_cast<float4*>(film), gf, moments, counters);
+        if (rc) return rc;
         VH_HIP(hipGetLastError());
         first += nb;
     }
     VH_HIP(hipEventRecord(e1, s));
-    uint32_t h[24];
+    uint32_t h[kCounterWords];
     VH_HIP(hipMemcpyAsync(h, counters, sizeof(h), hipMemcpyDeviceToHost, s));
     VH_HIP(hipStreamSynchronize(s));
     float ms = 0;
     (void)hipEventElapsedTime(&ms, e0, e1);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    double stage_ms[3] = {0, 0, 0};
-    uint64_t stage_n[3] = {0, 0, 0};
+    double stage_ms[4] = {0, 0, 0, 0};
+    uint64_t stage_n[4] = {0, 0, 0, 0};
     for (const Mark& m : marks) {
         float t = 0;
         (void)hipEventElapsedTime(&t, m.a, m.b);
         stage_ms[m.stage] += t;
         stage_n[m.stage]++;
+        if (m.stage == 3) {  // the camera bounce is part of the extend stage
+            stage_ms[0] += t;
+            stage_n[0]++;
+        }
         (void)hipEventDestroy(m.a);
         (void)hipEventDestroy(m.b);
     }
-    uint64_t ext, shd, stopped, v[4];
+    uint64_t ext, shd, stopped, v[4], cv[2], crays;
     std::memcpy(&stopped, &h[10], 8);
     std::memcpy(&ext, &h[4], 8);
     std::memcpy(&shd, &h[6], 8);
     std::memcpy(v, &h[16], 32);
+    std::memcpy(cv, &h[kCameraVisits], 16);
+    std::memcpy(&crays, &h[kCameraRays], 8);
+    v[0] += cv[0];
+    v[1] += cv[1];
+    sc->stats.extend_camera_ms = stage_ms[3];
+    sc->stats.extend_camera_launches = stage_n[3];
+    sc->stats.extend_camera_rays = crays;
+    sc->stats.extend_camera_node_visits = cv[0];
+    sc->stats.extend_camera_triangle_tests = cv[1];
     sc->stats.extend_ms = stage_ms[0];
     sc->stats.shade_ms = stage_ms[1];
     sc->stats.shadow_ms = stage_ms[2];

@@ -6,7 +6,7 @@
 namespace vh {
 namespace {
 
-constexpr int kTraceBlock = 256;
+constexpr int kTraceBlock = VH_BLOCK;
 
 // Rays from a vh_ray array; results to vh_hit / occlusion bytes / visit counts
 // (nodes, triangles, deepest stack entry, 0).

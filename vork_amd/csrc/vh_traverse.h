@@ -57,8 +57,12 @@ constexpr uint32_t kItemGroup = 0x80000000u;
 #ifndef VH_ANY_WAVES_PER_EU
 #define VH_ANY_WAVES_PER_EU 6
 #endif
-#define VH_TRACE_KERNEL __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VH_TRACE_WAVES_PER_EU)))
-#define VH_ANY_KERNEL __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VH_ANY_WAVES_PER_EU)))
+// Threads per workgroup of the trace and wavefront kernels.
+#ifndef VH_BLOCK
+#define VH_BLOCK 256
+#endif
+#define VH_TRACE_KERNEL __global__ __launch_bounds__(VH_BLOCK) __attribute__((amdgpu_waves_per_eu(VH_TRACE_WAVES_PER_EU)))
+#define VH_ANY_KERNEL __global__ __launch_bounds__(VH_BLOCK) __attribute__((amdgpu_waves_per_eu(VH_ANY_WAVES_PER_EU)))
 constexpr uint32_t kRefillMin = VH_TRACE_REFILL_MIN;
 constexpr int kItemsPerStep = VH_TRACE_ITEMS_PER_STEP;
 

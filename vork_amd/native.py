@@ -19,7 +19,9 @@ import numpy as np
 from . import abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libvork_hip.so")
+# VORK_HIP_LIB names another build of the library (the autotuner measures its
+# variants this way); the default is the in-tree build.
+LIB_PATH = os.environ.get("VORK_HIP_LIB") or os.path.join(HERE, "libvork_hip.so")
 
 EXPORTS = [
     "vh_abi_version", "vh_last_error", "vh_device_create", "vh_device_release", "vh_device_synchronize",

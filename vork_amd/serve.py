@@ -31,7 +31,14 @@ PATH_BYTES = 262                   # path state 134 + extension queues 2x32 + sh
 FILM_BYTES = 16                    # RGBA32F per pixel
 MOMENT_BYTES = 48                  # adaptive moment per pixel
 DENOISE_BYTES = 32 + 16 + 1100     # guides, denoised output, U-Net activations (bf16, all levels) per pixel
-BATCH_PATHS = 1 << 27              # paths per wavefront batch the library aims for
+def _tuned(knob):
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")) as f:
+        return json.load(f)["knobs"][knob]
+
+
+BATCH_PATHS = 1 << _tuned("VH_BATCH_PATHS_LOG2")   # paths per wavefront batch the library aims for
 MAX_BATCH_PATHS = (1 << 30) - 1    # paths of one batch (30-bit path ids)
 
 
