@@ -78,6 +78,8 @@ def main():
     ap.add_argument("--birds", type=int, default=1_000_000)
     ap.add_argument("--frames", type=int, default=4, help="config2: distinct animation frames cycled by the steps")
     ap.add_argument("--batch", type=int, default=0, help="exposure samples per wavefront batch (0 = auto)")
+    ap.add_argument("--shade", choices=["default", "sorted", "arrival"], default="default",
+                    help="shading order: hits sorted into per-closure queues, or as traced (default: the tuned one)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-traffic", action="store_true",
@@ -166,10 +168,12 @@ def main():
     frame_no = [0]
     tlas_ms = []
 
+    shade_flag = {"default": 0, "sorted": abi.VH_RENDER_SHADE_SORTED, "arrival": abi.VH_RENDER_SHADE_ARRIVAL}[args.shade]
+
     def step(flags=0):
         p = sc.render_params(adaptive=False)
         p.batch_samples = args.batch
-        p.flags = flags
+        p.flags = flags | shade_flag
         if gpu_frames is None:
             blases = build_blases()
             tlas = native.Tlas(dev, blases, instances)

@@ -343,6 +343,12 @@ typedef struct vh_render_params {
 #define VH_RENDER_PROFILE 1u
 #define VH_RENDER_COUNT_VISITS 2u
 #define VH_RENDER_ADAPTIVE 4u
+/* Shading order (the film is the same either way): SORTED sorts each bounce's
+ * hits into queues by closure class (diffuse, dielectric GGX, metal GGX) so a
+ * wave shades one closure; ARRIVAL shades the queue as traced.  Neither: the
+ * library's tuned default. */
+#define VH_RENDER_SHADE_SORTED 8u
+#define VH_RENDER_SHADE_ARRIVAL 16u
 
 /* Fills `params` with the reference's production defaults for a
  * width x height x samples render (adaptive sampling on). */

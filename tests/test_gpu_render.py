@@ -311,3 +311,29 @@ def test_graph_replays_equal_direct_launches(renderer):
             x = torch.randn(1 << 20, device="cuda")
             torch.cuda.synchronize()
             assert float((x * 0).sum()) == 0.0
+
+
+@pytest.mark.parametrize("make", [lambda: S.config0(width=48, height=40),
+                                  lambda: S.small_config1(width=64, height=36),
+                                  lambda: S.config2(width=48, height=27, spp=4, birds=3000)])
+def test_material_sorted_shading_gives_the_same_film(renderer, make):
+    """Hits sorted into per-closure queues (diffuse, dielectric GGX, metal
+    GGX; misses shaded by the classify pass) shade every path exactly as the
+    arrival-order queue does: the films are bit-identical, with guides, with
+    several batches and adaptively."""
+    from vork_amd import abi
+    sc = make()
+    inst = renderer.install(sc)
+    for batch, adaptive in ((0, False), (2, False), (0, True)):
+        films = []
+        for order in (abi.VH_RENDER_SHADE_ARRIVAL, abi.VH_RENDER_SHADE_SORTED):
+            p = sc.render_params(samples=12, seed=4, adaptive=adaptive)
+            p.batch_samples = batch
+            p.flags |= order
+            if adaptive:
+                p.adaptive_threshold, p.adaptive_min_samples = 0.1, 4
+            films.append(inst.render_guides(p))
+            stats = inst.stats()
+            assert stats["dropped_samples"] == 0 or batch != 0
+        assert np.array_equal(films[0][0], films[1][0]), (batch, adaptive)
+        assert np.array_equal(films[0][1], films[1][1]), (batch, adaptive)

@@ -103,6 +103,7 @@ class RenderStats(C.Structure):
 
 
 VH_RENDER_PROFILE, VH_RENDER_COUNT_VISITS, VH_RENDER_ADAPTIVE = 1, 2, 4
+VH_RENDER_SHADE_SORTED, VH_RENDER_SHADE_ARRIVAL = 8, 16
 
 
 def default_params(width, height, samples, seed=1, max_bounces=12, adaptive=True):

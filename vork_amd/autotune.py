@@ -4,7 +4,8 @@ The kernels' launch shapes and queue sizes are compile-time constants of the
 HIP sources (workgroup size, waves per SIMD each kernel is compiled for, how
 many idle lanes a traversal wave waits for before refilling, how many of a
 pixel's samples share a wave, persistent-grid workgroups per CU, wavefront
-batch length, BVH leaf size and clustering radius).  This module searches them
+batch length, BVH leaf size and clustering radius, and whether hits are
+sorted into per-closure shading queues).  This module searches them
 by measurement and records the winners in vork_amd/tuning.json, which
 vork_amd/build.py compiles in.
 
@@ -53,7 +54,10 @@ KNOBS = {
     "VH_BATCH_PATHS_LOG2": [25, 26, 27],
     "VH_LEAF_MAX": [1, 2, 4],
     "VH_PLOC_RADIUS": [8, 16, 24],
+    "VH_SHADE_SORT": [0, 1],
 }
+# knobs swept when none are named (VH_SHADE_SORT is measured by bench.py --shade instead)
+DEFAULT_SWEEP = [k for k in KNOBS if k != "VH_SHADE_SORT"]
 # the bench command every variant is timed with
 BENCH = ["--steps", "4", "--warmup", "2", "--no-cpu-baseline", "--no-roofline", "--no-traffic"]
 NOISE = 0.005   # relative: differences below this are run-to-run noise on this workload
@@ -86,7 +90,7 @@ def build_variant(values):
 def cmd_build(args):
     table = load_table()
     base = dict(table["knobs"])
-    knobs = args.knobs.split(",") if args.knobs else list(KNOBS)
+    knobs = args.knobs.split(",") if args.knobs else DEFAULT_SWEEP
     plan = {"baseline": {"values": base, "lib": None}, "variants": []}
     plan["baseline"]["lib"] = build_variant(base)
     if args.from_results:

@@ -65,7 +65,7 @@ struct BuildScratch {
 // each other while calls on different streams overlap.
 struct StreamState {
     std::mutex lock;
-    DeviceBuffer paths, rays_a, rays_b, shadow_rays, counters, overflow, scratch_u32, launch_args;
+    DeviceBuffer paths, rays_a, rays_b, shadow_rays, counters, overflow, scratch_u32, launch_args, sorted;
     size_t capacity = 0;  // paths the wavefront buffers hold
     BuildScratch build;
     // instantiated launch sequences of render batches by shape (hipGraphExec_t),

@@ -171,6 +171,19 @@ struct Scene {
 // rays (u64)
 constexpr int kExtendClaims = 24, kShadowClaims = 25, kLiveOut = 28, kPending = 29, kShadowB = 2;
 constexpr int kCameraVisits = 32, kCameraRays = 36, kCounterWords = 64;
+// [40..42] hits per closure class in the sorted shading queues
+constexpr int kClassCount = 40;
+
+// Material-sorted shading: a classify pass adds the sky for every miss and
+// appends each hit's queue entry to the queue of its closure class (diffuse
+// only, dielectric GGX, metal GGX), so each wave of the shading kernel runs
+// one closure's code.  Otherwise the shading kernel takes the queue in
+// arrival order and walks past misses.  This is the default when a render's
+// flags choose neither VH_RENDER_SHADE_SORTED nor VH_RENDER_SHADE_ARRIVAL.
+#ifndef VH_SHADE_SORT
+#define VH_SHADE_SORT 0
+#endif
+constexpr uint32_t kClasses = 3;
 
 // Paths a wavefront batch holds at most when the caller leaves the batch size
 // to the library (the queue length of the wavefront).
@@ -199,6 +212,8 @@ struct LaunchArgs {
     float4* film;
     float4* guides;    // null: no guides
     Moment* moments;   // null unless adaptive
+    uint32_t* sorted;  // sorted shading: kClasses queues of `cap` extension-queue entries; null: arrival order
+    uint32_t cap;      // paths the wavefront buffers hold
     uint32_t n;        // paths of the batch
 };
 #define VH_ARGS const LaunchArgs* __restrict__
@@ -210,6 +225,7 @@ __device__ __forceinline__ void reset_claims(uint32_t* counters) {
 // Opens an adaptive batch: its camera pass appends to an empty queue.
 __global__ void k_open_batch(VH_ARGS A) {
     uint32_t* counters = A->counters;
+    if (threadIdx.x < kClasses) counters[kClassCount + threadIdx.x] = 0;
     if (threadIdx.x == 0) {
         counters[0] = 0;
         counters[kLiveOut] = 0;
@@ -286,6 +302,7 @@ template <bool kAdaptive>
 __global__ void k_camera(VH_ARGS A) {
     const uint32_t n = A->n;
     uint32_t* counters = A->counters;
+    if (!kAdaptive && blockIdx.x == 0 && threadIdx.x < kClasses) counters[kClassCount + threadIdx.x] = 0;
     if (!kAdaptive && blockIdx.x == 0 && threadIdx.x == 0) {
         counters[0] = n;
         counters[kLiveOut] = 0;
@@ -481,6 +498,61 @@ __device__ __forceinline__ f3 path_origin(const Surface& sf, f3 error, f3 direct
     return surface_ray_origin(sf.world, error, reflection ? sf.smooth : sf.face, direction);
 }
 
+// Closure class of a material for the sorted shading queues: metal GGX
+// (the F82 lobe alone), dielectric GGX over a diffuse base, diffuse alone.
+__device__ __forceinline__ uint32_t closure_class(const vh_material& m) {
+    if (m.base_metalness >= 1.f) return 2u;
+    return m.specular_weight > 0.f ? 1u : 0u;
+}
+
+// Misses add the sky; hits go to the queue of their closure class, one
+// atomic per class and wave (ballot compaction).
+__global__ __launch_bounds__(kBlock) void k_classify(VH_ARGS A, uint32_t bounce_parity) {
+    uint32_t* counters = A->counters;
+    const Paths& P = A->P;
+    const Consts& c = A->c;
+    const Scene& S = A->S;
+    const float4* rays = A->rays[bounce_parity & 1u];
+    const uint32_t n = counters[0];
+    const uint64_t lt = (1ull << __lane_id()) - 1ull;
+    uint32_t dropped = 0;
+    // whole waves walk the queue together so that the ballots see every lane
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n; base += stride) {
+        const uint32_t i = base + __lane_id();
+        uint32_t cls = kClasses;  // none: a miss or past the end
+        if (i < n) {
+            const float4 rd4 = rays[2 * i + 1];
+            const uint32_t p = f2u(rd4.w) & kPathMask;
+            const float4 h = P.hit[p];
+            if (h.x >= 0.f) {
+                const InstanceRecord& ir = S.tv.instances[P.hit_inst[p]];
+                const BlasRecord& br = S.tv.blases[ir.blas];
+                const uint32_t local = f2u(br.surf[kSurfFloats * (size_t)f2u(h.w) + 18]);
+                const uint32_t m = ir.material + local;
+                cls = closure_class(S.materials[m < c.material_count ? m : c.material_count - 1]);
+            } else {
+                const uint4 info = P.info[p];
+                const float4 t4 = P.thr[p], r4 = P.rad[p];
+                const float limit = ((info.w >> kBounceShift) & 0xffu) == 0 ? c.clamp_direct : c.clamp_indirect;
+                const f3 sky = clamp_sample(mk(t4.x, t4.y, t4.z) * sky_radiance(c.sky, mk(rd4.x, rd4.y, rd4.z)),
+                                            limit, dropped);
+                P.rad[p] = make_float4(r4.x + sky.x, r4.y + sky.y, r4.z + sky.z, r4.w);
+            }
+        }
+        for (uint32_t k = 0; k < kClasses; k++) {
+            const uint64_t m = __ballot(cls == k);
+            if (!m) continue;
+            const int leader = __ffsll((unsigned long long)m) - 1;
+            uint32_t slot = 0;
+            if ((int)__lane_id() == leader) slot = atomicAdd(&counters[kClassCount + k], (uint32_t)__popcll(m));
+            slot = __shfl(slot, leader);
+            if (cls == k) A->sorted[(size_t)k * A->cap + slot + (uint32_t)__popcll(m & lt)] = i;
+        }
+    }
+    if (dropped) atomicAdd(&counters[3], dropped);
+}
+
 // Waves per SIMD k_shade is compiled for (bounds its VGPRs to 512 / waves).
 // Its time goes to waiting on the chain of scene loads, so it runs at the
 // most waves that keep its spills small (4: 20 B per lane; 5 spills 164 B).
@@ -498,7 +570,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VH_SHADE
     float4* next = A->rays[(bounce_parity + 1u) & 1u];
     float4* shadow_rays = A->shadow_a;
     float4* shadow_b = A->shadow_b;
-    const uint32_t n = counters[0];
+    // with sorted queues the entries are the hits of the classes, in class
+    // order; else every extension ray
+    const uint32_t* sorted = A->sorted;
+    const uint32_t c0 = sorted ? counters[kClassCount] : 0u;
+    const uint32_t c1 = sorted ? c0 + counters[kClassCount + 1] : 0u;
+    const uint32_t n = sorted ? c1 + counters[kClassCount + 2] : counters[0];
     uint32_t dropped = 0;
     uint32_t i = gid;
     while (true) {
@@ -511,7 +588,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VH_SHADE
         uint32_t p = 0;
         bool held = false;
         while (i < n) {
-            rd4 = rays[2 * i + 1];
+            uint32_t e = i;
+            if (sorted) {
+                const uint32_t k = i < c0 ? 0u : i < c1 ? 1u : 2u;
+                e = sorted[(size_t)k * A->cap + (i - (k == 0 ? 0u : k == 1 ? c0 : c1))];
+            }
+            rd4 = rays[2 * e + 1];
             i += threads;
             p = f2u(rd4.w) & kPathMask;
             info = P.info[p];
@@ -875,6 +957,7 @@ __global__ __launch_bounds__(kBlock) void k_film(VH_ARGS A) {
 
 __global__ void k_swap_counts(VH_ARGS A) {
     uint32_t* counters = A->counters;
+    if (threadIdx.x < kClasses) counters[kClassCount + threadIdx.x] = 0;
     if (threadIdx.x == 0) {
         counters[0] = counters[kLiveOut];
         counters[kLiveOut] = 0;
@@ -1023,9 +1106,9 @@ struct BatchShape {
     uint32_t bounces = 0, grid = 0;
     uint32_t paths = 0, pixels = 0;  // one thread per path (camera) and per pixel (decide, film)
     uint32_t mode = 0;  // 0 plain, 1 adaptive up to `minimum`, 2 adaptive after a decide pass
-    bool count = false, packet = false;
+    bool count = false, packet = false, sort = false;
     StreamState::GraphKey key() const {
-        return {bounces, grid, paths, pixels, mode, count ? 1u : 0u, packet ? 1u : 0u, 0u};
+        return {bounces, grid, paths, pixels, mode, count ? 1u : 0u, packet ? 1u : 0u, sort ? 1u : 0u};
     }
 };
 // Graphs a stream keeps (the oldest goes first): renders of a few image sizes
@@ -1054,6 +1137,7 @@ int launch_batch(const BatchShape& b, const LaunchArgs* A, hipStream_t s, Mark&&
         else hipLaunchKernelGGL(k_extend<false>, grid, block, 0, s, A, bounce);
         if ((rc = mark(stage, false))) return rc;
         if ((rc = mark(1, true))) return rc;
+        if (b.sort) hipLaunchKernelGGL(k_classify, grid, block, 0, s, A, bounce);
         hipLaunchKernelGGL(k_shade, grid, block, 0, s, A, bounce);
         if ((rc = mark(1, false))) return rc;
         if ((rc = mark(2, true))) return rc;
@@ -1222,6 +1306,13 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, float* g
     args.film = reinterpret_cast<float4*>(film);
     args.guides = reinterpret_cast<float4*>(guides);
     args.moments = moments;
+    args.cap = (uint32_t)cap;
+    const bool sort = (rp->flags & VH_RENDER_SHADE_SORTED) ? true
+                      : (rp->flags & VH_RENDER_SHADE_ARRIVAL) ? false : VH_SHADE_SORT != 0;
+    if (sort) {
+        VH_HIP(st->sorted.reserve(kClasses * cap * sizeof(uint32_t)));
+        args.sorted = st->sorted.as<uint32_t>();
+    }
     const LaunchArgs* dargs = st->launch_args.as<LaunchArgs>();
     // stage events: (stage, start, stop) triples recorded around each launch
     // when profiling (stage 3: the camera bounce's extend launch, also
@@ -1253,6 +1344,7 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, float* g
         shape.mode = !adaptive ? 0u : c.decide ? 2u : 1u;
         shape.count = count;
         shape.packet = VH_PACKET_CAMERA && !count && c.group >= 16;
+        shape.sort = sort;
         if (profile) {
             // launched one by one, with events around the stage launches
             rc = launch_batch(shape, dargs, s, [&](int stage, bool begin) -> int {
