@@ -439,16 +439,28 @@ def serve_bench(args):
         for key in keys:   # every scene resident before timing
             server.submit(key, pool[key], 32, 32, 1, 0).result(600)
 
-    out = {}
+    def sweep(server, name):
+        res = {}
+        warm(server)
+        for rate in [float(r) for r in args.rates.split(",")]:
+            rep = stream(server, rate)
+            res["inf" if rate == float("inf") else f"{rate:g}/s"] = {
+                "msamples_per_s": round(rep["msamples_per_s"], 2), "wall_s": round(rep["wall_s"], 3),
+                "latency_ms": {k: round(v, 1) for k, v in rep["latency_ms"].items()}}
+            log(f"{name} rate {rate}: {res[list(res)[-1]]}")
+        server.close()
+        return res
+
+    # two designs of the same loop: one HIP stream per slot (kernels of the
+    # admitted requests overlap on the device), and one shared wavefront
+    # (the admitted requests' paths in the same queues, vh_render_shared)
     server = serve.Server(backend, slots=args.slots)
-    warm(server)
-    for rate in [float(r) for r in args.rates.split(",")]:
-        rep = stream(server, rate)
-        out["inf" if rate == float("inf") else f"{rate:g}/s"] = {
-            "msamples_per_s": round(rep["msamples_per_s"], 2), "wall_s": round(rep["wall_s"], 3),
-            "latency_ms": {k: round(v, 1) for k, v in rep["latency_ms"].items()}}
-        log(f"rate {rate}: {out[list(out)[-1]]}")
-    server.close()
+    streams_res = sweep(server, "streams")
+    shared_server = serve.Server(backend, slots=16, shared=True)
+    shared_res = sweep(shared_server, "shared")
+    best_shared = (shared_res.get("inf") or {}).get("msamples_per_s", 0) > \
+        (streams_res.get("inf") or {}).get("msamples_per_s", 0)
+    out = shared_res if best_shared else streams_res
     serial = serve.Server(backend, slots=1)
     warm(serial)
     srep = stream(serial, float("inf"))
@@ -466,6 +478,9 @@ def serve_bench(args):
                                "spp 1..64, fixed spp", "requests": len(mix), "samples": samples,
                    "slots": args.slots, "scenes": tris},
         "latency_ms": sat["latency_ms"], "rates": out,
+        "design": "shared wavefront queues (16 requests per call)" if best_shared else
+                  f"one HIP stream per slot ({args.slots} slots)",
+        "designs": {"streams": streams_res, "shared": shared_res},
         "serial": {"msamples_per_s": round(srep["msamples_per_s"], 2),
                    "latency_ms": {k: round(v, 1) for k, v in srep["latency_ms"].items()}},
         "evictions": server.evictions}

@@ -193,7 +193,7 @@ int vh_trace_any(const vh_tlas* tlas, const vh_ray* rays, uint32_t count, uint32
  * the same kernel, and allocates nothing: its counters and traversal
  * overflow stacks live in a static device array of the library (64 MB per
  * GPU, looked up at vh_device_create, held until the process exits; at most
- * 128 workgroups of it).  Every call on a GPU uses that GPU's one array, so
+ * 32768 threads of it).  Every call on a GPU uses that GPU's one array, so
  * the library serializes these calls per GPU and each returns once its work
  * is done; they cannot overlap one another. */
 int vh_trace_closest_static(const vh_tlas* tlas, const vh_ray* rays, uint32_t count, uint32_t mask, void* stream,
@@ -372,6 +372,28 @@ int vh_render(vh_scene* scene, const vh_render_params* params, uint32_t flags, v
 int vh_render_guides(vh_scene* scene, const vh_render_params* params, uint32_t flags, void* stream, float* film,
                      float* guides);
 
+/* One job of vh_render_shared: a scene, the sample range to render (plain
+ * sampling: VH_RENDER_ADAPTIVE is refused) and its film, plus optional guides,
+ * as device memory that the job's samples are ADDED to, as vh_render_guides
+ * adds them. */
+typedef struct vh_render_job {
+    vh_scene* scene;
+    vh_render_params params;
+    float* film;
+    float* guides;
+} vh_render_job;
+
+/* Renders up to 16 jobs (scenes on one device, each scene in at most one job)
+ * with their paths in one wavefront: every batch gives each job with samples
+ * left a share of the path slots, the jobs' camera rays fill one extension
+ * queue, their hits one shading queue and their shadow rays one shadow queue,
+ * and every path traverses and shades with its own job's scene.  Each film
+ * equals what vh_render_guides gives for the job alone.  This is the shared
+ * wavefront queue of the serving loop; the reference serves one render at a
+ * time (crates/vork-cli/src/session.rs:1-7).  Each job's scene stats get its
+ * paths; rays and device time are those of the whole call. */
+int vh_render_shared(const vh_render_job* jobs, uint32_t count, void* stream);
+
 /* ---------------------------------------------------------------- denoiser */
 
 /* The learned denoiser that replaces the OIDN pass of
@@ -392,6 +414,35 @@ int vh_denoiser_create(vh_device* device, const void* weights, size_t bytes, vh_
 int vh_denoise(vh_denoiser* denoiser, uint32_t width, uint32_t height, float samples, const float* film,
                const float* guides, uint32_t flags, void* stream, float* out);
 void vh_denoiser_release(vh_denoiser* denoiser);
+
+/* Training of the denoiser with hand-written kernels (the reference trains
+ * none: it ships OIDN's weights; config 4 trains this U-Net on pairs the path
+ * tracer renders).  A trainer holds fp32 master weights, their Adam moments
+ * and the bf16 copies the MFMA kernels read.  `weights` is a version-1 blob to
+ * start from, or NULL for a seeded default initialisation (uniform in
+ * +-1/sqrt(fan in)). */
+typedef struct vh_denoiser_trainer vh_denoiser_trainer;
+int vh_trainer_create(vh_device* device, const void* weights, size_t bytes, uint32_t seed,
+                      vh_denoiser_trainer** out);
+/* One step over `images` images of width x height (device pointers): film
+ * and guide sums of `samples` samples as vh_render_guides writes them, and
+ * target mean radiance (3 floats per pixel).  Runs the forward pass keeping
+ * every activation, the loss (relative MSE of the radiance plus L1 of its
+ * log1p, averaged over the batch, returned in *loss), the backward pass
+ * (weight and bias gradients summed over the images) and, unless `flags` has
+ * VH_TRAIN_NO_UPDATE, an Adam step with `learning_rate`. */
+#define VH_TRAIN_NO_UPDATE 1u
+int vh_trainer_step(vh_denoiser_trainer* trainer, uint32_t images, uint32_t width, uint32_t height, float samples,
+                    const float* films, const float* guides, const float* targets, float learning_rate,
+                    uint32_t flags, void* stream, float* loss);
+/* The last step's fp32 gradients in blob order (per layer [tap][cout][cin]
+ * weights; biases); the counts must be the network's. */
+int vh_trainer_gradients(vh_denoiser_trainer* trainer, float* weights, size_t weight_count, float* biases,
+                         size_t bias_count);
+/* Writes the current weights as a version-1 blob (*bytes = its size; with
+ * `blob` NULL only the size). */
+int vh_trainer_export(vh_denoiser_trainer* trainer, void* blob, size_t capacity, size_t* bytes);
+void vh_trainer_release(vh_denoiser_trainer* trainer);
 
 /* Kernel statistics of the last vh_render on this scene. */
 typedef struct vh_render_stats {

@@ -337,3 +337,43 @@ def test_material_sorted_shading_gives_the_same_film(renderer, make):
             assert stats["dropped_samples"] == 0 or batch != 0
         assert np.array_equal(films[0][0], films[1][0]), (batch, adaptive)
         assert np.array_equal(films[0][1], films[1][1]), (batch, adaptive)
+
+
+def test_shared_wavefront_queue_films_equal_each_job_alone(renderer):
+    """vh_render_shared: jobs of different scenes, sizes, sample counts, sample
+    ranges and bounce limits with their paths in one wavefront (one extension,
+    one shading and one shadow queue, each path traversing its own job's TLAS)
+    give each job exactly the film and guides vh_render_guides gives it alone.
+    The 1080p x 64-spp job does not fit one batch beside the others, so the
+    call runs several batches."""
+    torch = pytest.importorskip("torch")
+    from vork_amd import native
+    specs = [(S.config0(width=64, height=48), 16, 0, 16, 4, True),
+             (S.small_config1(width=96, height=54), 8, 0, 8, 8, False),
+             (S.config2(width=48, height=27, spp=4, birds=3000), 4, 0, 4, 3, True),
+             (S.config0(seed=2, width=32, height=32), 16, 3, 11, 4, False),
+             (S.small_config1(seed=3, width=1920, height=1080), 64, 0, 64, 8, False)]
+    jobs, alone = [], []
+    for sc, spp, b, e, depth, guides in specs:
+        inst = renderer.install(sc)
+        p = sc.render_params(samples=spp, seed=7, adaptive=False)
+        p.sample_begin, p.sample_end, p.max_bounces = b, e, depth
+        film = torch.zeros((sc.height, sc.width, 4), dtype=torch.float32, device="cuda")
+        g = torch.zeros((sc.height, sc.width, 2, 4), dtype=torch.float32, device="cuda") if guides else None
+        jobs.append((inst, p, film, g))
+        f1 = torch.zeros_like(film)
+        if guides:
+            g1 = torch.zeros_like(g)
+            inst.render_guides(p, f1, g1, device=True)
+        else:
+            g1 = None
+            inst.render(p, device_film=f1)
+        alone.append((f1, g1))
+    native.render_shared(jobs)
+    torch.cuda.synchronize()
+    for (inst, p, film, g), (f1, g1) in zip(jobs, alone):
+        assert torch.equal(film, f1), (p.width, p.height)
+        if g is not None:
+            assert torch.equal(g, g1)
+        assert inst.stats()["paths"] == p.width * p.height * (p.sample_end - p.sample_begin)
+    assert float(jobs[-1][2][..., 3].min()) == 64.0

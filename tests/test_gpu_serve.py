@@ -21,9 +21,9 @@ def requests():
     return scenes, mix
 
 
-def run(backend, slots, budget=None):
+def run(backend, slots, budget=None, shared=False):
     scenes, mix = requests()
-    s = serve.Server(backend, slots=slots, budget=budget)
+    s = serve.Server(backend, slots=slots, budget=budget, shared=shared)
     try:
         reqs = [s.submit(k, scenes[k], w, h, spp, seed, adaptive, denoise)
                 for k, w, h, spp, seed, adaptive, denoise in mix]
@@ -90,5 +90,24 @@ def test_a_1080p_64spp_request_on_a_large_scene_beside_small_ones():
             assert a.shape == (h, w, 4) and np.isfinite(a).all()
             assert np.array_equal(a[..., 3], np.full((h, w), float(spp), np.float32))
             assert np.array_equal(a, m), (k, w, h, spp)
+    finally:
+        backend.close()
+
+
+def test_shared_wavefront_server_gives_the_results_of_each_request_alone():
+    """The shared-queue server (every admitted request's next slice of samples
+    in one vh_render_shared call, arrivals joining the next call) returns each
+    request's film, adaptive and denoised ones included, bit-identical to the
+    request rendered alone."""
+    backend = serve.GpuBackend(0, slots=4)
+    try:
+        alone, _ = run(backend, 1)
+        for k in list(backend.scenes):
+            backend.release(k)
+        for slots in (4, 16):
+            shared, rep = run(backend, slots, shared=True)
+            assert rep["failed"] == 0 and rep["requests"] == len(alone)
+            for a, m in zip(alone, shared):
+                assert np.array_equal(a, m)
     finally:
         backend.close()

@@ -59,6 +59,61 @@ class FakeBackend:
         return np.full((height, width, 4), hash((key, seed)) % 1000, np.float32), {"slot": slot}
 
 
+class FakeSharedBackend(FakeBackend):
+    """Shared-wavefront stand-in: a film per request summing the sample
+    slices each call hands it (value = sample count), and a log of the calls'
+    jobs."""
+
+    def __init__(self, batch_paths=256, **kw):
+        super().__init__(**kw)
+        self.batch, self.calls, self.films = batch_paths, [], {}
+
+    def shared_batch_paths(self):
+        return self.batch
+
+    def shared_state_bytes(self):
+        return self.batch
+
+    def render_shared(self, items):
+        time.sleep(self.render_s)
+        self.calls.append([(rid, s0, s1) for _, _, _, rid, _, _, _, _, s0, s1, _ in items])
+        seats = [seat for seat, *_ in items]
+        assert len(set(seats)) == len(seats)
+        for seat, key, scene, rid, w, h, spp, seed, s0, s1, denoise in items:
+            f = self.films.setdefault(rid, np.zeros((h, w, 4), np.float32))
+            f += s1 - s0
+
+    def finish_shared(self, rid, spp, denoise):
+        return self.films.pop(rid), {}
+
+    def drop_shared(self, rid):
+        self.films.pop(rid, None)
+
+
+def test_shared_wavefront_slices_samples_and_admits_arrivals_between_calls():
+    b = FakeSharedBackend(batch_paths=256, render_s=0.01)
+    s = serve.Server(b, slots=4, shared=True)
+    big = Scene("big", width=16, height=16, spp=8)       # 256 px: one sample per call beside others
+    small = Scene("small", width=4, height=4, spp=2)
+    r_big = s.submit("big", big)
+    time.sleep(0.015)
+    r_small = [s.submit("small", small, seed=k) for k in range(6)]
+    s.wait_all(10)
+    s.close()
+    assert np.all(r_big.result() == 8) and all(np.all(r.result() == 2) for r in r_small)
+    # every call renders at most 4 requests, each slice inside its request's
+    # exposure, slices of one request consecutive; the big one shares calls
+    # with small ones that arrived after it started
+    seen = {}
+    for call in b.calls:
+        assert len(call) <= 4
+        for rid, s0, s1 in call:
+            assert s0 == seen.get(rid, 0) and s1 > s0
+            seen[rid] = s1
+    assert any(len(c) > 1 and r_big.rid in [rid for rid, _, _ in c] for c in b.calls)
+    assert s.report()["requests"] == 7
+
+
 def test_slots_bound_the_renders_in_flight_and_scenes_install_once():
     b = FakeBackend()
     s = serve.Server(b, slots=3)

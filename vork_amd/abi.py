@@ -89,6 +89,10 @@ class RenderParams(C.Structure):
                 ("reserved", C.c_uint32 * 2)]
 
 
+class RenderJob(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("params", RenderParams), ("film", C.c_void_p), ("guides", C.c_void_p)]
+
+
 class RenderStats(C.Structure):
     _fields_ = [("paths", C.c_uint64), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
                 ("dropped_samples", C.c_uint64), ("device_ms", C.c_double),
@@ -124,7 +128,7 @@ def default_params(width, height, samples, seed=1, max_bounces=12, adaptive=True
 
 def check_layout():
     sizes = {Geometry: 48, BlasInfo: 40, Instance: 64, Ray: 32, Hit: 24, Material: 32, RectLight: 80,
-             Camera: 64, Sky: 80, RenderParams: 76, RenderStats: 176}
+             Camera: 64, Sky: 80, RenderParams: 76, RenderStats: 176, RenderJob: 104}
     for t, n in sizes.items():
         if C.sizeof(t) != n:
             raise AssertionError(f"{t.__name__} is {C.sizeof(t)} bytes, header layout says {n}")

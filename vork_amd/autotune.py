@@ -12,9 +12,10 @@ vork_amd/build.py compiles in.
 The search is coordinate descent around the current table, in two halves
 because compiling and measuring happen on different machines:
 
-    python -m vork_amd.autotune build [--knobs a,b] [--from results.json]
+    python -m vork_amd.autotune build [--knobs a,b|none] [--from results.json]
         compiles one library per candidate (the current table with one knob
-        changed; with --from, also the table of the best values found so far)
+        changed; with --from, also the table of the best values found so far;
+        --knobs none builds only that one)
         into vork_amd/_tune/<name>/libvork_hip.so and writes the plan;
     python -m vork_amd.autotune measure --out results.json [--reps 2]
         (on the GPU) runs bench.py on the target workload for every variant
@@ -90,7 +91,7 @@ def build_variant(values):
 def cmd_build(args):
     table = load_table()
     base = dict(table["knobs"])
-    knobs = args.knobs.split(",") if args.knobs else DEFAULT_SWEEP
+    knobs = [] if args.knobs == "none" else args.knobs.split(",") if args.knobs else DEFAULT_SWEEP
     plan = {"baseline": {"values": base, "lib": None}, "variants": []}
     plan["baseline"]["lib"] = build_variant(base)
     if args.from_results:

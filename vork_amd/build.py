@@ -19,7 +19,7 @@ ARCH = os.environ.get("VORK_HIP_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-function",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-munsafe-fp-atomics"]
-SOURCES = ["vh_api.hip", "vh_build.hip", "vh_trace.hip", "vh_render.hip", "vh_denoise.hip"]
+SOURCES = ["vh_api.hip", "vh_build.hip", "vh_trace.hip", "vh_render.hip", "vh_denoise.hip", "vh_train.hip"]
 TUNING = os.path.join(HERE, "tuning.json")
 
 

@@ -613,6 +613,36 @@ int vh_render(vh_scene* sc, const vh_render_params* p, uint32_t flags, void* str
     return vh_render_guides(sc, p, flags, stream, film, nullptr);
 }
 
+int vh_render_shared(const vh_render_job* jobs, uint32_t count, void* stream) {
+    clear();
+    if (!jobs || count == 0 || count > 16) return fail(VH_ERR_USAGE, "vh_render_shared: 1 to 16 jobs");
+    std::vector<vh_scene*> scenes;
+    for (uint32_t j = 0; j < count; j++) {
+        const vh_render_job& jb = jobs[j];
+        const vh_render_params* p = &jb.params;
+        if (!jb.scene || !jb.film) return fail(VH_ERR_USAGE, "vh_render_shared: null scene or film");
+        if (jb.scene->device != jobs[0].scene->device)
+            return fail(VH_ERR_USAGE, "vh_render_shared: the scenes must be on one device");
+        if (p->width == 0 || p->height == 0 || p->samples == 0 || p->sample_end > p->samples ||
+            p->sample_begin > p->sample_end)
+            return fail(VH_ERR_USAGE, "vh_render_shared: job " + std::to_string(j) + " has an empty image or a "
+                                      "sample range outside its exposure");
+        if (p->flags & VH_RENDER_ADAPTIVE)
+            return fail(VH_ERR_USAGE, "vh_render_shared: adaptive sampling is rendered by vh_render");
+        for (vh_scene* q : scenes)
+            if (q == jb.scene) return fail(VH_ERR_USAGE, "vh_render_shared: a scene appears in two jobs");
+        scenes.push_back(jb.scene);
+    }
+    vh_device* dev = jobs[0].scene->device;
+    VH_HIP(hipSetDevice(dev->ordinal));
+    hipStream_t s = pick(dev, stream);
+    // the scenes' locks in address order, so that overlapping calls cannot deadlock
+    std::sort(scenes.begin(), scenes.end());
+    std::vector<std::unique_lock<std::mutex>> holds;
+    for (vh_scene* q : scenes) holds.emplace_back(q->render_lock);
+    return render_shared(jobs, count, s);
+}
+
 int vh_render_guides(vh_scene* sc, const vh_render_params* p, uint32_t flags, void* stream, float* film,
                      float* guides) {
     clear();

@@ -65,7 +65,7 @@ struct BuildScratch {
 // each other while calls on different streams overlap.
 struct StreamState {
     std::mutex lock;
-    DeviceBuffer paths, rays_a, rays_b, shadow_rays, counters, overflow, scratch_u32, launch_args, sorted;
+    DeviceBuffer paths, rays_a, rays_b, shadow_rays, counters, overflow, scratch_u32, launch_args, sorted, jobs;
     size_t capacity = 0;  // paths the wavefront buffers hold
     BuildScratch build;
     // instantiated launch sequences of render batches by shape (hipGraphExec_t),
@@ -235,6 +235,8 @@ int bake_luts(vh_device* dev);
 
 // --------------------------------------------------------------- render
 int render_scene(vh_scene* scene, const vh_render_params* p, float* film, float* guides, hipStream_t s);
+// Jobs checked by the caller: on one device, plain (not adaptive), device films.
+int render_shared(const vh_render_job* jobs, uint32_t count, hipStream_t s);
 // The scratch of stream s on this device, created on first use.
 StreamState* stream_state(vh_device* dev, hipStream_t s);
 

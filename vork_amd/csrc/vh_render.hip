@@ -195,6 +195,22 @@ constexpr uint32_t kClasses = 3;
 #define VH_GRID_BLOCKS_PER_CU 8
 #endif
 
+// One job of a shared-queue batch (vh_render_shared): a scene's own
+// constants, tables and film, its range of the batch's path slots and of the
+// film pass's tiles.
+struct JobArgs {
+    Consts c;
+    Scene S;
+    float4* film;
+    float4* guides;
+    uint32_t path_base, paths;
+    uint32_t tile_base, tiles;
+};
+// Jobs of one shared batch at most: a shadow ray carries its job in the top
+// 4 bits of its id (2 x event slot + estimator < 2^28 with at most 2^27
+// paths).
+constexpr uint32_t kMaxJobs = 16, kJobShift = 28, kShadowIdMask = (1u << kJobShift) - 1u;
+
 // Arguments of one batch's kernels, in device memory.  Every kernel of a batch
 // takes a pointer to this block (and its bounce), so the launch sequence of a
 // batch does not depend on the scene, the buffers or the batch: it is recorded
@@ -215,8 +231,17 @@ struct LaunchArgs {
     uint32_t* sorted;  // sorted shading: kClasses queues of `cap` extension-queue entries; null: arrival order
     uint32_t cap;      // paths the wavefront buffers hold
     uint32_t n;        // paths of the batch
+    const JobArgs* jobs;  // shared-queue batches: the jobs in path order (P, c, S, film unused)
+    uint32_t job_count;
 };
 #define VH_ARGS const LaunchArgs* __restrict__
+
+// The job whose path slots hold path p (jobs are in path order).
+__device__ __forceinline__ uint32_t job_of_path(const LaunchArgs* A, uint32_t p) {
+    uint32_t j = 0;
+    while (j + 1 < A->job_count && p >= A->jobs[j + 1].path_base) j++;
+    return j;
+}
 
 __device__ __forceinline__ void reset_claims(uint32_t* counters) {
     counters[kExtendClaims] = 0;
@@ -269,9 +294,9 @@ __global__ void k_decide(VH_ARGS A) {
 // adaptive ones append the rays of the pixels that still run.
 template <bool kAdaptive>
 __device__ __forceinline__ void camera_path(const Paths& P, const Consts& c, const Scene& S, float4* rays,
-                                            uint32_t* counters, const Moment* M, uint32_t p) {
+                                            uint32_t* counters, const Moment* M, uint32_t p, uint32_t local) {
     uint32_t s_local, t;
-    slot_sample(p, c.group, c.pixels, s_local, t);
+    slot_sample(local, c.group, c.pixels, s_local, t);
     const uint32_t pixel = slot_to_pixel(t, c.width, c.height);
     if (kAdaptive && c.decide && M[pixel].stopped) return;
     uint32_t px = pixel % c.width, py = pixel / c.width;
@@ -311,7 +336,25 @@ __global__ void k_camera(VH_ARGS A) {
         reset_claims(counters);
     }
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x)
-        camera_path<kAdaptive>(A->P, A->c, A->S, A->rays[0], counters, A->moments, p);
+        camera_path<kAdaptive>(A->P, A->c, A->S, A->rays[0], counters, A->moments, p, p);
+}
+// Camera rays of a shared-queue batch: path p belongs to the job whose slot
+// range holds it, and is that job's path p - path_base.
+__global__ void k_camera_shared(VH_ARGS A) {
+    const uint32_t n = A->n;
+    uint32_t* counters = A->counters;
+    if (blockIdx.x == 0 && threadIdx.x < kClasses) counters[kClassCount + threadIdx.x] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        counters[0] = n;
+        counters[kLiveOut] = 0;
+        counters[kShadowB] = 0;
+        counters[kPending] = 0;
+        reset_claims(counters);
+    }
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        const JobArgs& J = A->jobs[job_of_path(A, p)];
+        camera_path<false>(A->P, J.c, J.S, A->rays[0], counters, nullptr, p, p - J.path_base);
+    }
 }
 
 // Sums per-thread visit counts of a block into two u64 counters.
@@ -335,6 +378,8 @@ struct ExtendSource {
     uint32_t n;
     uint64_t* nodes;
     uint64_t* tris;
+    const LaunchArgs* A;
+    __device__ TlasView view(uint32_t p) const { return A->jobs[job_of_path(A, p)].S.tv; }
     __device__ uint32_t count() const { return n; }
     __device__ void fetch(uint32_t i, RayData& r) const {
         r.a = rays[2 * i];
@@ -357,7 +402,7 @@ struct ExtendSource {
     }
 };
 
-template <bool kCount>
+template <bool kCount, bool kShared = false>
 VH_TRACE_KERNEL void k_extend(VH_ARGS A, uint32_t bounce) {
     __shared__ uint64_t lds_stack[kLdsStack * kBlock];
     uint32_t* counters = A->counters;
@@ -367,8 +412,8 @@ VH_TRACE_KERNEL void k_extend(VH_ARGS A, uint32_t bounce) {
     Stack st = make_stack(lds_stack, reinterpret_cast<uint64_t*>(A->overflow));
     uint64_t nodes = 0, tris = 0;
     const TlasView tv = A->S.tv;
-    ExtendSource<kCount> src{A->P, A->rays[bounce & 1u], n, &nodes, &tris};
-    if (!trace_persistent<false, kCount>(tv, src, &counters[kExtendClaims], st))
+    ExtendSource<kCount> src{A->P, A->rays[bounce & 1u], n, &nodes, &tris, A};
+    if (!trace_persistent<false, kCount, ExtendSource<kCount>, kShared>(tv, src, &counters[kExtendClaims], st))
         atomicOr(&counters[8], kStackOverflow);
     if (gid == 0) {
         atomicAdd((unsigned long long*)&counters[4], (unsigned long long)n);
@@ -559,13 +604,16 @@ __global__ __launch_bounds__(kBlock) void k_classify(VH_ARGS A, uint32_t bounce_
 #ifndef VH_SHADE_WAVES_PER_EU
 #define VH_SHADE_WAVES_PER_EU 4
 #endif
+// With kShared the queue holds the paths of several jobs: each path shades
+// with its job's constants and tables, and its shadow rays carry the job.
+template <bool kShared = false>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VH_SHADE_WAVES_PER_EU))) void k_shade(
     VH_ARGS A, uint32_t bounce_parity) {
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, threads = gridDim.x * blockDim.x;
     uint32_t* counters = A->counters;
     const Paths& P = A->P;
-    const Consts& c = A->c;
-    const Scene& S = A->S;
+    const Consts& c_all = A->c;
+    const Scene& S_all = A->S;
     const float4* rays = A->rays[bounce_parity & 1u];
     float4* next = A->rays[(bounce_parity + 1u) & 1u];
     float4* shadow_rays = A->shadow_a;
@@ -604,12 +652,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VH_SHADE
                 held = true;
                 break;
             }
-            const float sky_limit = ((info.w >> kBounceShift) & 0xffu) == 0 ? c.clamp_direct : c.clamp_indirect;
-            const f3 sky = clamp_sample(mk(t4.x, t4.y, t4.z) * sky_radiance(c.sky, mk(rd4.x, rd4.y, rd4.z)),
+            const Consts& cm = kShared ? A->jobs[job_of_path(A, p)].c : c_all;
+            const float sky_limit = ((info.w >> kBounceShift) & 0xffu) == 0 ? cm.clamp_direct : cm.clamp_indirect;
+            const f3 sky = clamp_sample(mk(t4.x, t4.y, t4.z) * sky_radiance(cm.sky, mk(rd4.x, rd4.y, rd4.z)),
                                         sky_limit, dropped);
             P.rad[p] = make_float4(r4.x + sky.x, r4.y + sky.y, r4.z + sky.z, r4.w);
         }
         if (!held) break;
+        const uint32_t job = kShared ? job_of_path(A, p) : 0u;
+        const Consts& c = kShared ? A->jobs[job].c : c_all;
+        const Scene& S = kShared ? A->jobs[job].S : S_all;
+        const uint32_t job_bits = kShared ? job << kJobShift : 0u;
         uint32_t flags = info.w;
         const uint32_t bounce = (flags >> kBounceShift) & 0xffu;
         uint32_t diffuse = (flags >> kDiffuseShift) & 0xfu, glossy = (flags >> kGlossyShift) & 0xfu;
@@ -758,8 +811,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VH_SHADE
             // only b is pending), estimator b goes to the second queue
             P.pend[q] = make_float4(ta.x, ta.y, ta.z, limit);
             P.pend1[q] = make_float4(tb.x, tb.y, tb.z, u2f(p));
-            if (has_a) put_ray(shadow_rays, q, s0o, s0t, s0d, 2u * q);
-            else put_ray(shadow_rays, q, s1o, -1.f, s1d, 2u * q);
+            if (has_a) put_ray(shadow_rays, q, s0o, s0t, s0d, (2u * q) | job_bits);
+            else put_ray(shadow_rays, q, s1o, -1.f, s1d, (2u * q) | job_bits);
         }
         const uint64_t bm = __ballot(has_b);
         if (bm) {
@@ -767,7 +820,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VH_SHADE
             uint32_t base = 0;
             if ((int)__lane_id() == leader) base = atomicAdd(&counters[kShadowB], (uint32_t)__popcll(bm));
             base = __shfl(base, leader);
-            if (has_b) put_ray(shadow_b, base + (uint32_t)__popcll(bm & lt), s1o, s1t, s1d, 2u * q + 1u);
+            if (has_b) put_ray(shadow_b, base + (uint32_t)__popcll(bm & lt), s1o, s1t, s1d, (2u * q + 1u) | job_bits);
         }
         if (alive) {
             f3 origin = path_origin(sf, error, bs.direction);
@@ -781,7 +834,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VH_SHADE
 // Shadow rays of the pending next events: ray id 2q+k is estimator k of
 // event q.  The first `na` come from the estimator-a queue (slot q = event
 // q), the rest from the estimator-b queue.
-template <bool kCount>
+template <bool kCount, bool kShared>
 struct ShadowSource {
     Paths P;
     const float4* rays;
@@ -789,6 +842,8 @@ struct ShadowSource {
     uint32_t na, n;
     uint64_t* nodes;
     uint64_t* tris;
+    const LaunchArgs* A;
+    __device__ TlasView view(uint32_t id) const { return A->jobs[id >> kJobShift].S.tv; }
     __device__ uint32_t count() const { return n; }
     __device__ void fetch(uint32_t i, RayData& r) const {
         const float4* q = i < na ? rays + 2 * (size_t)i : rays_b + 2 * (size_t)(i - na);
@@ -805,12 +860,12 @@ struct ShadowSource {
         key = f2u(r.b.w);
     }
     __device__ void store(uint32_t r, const TraceHit& h, const TraceCounts& c) const {
-        P.vis[r] = h.found ? 1 : 0;
+        P.vis[kShared ? r & kShadowIdMask : r] = h.found ? 1 : 0;
         if (kCount) { *nodes += c.nodes; *tris += c.tris; }
     }
 };
 
-template <bool kCount>
+template <bool kCount, bool kShared = false>
 VH_ANY_KERNEL void k_shadow(VH_ARGS A) {
     __shared__ uint64_t lds_stack[kLdsStack * kBlock];
     uint32_t* counters = A->counters;
@@ -820,8 +875,8 @@ VH_ANY_KERNEL void k_shadow(VH_ARGS A) {
     Stack st = make_stack(lds_stack, reinterpret_cast<uint64_t*>(A->overflow));
     uint64_t nodes = 0, tris = 0;
     const TlasView tv = A->S.tv;
-    ShadowSource<kCount> src{A->P, A->shadow_a, A->shadow_b, na, n, &nodes, &tris};
-    if (!trace_persistent<true, kCount>(tv, src, &counters[kShadowClaims], st))
+    ShadowSource<kCount, kShared> src{A->P, A->shadow_a, A->shadow_b, na, n, &nodes, &tris, A};
+    if (!trace_persistent<true, kCount, ShadowSource<kCount, kShared>, kShared>(tv, src, &counters[kShadowClaims], st))
         atomicOr(&counters[8], kStackOverflow);
     if (gid == 0) atomicAdd((unsigned long long*)&counters[6], (unsigned long long)n);
     flush_counts<kCount>(nodes, tris, (unsigned long long*)&counters[20]);
@@ -953,6 +1008,22 @@ __global__ __launch_bounds__(kBlock) void k_film(VH_ARGS A) {
     __shared__ uint32_t s_flags[kBlock * kFilmStage];
     for (uint32_t tile = blockIdx.x; tile * kBlock < A->c.pixels; tile += gridDim.x)
         film_tile<kAdaptive>(A->P, A->c, A->film, A->guides, A->moments, A->counters, tile, s_rad, s_flags);
+}
+// The film pass of a shared-queue batch: tiles numbered across the jobs, each
+// job's tiles over its own pixels and path slots.
+__global__ __launch_bounds__(kBlock) void k_film_shared(VH_ARGS A) {
+    __shared__ float4 s_rad[kBlock * kFilmPad];
+    __shared__ uint32_t s_flags[kBlock * kFilmStage];
+    const JobArgs& last = A->jobs[A->job_count - 1];
+    for (uint32_t tile = blockIdx.x; tile < last.tile_base + last.tiles; tile += gridDim.x) {
+        uint32_t j = 0;
+        while (j + 1 < A->job_count && tile >= A->jobs[j + 1].tile_base) j++;
+        const JobArgs& J = A->jobs[j];
+        Paths P = A->P;  // the job's slots start at its path base
+        P.thr += J.path_base; P.rad += J.path_base; P.info += J.path_base; P.hit += J.path_base;
+        P.guide_a += J.path_base; P.guide_n += J.path_base; P.hit_inst += J.path_base;
+        film_tile<false>(P, J.c, J.film, J.guides, nullptr, A->counters, tile - J.tile_base, s_rad, s_flags);
+    }
 }
 
 __global__ void k_swap_counts(VH_ARGS A) {
@@ -1138,7 +1209,7 @@ int launch_batch(const BatchShape& b, const LaunchArgs* A, hipStream_t s, Mark&&
         if ((rc = mark(stage, false))) return rc;
         if ((rc = mark(1, true))) return rc;
         if (b.sort) hipLaunchKernelGGL(k_classify, grid, block, 0, s, A, bounce);
-        hipLaunchKernelGGL(k_shade, grid, block, 0, s, A, bounce);
+        hipLaunchKernelGGL(k_shade<false>, grid, block, 0, s, A, bounce);
         if ((rc = mark(1, false))) return rc;
         if ((rc = mark(2, true))) return rc;
         if (b.count) hipLaunchKernelGGL(k_shadow<true>, grid, block, 0, s, A);
@@ -1185,6 +1256,74 @@ int launch_batch_graph(StreamState* st, const BatchShape& b, const LaunchArgs* A
 }
 }  // namespace
 
+namespace {
+// Grows the stream's wavefront buffers to `want` paths (path state, two
+// extension queues, the estimator-a and -b shadow queues); on failure the
+// stream holds none.
+hipError_t reserve_wavefront(StreamState* st, size_t want) {
+    if (st->capacity >= want) return hipSuccess;
+    st->capacity = 0;
+    hipError_t e = st->paths.allocate(want * kPathStateBytes);
+    if (e == hipSuccess) e = st->rays_a.allocate(want * 2 * sizeof(float4));
+    if (e == hipSuccess) e = st->rays_b.allocate(want * 2 * sizeof(float4));
+    if (e == hipSuccess) e = st->shadow_rays.allocate(2 * want * 2 * sizeof(float4));  // estimator a, then b
+    if (e == hipSuccess) {
+        st->capacity = want;
+        return hipSuccess;
+    }
+    st->paths.release();
+    st->rays_a.release();
+    st->rays_b.release();
+    st->shadow_rays.release();
+    return e;
+}
+// The path-state arrays of `cap` paths in the stream's buffer.
+Paths path_layout(StreamState* st, size_t cap) {
+    Paths P;
+    char* base = st->paths.as<char>();
+    float4** f4[] = {&P.thr, &P.rad, (float4**)&P.info, &P.hit, &P.pend, &P.pend1, &P.guide_a, &P.guide_n};
+    size_t off = 0;
+    for (auto* q : f4) {
+        *q = reinterpret_cast<float4*>(base + off);
+        off += cap * sizeof(float4);
+    }
+    P.hit_inst = reinterpret_cast<uint32_t*>(base + off);
+    off += cap * sizeof(uint32_t);
+    P.vis = reinterpret_cast<uint8_t*>(base + off);
+    return P;
+}
+Scene scene_tables(const vh_scene* sc) {
+    const vh_device* dev = sc->device;
+    Scene S;
+    S.tv = sc->tlas->view();
+    S.materials = sc->materials.as<vh_material>();
+    S.lights = sc->lights.as<vh_rect_light>();
+    S.tree = sc->light_tree.as<LightNode>();
+    S.filter = sc->filter.as<float>();
+    S.luts.moments = dev->lut_moments.as<float4>();
+    S.luts.dielectric = dev->lut_dielectric.as<float>();
+    return S;
+}
+// A render's constants (the batch fields and the adaptive minimum are set per batch).
+Consts scene_consts(const vh_scene* sc, const vh_render_params* rp, bool guides) {
+    Consts c;
+    std::memset(&c, 0, sizeof(c));
+    c.width = rp->width; c.height = rp->height; c.samples = rp->samples; c.seed = rp->seed;
+    c.max_bounces = std::min(rp->max_bounces, 254u);
+    c.diffuse_bounces = std::min(rp->diffuse_bounces, 14u);
+    c.glossy_bounces = std::min(rp->glossy_bounces, 14u);
+    c.caustics = rp->caustics;
+    c.clamp_direct = rp->clamp_direct; c.clamp_indirect = rp->clamp_indirect; c.light_threshold = rp->light_threshold;
+    c.light_count = sc->light_count; c.light_nodes = sc->light_nodes; c.material_count = sc->material_count;
+    c.pixels = rp->width * rp->height;
+    c.threshold = rp->adaptive_threshold > 0.f ? rp->adaptive_threshold : 0.01f;
+    c.guides = guides;
+    c.camera = sc->camera;
+    c.sky = sc->sky;
+    return c;
+}
+}  // namespace
+
 int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, float* guides, hipStream_t s) {
     vh_device* dev = sc->device;
     const uint32_t pixels = rp->width * rp->height;
@@ -1206,20 +1345,8 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, float* g
     std::lock_guard<std::mutex> hold(st->lock);
     // grow the stream's wavefront state; halve the batch while HBM is short
     while (st->capacity < (size_t)pixels * batch) {
-        const size_t want = (size_t)pixels * batch;
-        st->capacity = 0;
-        hipError_t e = st->paths.allocate(want * kPathStateBytes);
-        if (e == hipSuccess) e = st->rays_a.allocate(want * 2 * sizeof(float4));
-        if (e == hipSuccess) e = st->rays_b.allocate(want * 2 * sizeof(float4));
-        if (e == hipSuccess) e = st->shadow_rays.allocate(2 * want * 2 * sizeof(float4));  // estimator a, then b
-        if (e == hipSuccess) {
-            st->capacity = want;
-            break;
-        }
-        st->paths.release();
-        st->rays_a.release();
-        st->rays_b.release();
-        st->shadow_rays.release();
+        const hipError_t e = reserve_wavefront(st, (size_t)pixels * batch);
+        if (e == hipSuccess) break;
         if (e != hipErrorOutOfMemory || batch == 1) return hip_fail(e, "vh_render: wavefront state");
         batch = (batch + 1) / 2;
     }
@@ -1249,42 +1376,10 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, float* g
         sc->moments_next = UINT32_MAX;  // set again once this range is rendered
         moments = sc->moments.as<Moment>();
     }
-    Paths P;
-    {
-        char* base = st->paths.as<char>();
-        float4** f4[] = {&P.thr, &P.rad, (float4**)&P.info, &P.hit, &P.pend, &P.pend1, &P.guide_a, &P.guide_n};
-        size_t off = 0;
-        for (auto* q : f4) {
-            *q = reinterpret_cast<float4*>(base + off);
-            off += cap * sizeof(float4);
-        }
-        P.hit_inst = reinterpret_cast<uint32_t*>(base + off);
-        off += cap * sizeof(uint32_t);
-        P.vis = reinterpret_cast<uint8_t*>(base + off);
-    }
-    Scene S;
-    S.tv = sc->tlas->view();
-    S.materials = sc->materials.as<vh_material>();
-    S.lights = sc->lights.as<vh_rect_light>();
-    S.tree = sc->light_tree.as<LightNode>();
-    S.filter = sc->filter.as<float>();
-    S.luts.moments = dev->lut_moments.as<float4>();
-    S.luts.dielectric = dev->lut_dielectric.as<float>();
-    Consts c;
-    std::memset(&c, 0, sizeof(c));
-    c.width = rp->width; c.height = rp->height; c.samples = rp->samples; c.seed = rp->seed;
-    c.max_bounces = std::min(rp->max_bounces, 254u);
-    c.diffuse_bounces = std::min(rp->diffuse_bounces, 14u);
-    c.glossy_bounces = std::min(rp->glossy_bounces, 14u);
-    c.caustics = rp->caustics;
-    c.clamp_direct = rp->clamp_direct; c.clamp_indirect = rp->clamp_indirect; c.light_threshold = rp->light_threshold;
-    c.light_count = sc->light_count; c.light_nodes = sc->light_nodes; c.material_count = sc->material_count;
-    c.pixels = pixels;
+    const Paths P = path_layout(st, cap);
+    const Scene S = scene_tables(sc);
+    Consts c = scene_consts(sc, rp, guides != nullptr);
     c.minimum = minimum;
-    c.threshold = rp->adaptive_threshold > 0.f ? rp->adaptive_threshold : 0.01f;
-    c.guides = guides != nullptr;
-    c.camera = sc->camera;
-    c.sky = sc->sky;
     const uint32_t grid = std::min<uint32_t>(blocks(cap, kBlock), (uint32_t)dev->cu_count * VH_GRID_BLOCKS_PER_CU);
     int rc = ensure_overflow(st->overflow, grid * kBlock);
     if (rc) return rc;
@@ -1420,6 +1515,136 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, float* g
     sc->stats.device_ms = ms;
     if (h[8] & kStackOverflow) return fail(VH_ERR_DEVICE, "traversal stack overflow during render");
     if (adaptive) sc->moments_next = rp->sample_end;
+    return VH_OK;
+}
+
+
+// Several jobs in one wavefront (vh_render_shared): every batch gives each job
+// with samples left a share of the path slots; the jobs' camera rays fill one
+// extension queue, their hits one shading queue and their shadow rays one
+// shadow queue, each path finding its job by its slot.  A path's arithmetic
+// does not depend on the batch it is in, and each job's film sums its samples
+// in sample order, so each film equals the job rendered alone.
+int render_shared(const vh_render_job* jobs, uint32_t count, hipStream_t s) {
+    vh_device* dev = jobs[0].scene->device;
+    const size_t cap = std::min<size_t>((size_t)1 << VH_BATCH_PATHS_LOG2, (size_t)1 << 27);
+    for (uint32_t j = 0; j < count; j++) {
+        const vh_render_params& rp = jobs[j].params;
+        if ((size_t)rp.width * rp.height > cap)
+            return fail(VH_ERR_USAGE, "vh_render_shared: job " + std::to_string(j) + " has more pixels than a batch");
+        jobs[j].scene->stats = vh_render_stats{};
+    }
+    StreamState* st = stream_state(dev, s);
+    std::lock_guard<std::mutex> hold(st->lock);
+    VH_HIP(reserve_wavefront(st, cap));
+    VH_HIP(st->counters.reserve(kCounterWords * sizeof(uint32_t)));
+    VH_HIP(st->launch_args.reserve(sizeof(LaunchArgs)));
+    VH_HIP(st->jobs.reserve(kMaxJobs * sizeof(JobArgs)));
+    const uint32_t grid = std::min<uint32_t>(blocks(cap, kBlock), (uint32_t)dev->cu_count * VH_GRID_BLOCKS_PER_CU);
+    int rc = ensure_overflow(st->overflow, grid * kBlock);
+    if (rc) return rc;
+    uint32_t* counters = st->counters.as<uint32_t>();
+    VH_HIP(hipMemsetAsync(counters, 0, kCounterWords * sizeof(uint32_t), s));
+    LaunchArgs args;
+    std::memset(&args, 0, sizeof(args));
+    args.P = path_layout(st, cap);
+    args.rays[0] = st->rays_a.as<float4>();
+    args.rays[1] = st->rays_b.as<float4>();
+    args.shadow_a = st->shadow_rays.as<float4>();
+    args.shadow_b = args.shadow_a + 2 * cap;
+    args.counters = counters;
+    args.overflow = st->overflow.as<uint32_t>();
+    args.cap = (uint32_t)cap;
+    args.jobs = st->jobs.as<JobArgs>();
+    // the first job's tables stand in for the kernel-wide ones the shared
+    // kernels do not read
+    args.S = scene_tables(jobs[0].scene);
+    args.c = scene_consts(jobs[0].scene, &jobs[0].params, false);
+    LaunchArgs* dargs = st->launch_args.as<LaunchArgs>();
+    std::vector<uint32_t> next(count);
+    for (uint32_t j = 0; j < count; j++) next[j] = jobs[j].params.sample_begin;
+    std::vector<std::vector<JobArgs>> tables;  // kept until the copies are done
+    hipEvent_t e0, e1;
+    VH_HIP(hipEventCreate(&e0));
+    VH_HIP(hipEventCreate(&e1));
+    VH_HIP(hipEventRecord(e0, s));
+    while (true) {
+        std::vector<uint32_t> live;
+        for (uint32_t j = 0; j < count; j++)
+            if (next[j] < jobs[j].params.sample_end) live.push_back(j);
+        if (live.empty()) break;
+        // an equal share of the slots per job, at least one sample each while
+        // they fit; jobs that do not fit wait for the next batch
+        const size_t share = cap / live.size();
+        std::vector<JobArgs> table;
+        size_t used = 0;
+        uint32_t tiles = 0, bounces = 0;
+        for (uint32_t j : live) {
+            const vh_render_params& rp = jobs[j].params;
+            const size_t pixels = (size_t)rp.width * rp.height;
+            uint32_t nb = (uint32_t)std::min<size_t>(rp.sample_end - next[j], std::max<size_t>(1, share / pixels));
+            while (nb > 0 && used + pixels * nb > cap) nb--;
+            if (nb == 0 || table.size() == kMaxJobs) continue;
+            JobArgs J;
+            std::memset(&J, 0, sizeof(J));
+            J.c = scene_consts(jobs[j].scene, &rp, jobs[j].guides != nullptr);
+            J.c.minimum = rp.samples;
+            J.c.batch_first = next[j];
+            J.c.batch_samples = nb;
+            J.c.group = sample_group(nb);
+            J.S = scene_tables(jobs[j].scene);
+            J.film = reinterpret_cast<float4*>(jobs[j].film);
+            J.guides = reinterpret_cast<float4*>(jobs[j].guides);
+            J.path_base = (uint32_t)used;
+            J.paths = (uint32_t)(pixels * nb);
+            J.tile_base = tiles;
+            J.tiles = blocks(pixels, kBlock);
+            table.push_back(J);
+            used += pixels * nb;
+            tiles += J.tiles;
+            bounces = std::max(bounces, J.c.max_bounces + 1);
+            jobs[j].scene->stats.paths += pixels * nb;
+            next[j] += nb;
+        }
+        tables.push_back(std::move(table));
+        const std::vector<JobArgs>& t = tables.back();
+        VH_HIP(hipMemcpyAsync(st->jobs.ptr, t.data(), t.size() * sizeof(JobArgs), hipMemcpyHostToDevice, s));
+        args.n = (uint32_t)used;
+        args.job_count = (uint32_t)t.size();
+        hipLaunchKernelGGL(k_store_args, dim3(1), dim3(kBlock), 0, s, dargs, args);
+        const LaunchArgs* A = dargs;
+        const dim3 g(grid), block(kBlock), one(1), wave(64);
+        hipLaunchKernelGGL(k_camera_shared, dim3(blocks(used, kBlock)), block, 0, s, A);
+        for (uint32_t bounce = 0; bounce < bounces; bounce++) {
+            hipLaunchKernelGGL((k_extend<false, true>), g, block, 0, s, A, bounce);
+            hipLaunchKernelGGL(k_shade<true>, g, block, 0, s, A, bounce);
+            hipLaunchKernelGGL((k_shadow<false, true>), g, block, 0, s, A);
+            hipLaunchKernelGGL(k_resolve, g, block, 0, s, A);
+            hipLaunchKernelGGL(k_swap_counts, one, wave, 0, s, A);
+        }
+        hipLaunchKernelGGL(k_film_shared, dim3(std::min<uint32_t>(tiles, grid)), block, 0, s, A);
+        VH_HIP(hipGetLastError());
+    }
+    VH_HIP(hipEventRecord(e1, s));
+    uint32_t h[kCounterWords];
+    VH_HIP(hipMemcpyAsync(h, counters, sizeof(h), hipMemcpyDeviceToHost, s));
+    VH_HIP(hipStreamSynchronize(s));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    uint64_t ext, shd;
+    std::memcpy(&ext, &h[4], 8);
+    std::memcpy(&shd, &h[6], 8);
+    for (uint32_t j = 0; j < count; j++) {
+        // rays, drops and time belong to the shared batches as a whole
+        vh_render_stats& stj = jobs[j].scene->stats;
+        stj.extension_rays = ext;
+        stj.shadow_rays = shd;
+        stj.dropped_samples = h[9];
+        stj.device_ms = ms;
+    }
+    if (h[8] & kStackOverflow) return fail(VH_ERR_DEVICE, "traversal stack overflow during render");
     return VH_OK;
 }
 

@@ -77,12 +77,12 @@ __global__ void k_clear_words(uint32_t* w, uint32_t n) {
 // allocates nothing: 256 bytes of counters ([0] error flags, [1] ray claims),
 // then the traversal overflow stacks of a grid of at most kStaticBlocks
 // workgroups (kGlobalStack 8-byte entries per thread, the same depth as the
-// allocating entry point, so both overflow on the same rays).  128
-// workgroups x 256 threads x 244 entries x 8 B = 64 MB per GPU, held by every
+// allocating entry point, so both overflow on the same rays).  32768
+// threads x 244 entries x 8 B = 64 MB per GPU, held by every
 // process that loads the library until it exits.  Host code never uses this
 // name as an address: each device's copy is looked up once with
 // hipGetSymbolAddress and the kernels receive that pointer.
-constexpr uint32_t kStaticBlocks = 128;
+constexpr uint32_t kStaticBlocks = 128 * 256 / kTraceBlock;  // 32768 threads whatever the workgroup size
 constexpr size_t kStaticCounterBytes = 256;
 constexpr size_t kStaticWords =
     kStaticCounterBytes / sizeof(uint64_t) + (size_t)kStaticBlocks * kTraceBlock * kGlobalStack;

@@ -375,7 +375,10 @@ struct Tracer {
 //   void unpack(uint32_t i, const RayData& r, f3& o, f3& d, float& tmin, float& tmax, uint32_t& mask,
 //               uint32_t& key) const;
 //   void store(uint32_t key, const TraceHit& h, const TraceCounts& c) const;
-// `next` is a global counter of claimed ray indices, zero at launch.
+//   TlasView view(uint32_t key) const;            // with kPerRay: the ray's own TLAS
+// `next` is a global counter of claimed ray indices, zero at launch.  With
+// kPerRay every ray traverses the TLAS the source names for it (the rays of
+// several scenes in one queue); otherwise all traverse `tv`.
 //
 // Lanes that finish wait until kRefillMin lanes of the wave are idle, then
 // claim consecutive rays with one atomic per wave.  All waves claim from the
@@ -385,10 +388,13 @@ struct Tracer {
 // lane takes one traversal step.  Every lane leaves the outer loop once the
 // queue is drained and its ray is stored.  Returns false when a stack
 // overflowed.
-template <bool kAny, bool kCount, class Src>
-__device__ bool trace_persistent(const TlasView& tv, const Src& src, uint32_t* next, Stack& st) {
+template <bool kAny, bool kCount, class Src, bool kPerRay = false>
+__device__ bool trace_persistent(const TlasView& tv0, const Src& src, uint32_t* next, Stack& st) {
     const uint32_t n = src.count();
     Tracer<kAny, kCount> tr;
+    TlasView lane_tv;
+    if (kPerRay) lane_tv = tv0;
+    const TlasView& tv = kPerRay ? lane_tv : tv0;
     bool active = false, exhausted = false, ok = true;
     uint32_t key = 0;
     const uint64_t lt_mask = (1ull << __lane_id()) - 1ull;
@@ -418,6 +424,7 @@ __device__ bool trace_persistent(const TlasView& tv, const Src& src, uint32_t* n
                     float t0, t1;
                     uint32_t m;
                     src.unpack(idx, rd, o, d, t0, t1, m, key);
+                    if constexpr (kPerRay) lane_tv = src.view(key);
                     tr.begin(tv, o, d, t0, t1, m, st);
                     active = !tr.done;
                     if (!active) src.store(key, tr.hit, tr.cnt);

@@ -201,6 +201,88 @@ class Denoiser:
             pass
 
 
+class Trainer:
+    """The denoiser's hand-written training kernels on a device
+    (vh_trainer_*): forward pass, loss, backward pass and Adam in HIP."""
+
+    def __init__(self, device, blob=None, seed=0):
+        from . import native
+        self.native = native
+        self.device = device
+        self._blob = C.create_string_buffer(blob, len(blob)) if blob is not None else None
+        self.handle = C.c_void_p()
+        native.check(native.lib().vh_trainer_create(device.handle, self._blob, len(blob) if blob else 0, seed,
+                                                    C.byref(self.handle)))
+
+    def step(self, films, guides, targets, samples, lr, update=True, stream=None):
+        """One step over a batch: torch CUDA tensors films (N, H, W, 4) and
+        guides (N, H, W, 2, 4) of `samples` samples, targets (N, H, W, 3) mean
+        radiance.  Returns the loss."""
+        native = self.native
+        n, h, w = films.shape[:3]
+        for t in (films, guides, targets):
+            assert t.is_cuda and t.is_contiguous() and t.dtype == _torch().float32
+        loss = C.c_float()
+        native.check(native.lib().vh_trainer_step(
+            self.handle, n, w, h, float(samples), native._dev_ptr(films), native._dev_ptr(guides),
+            native._dev_ptr(targets), float(lr), 0 if update else 1, C.c_void_p(stream) if stream else None,
+            C.byref(loss)))
+        return loss.value
+
+    def gradients(self):
+        """(weights, biases) fp32 gradients of the last step, blob order."""
+        convs = make_unet().convs()
+        nw = sum(9 * (32 if k == 0 else m.in_channels) * (16 if k == len(convs) - 1 else m.out_channels)
+                 for k, m in enumerate(convs))
+        nb = sum(16 if k == len(convs) - 1 else m.out_channels for k, m in enumerate(convs))
+        gw, gb = np.zeros(nw, np.float32), np.zeros(nb, np.float32)
+        self.native.check(self.native.lib().vh_trainer_gradients(self.handle, gw.ctypes.data, nw, gb.ctypes.data,
+                                                                 nb))
+        return gw, gb
+
+    def export(self):
+        native = self.native
+        n = C.c_size_t()
+        native.check(native.lib().vh_trainer_export(self.handle, None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        native.check(native.lib().vh_trainer_export(self.handle, buf, n.value, C.byref(n)))
+        return buf.raw
+
+    def close(self):
+        if self.handle:
+            self.native.lib().vh_trainer_release(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def split_gradients(gw, gb):
+    """Per layer (weight gradient (cout, cin, 3, 3) at padded sizes, bias
+    gradient) from the blob-order arrays of Trainer.gradients."""
+    out, ow, ob = [], 0, 0
+    convs = make_unet().convs()
+    for k, m in enumerate(convs):
+        ci = 32 if k == 0 else m.in_channels
+        co = 16 if k == len(convs) - 1 else m.out_channels
+        w = gw[ow:ow + 9 * ci * co].reshape(3, 3, co, ci).transpose(2, 3, 0, 1)
+        out.append((w, gb[ob:ob + co]))
+        ow += 9 * ci * co
+        ob += co
+    return out
+
+
+def loss_fn(pred, target):
+    """The training loss: relative MSE of the radiance plus L1 of log1p, the
+    kernels' k_loss in torch (pred (N, 3, H, W) log radiance)."""
+    torch = _torch()
+    d = radiance(pred)
+    return ((d - target) ** 2 / (target ** 2 + 1e-2)).mean() + (pred - torch.log1p(target)).abs().mean()
+
+
 def flops(width, height):
     """Multiply-adds x 2 of one denoise (the padded channel counts the kernels run)."""
     h, w = [height], [width]

@@ -29,6 +29,8 @@ EXPORTS = [
     "vh_trace_closest", "vh_trace_any", "vh_trace_closest_counts", "vh_scene_create", "vh_scene_set_camera",
     "vh_scene_release", "vh_render_params_default", "vh_render", "vh_render_guides", "vh_render_get_stats",
     "vh_denoiser_create", "vh_denoise", "vh_denoiser_release", "vh_sampler_bits", "vh_trace_closest_static",
+    "vh_trainer_create", "vh_trainer_step", "vh_trainer_gradients", "vh_trainer_export", "vh_trainer_release",
+    "vh_render_shared",
 ]
 
 _lib = None
@@ -86,6 +88,8 @@ def lib():
     L.vh_render_params_default.argtypes = [C.POINTER(abi.RenderParams), u32, u32, u32]
     L.vh_render.restype = i32
     L.vh_render.argtypes = [vp, C.POINTER(abi.RenderParams), u32, vp, vp]
+    L.vh_render_shared.restype = i32
+    L.vh_render_shared.argtypes = [vp, u32, vp]
     L.vh_render_guides.restype = i32
     L.vh_render_guides.argtypes = [vp, C.POINTER(abi.RenderParams), u32, vp, vp, vp]
     L.vh_denoiser_create.restype = i32
@@ -96,6 +100,16 @@ def lib():
     L.vh_denoiser_release.argtypes = [vp]
     L.vh_sampler_bits.restype = i32
     L.vh_sampler_bits.argtypes = [vp, vp, u32, u32, u32, u32, vp, vp]
+    L.vh_trainer_create.restype = i32
+    L.vh_trainer_create.argtypes = [vp, vp, C.c_size_t, u32, vp]
+    L.vh_trainer_step.restype = i32
+    L.vh_trainer_step.argtypes = [vp, u32, u32, u32, C.c_float, vp, vp, vp, C.c_float, u32, vp, vp]
+    L.vh_trainer_gradients.restype = i32
+    L.vh_trainer_gradients.argtypes = [vp, vp, C.c_size_t, vp, C.c_size_t]
+    L.vh_trainer_export.restype = i32
+    L.vh_trainer_export.argtypes = [vp, vp, C.c_size_t, vp]
+    L.vh_trainer_release.restype = None
+    L.vh_trainer_release.argtypes = [vp]
     L.vh_render_get_stats.restype = i32
     L.vh_render_get_stats.argtypes = [vp, C.POINTER(abi.RenderStats)]
     _lib = L
@@ -317,6 +331,20 @@ class InstalledScene:
             self.close()
         except Exception:
             pass
+
+
+def render_shared(jobs, stream=None):
+    """Renders several jobs with their paths in one wavefront
+    (vh_render_shared): `jobs` is a list of (InstalledScene, params, film,
+    guides or None), films and guides torch CUDA tensors that the samples are
+    added to."""
+    arr = (abi.RenderJob * len(jobs))()
+    for k, (inst, p, film, guides) in enumerate(jobs):
+        arr[k].scene = inst.handle.value
+        arr[k].params = p
+        arr[k].film = film.data_ptr()
+        arr[k].guides = guides.data_ptr() if guides is not None else None
+    check(lib().vh_render_shared(arr, len(jobs), C.c_void_p(stream) if stream else None))
 
 
 class Renderer:

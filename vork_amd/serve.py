@@ -17,6 +17,15 @@ resident scenes no admitted request uses (least recently used first): its
 film, its scene if not resident, and the growth of the slot's stream scratch
 (kept at its high-water mark, as the library keeps it).
 
+With `shared=True` the admitted requests instead share one wavefront: a
+single loop repeatedly hands every admitted request's next slice of samples
+(an equal share of one batch's path slots) to vh_render_shared, whose
+extension, shading and shadow queues hold the paths of all of them, each
+path traversing and shading its own scene; requests arriving meanwhile join
+at the next call, and a request is done once its last slice is.  Adaptive
+requests, which vh_render_shared does not take, render alone in the same
+loop.
+
 `Server(backend, ...)` holds the scheduling; `GpuBackend` installs and renders
 through vork_amd.native.  A request's latency is finish - submit; the server
 reports p50/p99 latency beside throughput (samples per second over the run).
@@ -87,14 +96,19 @@ class _Scene:
 
 
 class Server:
-    def __init__(self, backend, slots=4, budget=None):
+    def __init__(self, backend, slots=4, budget=None, shared=False):
         self.backend = backend
+        self.shared = shared
+        if shared and not 1 <= slots <= 16:
+            raise ValueError("a shared wavefront takes 1 to 16 requests at once")
         self.slots = slots
         self.budget = budget if budget is not None else backend.memory_budget()
         self.cv = threading.Condition()
         self.queue = collections.deque()
         self.free = list(range(slots))
-        self.base_bytes = backend.slot_base_bytes() * slots     # the streams' traversal overflow stacks
+        # the streams' traversal overflow stacks (shared: one stream and its whole-batch wavefront state)
+        self.base_bytes = (backend.slot_base_bytes() + backend.shared_state_bytes() if shared
+                           else backend.slot_base_bytes() * slots)
         self.slot_bytes = [0] * slots                           # wavefront scratch high-water mark per slot
         self.resident = collections.OrderedDict()               # key -> _Scene, least recently used first
         self.active = {}                                        # rid -> Request
@@ -103,7 +117,10 @@ class Server:
         self.next_id = 0
         self.running = True
         self.work = [collections.deque() for _ in range(slots)]
-        self.threads = [threading.Thread(target=self._slot_loop, args=(k,), daemon=True) for k in range(slots)]
+        if shared:
+            self.threads = [threading.Thread(target=self._shared_loop, daemon=True)]
+        else:
+            self.threads = [threading.Thread(target=self._slot_loop, args=(k,), daemon=True) for k in range(slots)]
         for t in self.threads:
             t.start()
 
@@ -176,8 +193,8 @@ class Server:
         return self.backend.scene_bytes(req.scene if req.scene is not None else self._scene_of(req.key))
 
     def _need(self, req, slot_hw, scene_bytes):
-        return self._film_bytes(req) + max(0, self.backend.state_bytes(req.width, req.height, req.spp) - slot_hw) \
-            + scene_bytes
+        state = 0 if self.shared and not req.adaptive else self.backend.state_bytes(req.width, req.height, req.spp)
+        return self._film_bytes(req) + max(0, state - slot_hw) + scene_bytes
 
     @staticmethod
     def _film_bytes(req):
@@ -196,7 +213,8 @@ class Server:
             for req in list(self.queue):
                 if not self.free:
                     break
-                state = self.backend.state_bytes(req.width, req.height, req.spp)
+                state = 0 if self.shared and not req.adaptive else \
+                    self.backend.state_bytes(req.width, req.height, req.spp)
                 # prefer a slot whose scratch already holds this request
                 slot = min(self.free, key=lambda k: (self.slot_bytes[k] < state, -self.slot_bytes[k]))
                 sb = self._scene_bytes(req)
@@ -236,6 +254,105 @@ class Server:
         return True
 
     # -------------------------------------------------------------- slots
+    def _ensure_installed(self, req):
+        """Installs the request's scene unless it is (or is being) installed;
+        raises the install's error."""
+        with self.cv:
+            entry = self.resident[req.key]
+            install = not entry.installing and not entry.ready.is_set()
+            if install:
+                entry.installing = True
+        if install:
+            try:
+                nbytes = self.backend.install(req.key, entry.scene)
+                with self.cv:
+                    entry.bytes = nbytes
+            except Exception as e:  # the install failed: every request of the scene fails with it
+                entry.error = e
+            finally:
+                entry.ready.set()
+        entry.ready.wait()
+        if entry.error is not None:
+            raise entry.error
+        return entry
+
+    def _finish(self, req, seat):
+        req.finished = time.perf_counter()
+        with self.cv:
+            entry = self.resident[req.key]
+            entry.users -= 1
+            if entry.error is not None and entry.users == 0 and self.resident.get(req.key) is entry:
+                del self.resident[req.key]
+            del self.active[req.rid]
+            self.finished.append(req)
+            self.free.append(seat)
+            self._admit()
+            self.cv.notify_all()
+        req.done.set()
+
+    def _shared_loop(self):
+        """Every admitted request's next slice of samples in one
+        vh_render_shared call, again and again; arrivals join the next call."""
+        done_samples = {}
+        while True:
+            with self.cv:
+                while self.running and not self.active:
+                    self.cv.wait()
+                if not self.active:
+                    return
+                batch = sorted(self.active.values(), key=lambda r: r.rid)
+                for req in batch:
+                    for q in self.work:
+                        if req in q:
+                            q.remove(req)
+            jobs = []
+            for req in batch:
+                try:
+                    self._ensure_installed(req)
+                except Exception as e:
+                    req.error = e
+                    self._finish(req, req.slot)
+                    continue
+                if req.started is None:
+                    req.started = time.perf_counter()
+                if req.adaptive:   # vh_render_shared renders plain sampling only
+                    try:
+                        entry = self.resident[req.key]
+                        req.film, req.stats = self.backend.render(0, req.key, entry.scene, req.width, req.height,
+                                                                  req.spp, req.seed, True, req.denoise)
+                    except Exception as e:
+                        req.error = e
+                    self._finish(req, req.slot)
+                    continue
+                jobs.append(req)
+            if not jobs:
+                continue
+            share = self.backend.shared_batch_paths() // len(jobs)
+            items = []
+            for req in jobs:
+                s0 = done_samples.get(req.rid, 0)
+                s1 = min(req.spp, s0 + max(1, share // (req.width * req.height)))
+                items.append((req, s0, s1))
+            try:
+                self.backend.render_shared([(req.slot, req.key, self.resident[req.key].scene, req.rid, req.width,
+                                             req.height, req.spp, req.seed, s0, s1, req.denoise)
+                                            for req, s0, s1 in items])
+            except Exception as e:
+                for req, _, _ in items:
+                    req.error = e
+                    self.backend.drop_shared(req.rid)
+                    self._finish(req, req.slot)
+                continue
+            for req, s0, s1 in items:
+                done_samples[req.rid] = s1
+                if s1 == req.spp:
+                    del done_samples[req.rid]
+                    try:
+                        req.film, req.stats = self.backend.finish_shared(req.rid, req.spp, req.denoise)
+                    except Exception as e:
+                        req.error = e
+                    self._finish(req, req.slot)
+
     def _slot_loop(self, k):
         while True:
             with self.cv:
@@ -297,6 +414,7 @@ class GpuBackend:
         self.scenes = {}
         self.lock = threading.Lock()
         self.denoisers = [None] * slots
+        self.shared_films = {}   # request id -> (film, guides or None) on the device, summed over slices
 
     def memory_budget(self):
         return int(0.85 * self.total)
@@ -307,6 +425,13 @@ class GpuBackend:
 
     def state_bytes(self, width, height, spp):
         return batch_paths(width, height, spp) * PATH_BYTES
+
+    def shared_batch_paths(self):
+        """Path slots of one vh_render_shared batch."""
+        return min(BATCH_PATHS, 1 << 27)
+
+    def shared_state_bytes(self):
+        return self.shared_batch_paths() * PATH_BYTES
 
     def scene_bytes(self, scene):
         tris = scene.triangle_count if not isinstance(scene.instances, np.ndarray) else \
@@ -356,6 +481,45 @@ class GpuBackend:
             with self.lock:
                 entry["views"][(slot, width, height)] = v
         return v
+
+    def render_shared(self, items):
+        """Samples [s0, s1) of each item (seat, key, scene, request id, width,
+        height, spp, seed, s0, s1, denoise) in one vh_render_shared call,
+        added to the request's device film (and guides when it is denoised)."""
+        torch = self.torch
+        dev = f"cuda:{self.ordinal}"
+        stream = self.streams[0]
+        jobs = []
+        with torch.cuda.stream(stream):
+            for seat, key, scene, rid, width, height, spp, seed, s0, s1, denoise in items:
+                view = self._view(seat, key, scene, width, height)
+                if rid not in self.shared_films:
+                    film = torch.zeros((height, width, 4), dtype=torch.float32, device=dev)
+                    guides = torch.zeros((height, width, 2, 4), dtype=torch.float32, device=dev) if denoise else None
+                    self.shared_films[rid] = (film, guides, seat, view)
+                film, guides, _, _ = self.shared_films[rid]
+                p = view.scene.render_params(samples=spp, seed=seed, adaptive=False)
+                p.sample_begin, p.sample_end = s0, s1
+                jobs.append((view, p, film, guides))
+            self.native.render_shared(jobs, stream=stream.cuda_stream)
+
+    def finish_shared(self, rid, spp, denoise):
+        """The request's result (film sums, or the denoised mean radiance) and
+        its view's stats; frees its device film."""
+        torch = self.torch
+        film, guides, seat, view = self.shared_films.pop(rid)
+        stream = self.streams[0]
+        with torch.cuda.stream(stream):
+            if not denoise:
+                return film.cpu().numpy(), view.stats()
+            from .denoise import Denoiser
+            if self.denoisers[0] is None:
+                self.denoisers[0] = Denoiser(self.renderer.device)
+            out = self.denoisers[0](film, guides, spp, stream=stream.cuda_stream)
+            return out.cpu().numpy(), view.stats()
+
+    def drop_shared(self, rid):
+        self.shared_films.pop(rid, None)
 
     def render(self, slot, key, scene, width, height, spp, seed, adaptive, denoise=False):
         torch = self.torch
