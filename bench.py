@@ -47,8 +47,8 @@ HBM_PEAK_GBPS = 8000.0
 # HBM-side traffic of the traced kernels, measured by this script's own
 # rocprofv3 children (FETCH_SIZE, then WRITE_SIZE, each over one render step
 # and over a plain device copy of known size that scales the counter to bytes)
-TRAFFIC_KERNELS = {"k_extend_packet": "k_extend_packet", "k_extend": "k_extend<false>", "k_shade": "k_shade",
-                   "k_shadow": "k_shadow<false>"}
+TRAFFIC_KERNELS = {"k_extend_packet": "k_extend_packet", "k_extend": "k_extend<false", "k_shade": "k_shade<",
+                   "k_shadow": "k_shadow<false"}
 COPY_MIB = 1024
 
 

@@ -420,8 +420,8 @@ class GpuBackend:
         return int(0.85 * self.total)
 
     def slot_base_bytes(self):
-        # traversal overflow stacks of one stream: 8 workgroups per CU x 256 lanes x 244 8-byte entries
-        return self.cus * 8 * 256 * 244 * 8
+        # traversal overflow stacks of one stream: the persistent grid's threads x 244 8-byte entries
+        return self.cus * _tuned("VH_GRID_BLOCKS_PER_CU") * _tuned("VH_BLOCK") * 244 * 8
 
     def state_bytes(self, width, height, spp):
         return batch_paths(width, height, spp) * PATH_BYTES
