@@ -28,8 +28,11 @@
  *    grows it to the largest call and frees it with the device.  Calls on
  *    one stream wait for each other; calls on different streams run
  *    concurrently from several threads and their kernels overlap on the
- *    device.  vh_render calls on one scene also wait for each other (the
- *    scene holds its adaptive moments and render stats).
+ *    device.  A scene is not written by rendering it: renders of one scene
+ *    on different streams run at once (an adaptive exposure in progress
+ *    keeps its moments per scene and stream), and vh_render_get_stats reads
+ *    the stats of the last render of the scene that finished.
+ *    vh_scene_set_camera must not overlap a render of its scene.
  */
 #ifndef VORK_HIP_H
 #define VORK_HIP_H
@@ -300,6 +303,9 @@ typedef struct vh_scene_desc {
  * and sky: the resident part of InstalledScene (crates/vork-render/src/
  * install.rs:401) that the path pass reads. */
 int vh_scene_create(vh_device* device, const vh_tlas* tlas, const vh_scene_desc* desc, vh_scene** out);
+/* Replaces the scene's camera for the renders that start after it returns
+ * (a camera that moves over one installed scene);
+ * it must not overlap a render of the scene. */
 int vh_scene_set_camera(vh_scene* scene, const vh_camera* camera);
 void vh_scene_release(vh_scene* scene);
 
@@ -390,8 +396,9 @@ typedef struct vh_render_job {
  * and every path traverses and shades with its own job's scene.  Each film
  * equals what vh_render_guides gives for the job alone.  This is the shared
  * wavefront queue of the serving loop; the reference serves one render at a
- * time (crates/vork-cli/src/session.rs:1-7).  Each job's scene stats get its
- * paths; rays and device time are those of the whole call. */
+ * time (crates/vork-cli/src/session.rs:1-7).  A scene may appear in several
+ * jobs.  Each job's scene stats get its paths (the last job's, for a scene
+ * in several); rays and device time are those of the whole call. */
 int vh_render_shared(const vh_render_job* jobs, uint32_t count, void* stream);
 
 /* ---------------------------------------------------------------- denoiser */

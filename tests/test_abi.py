@@ -44,3 +44,12 @@ def test_usage_errors_need_no_device():
     assert b"null" in buf.value
     assert L.vh_render(None, None, 0, None, None) == abi.VH_ERR_USAGE
     del out
+
+
+def test_rust_binding_in_the_integration_notes_covers_every_entry_point():
+    import os
+    import re
+    here = os.path.dirname(os.path.abspath(__file__))
+    text = open(os.path.join(here, "..", "INTEGRATION.md")).read()
+    bound = set(re.findall(r"pub fn (vh_\w+)\(", text))
+    assert bound == set(declared_symbols())

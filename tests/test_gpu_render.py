@@ -377,3 +377,50 @@ def test_shared_wavefront_queue_films_equal_each_job_alone(renderer):
             assert torch.equal(g, g1)
         assert inst.stats()["paths"] == p.width * p.height * (p.sample_end - p.sample_begin)
     assert float(jobs[-1][2][..., 3].min()) == 64.0
+
+
+def test_one_scene_renders_on_two_streams_at_once(renderer):
+    """A scene is not written by rendering it: two threads, each with its own
+    stream, render one scene at once, each an adaptive exposure split into
+    sample ranges (the moments of an exposure in progress belong to the scene
+    and stream), and a shared call may name the scene twice.  Every film
+    equals the same render made alone."""
+    import threading
+    torch = pytest.importorskip("torch")
+    from vork_amd import native
+    sc = S.config0(width=64, height=48)
+    inst = renderer.install(sc)
+
+    def exposure(seed, cut, stream=None):
+        p1 = sc.render_params(samples=32, seed=seed)
+        p1.adaptive_threshold, p1.sample_end = 0.05, cut
+        p2 = sc.render_params(samples=32, seed=seed)
+        p2.adaptive_threshold, p2.sample_begin = 0.05, cut
+        return inst.render(p2, film=inst.render(p1, stream=stream), stream=stream)
+
+    alone = {seed: exposure(seed, 12) for seed in (3, 4)}
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    got, errors = {}, []
+
+    def worker(k, seed):
+        try:
+            for rep in range(4):
+                got[(seed, rep)] = exposure(seed, 12, streams[k].cuda_stream)
+        except Exception as e:  # reported by the main thread
+            errors.append(e)
+
+    threads = [threading.Thread(target=worker, args=(k, seed)) for k, seed in enumerate((3, 4))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors
+    for (seed, rep), f in got.items():
+        assert np.array_equal(f, alone[seed]), (seed, rep)
+    # one scene in two jobs of one shared wavefront
+    plain = sc.render_params(samples=8, seed=6, adaptive=False)
+    films = [torch.zeros((sc.height, sc.width, 4), dtype=torch.float32, device="cuda") for _ in range(2)]
+    native.render_shared([(inst, plain, films[0], None), (inst, plain, films[1], None)])
+    torch.cuda.synchronize()
+    ref = inst.render(plain)
+    assert np.array_equal(films[0].cpu().numpy(), ref) and np.array_equal(films[1].cpu().numpy(), ref)

@@ -616,7 +616,6 @@ int vh_render(vh_scene* sc, const vh_render_params* p, uint32_t flags, void* str
 int vh_render_shared(const vh_render_job* jobs, uint32_t count, void* stream) {
     clear();
     if (!jobs || count == 0 || count > 16) return fail(VH_ERR_USAGE, "vh_render_shared: 1 to 16 jobs");
-    std::vector<vh_scene*> scenes;
     for (uint32_t j = 0; j < count; j++) {
         const vh_render_job& jb = jobs[j];
         const vh_render_params* p = &jb.params;
@@ -629,17 +628,10 @@ int vh_render_shared(const vh_render_job* jobs, uint32_t count, void* stream) {
                                       "sample range outside its exposure");
         if (p->flags & VH_RENDER_ADAPTIVE)
             return fail(VH_ERR_USAGE, "vh_render_shared: adaptive sampling is rendered by vh_render");
-        for (vh_scene* q : scenes)
-            if (q == jb.scene) return fail(VH_ERR_USAGE, "vh_render_shared: a scene appears in two jobs");
-        scenes.push_back(jb.scene);
     }
     vh_device* dev = jobs[0].scene->device;
     VH_HIP(hipSetDevice(dev->ordinal));
     hipStream_t s = pick(dev, stream);
-    // the scenes' locks in address order, so that overlapping calls cannot deadlock
-    std::sort(scenes.begin(), scenes.end());
-    std::vector<std::unique_lock<std::mutex>> holds;
-    for (vh_scene* q : scenes) holds.emplace_back(q->render_lock);
     return render_shared(jobs, count, s);
 }
 
@@ -656,7 +648,6 @@ int vh_render_guides(vh_scene* sc, const vh_render_params* p, uint32_t flags, vo
     vh_device* dev = sc->device;
     VH_HIP(hipSetDevice(dev->ordinal));
     hipStream_t s = pick(dev, stream);
-    std::lock_guard<std::mutex> hold(sc->render_lock);
     if (flags & VH_DEVICE_POINTERS) return render_scene(sc, p, film, guides, s);
     size_t bytes = sizeof(float) * 4 * (size_t)p->width * p->height;
     DeviceBuffer f, g;
@@ -677,7 +668,7 @@ int vh_render_guides(vh_scene* sc, const vh_render_params* p, uint32_t flags, vo
 int vh_render_get_stats(const vh_scene* sc, vh_render_stats* stats) {
     clear();
     if (!sc || !stats) return fail(VH_ERR_USAGE, "vh_render_get_stats: null argument");
-    std::lock_guard<std::mutex> hold(const_cast<vh_scene*>(sc)->render_lock);
+    std::lock_guard<std::mutex> hold(const_cast<vh_scene*>(sc)->state_lock);
     *stats = sc->stats;
     return VH_OK;
 }

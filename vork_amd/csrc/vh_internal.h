@@ -169,14 +169,19 @@ struct vh_scene {
     uint32_t material_count = 0, light_count = 0, light_nodes = 0;
     vh_camera camera{};
     vh_sky sky{};
-    // vh_render calls on one scene are serialized under render_lock (the
-    // adaptive moments and the stats below are the scene's own)
-    std::mutex render_lock;
-    // adaptive sampling moments (48 B per pixel) and the exposure they belong
-    // to: size, seed, sample count and the next sample index
-    vh::DeviceBuffer moments;
-    uint32_t moments_width = 0, moments_height = 0, moments_seed = 0, moments_samples = 0, moments_next = 0;
-    vh_render_stats stats{};
+    // The scene's tables are not written after vh_scene_create (but for
+    // vh_scene_set_camera), so renders of the scene on different streams run
+    // at once.  An adaptive exposure in progress keeps its per-pixel moments
+    // (48 B each) and its size, seed, sample count and next sample index per
+    // stream: calls on one stream are ordered, so the stream's own exposure
+    // needs no lock; `state_lock` guards the map and the stats.
+    struct Exposure {
+        vh::DeviceBuffer moments;
+        uint32_t width = 0, height = 0, seed = 0, samples = 0, next = 0;
+    };
+    std::mutex state_lock;
+    std::map<hipStream_t, std::unique_ptr<Exposure>> exposures;
+    vh_render_stats stats{};  // of the last render of the scene that finished
 };
 
 namespace vh {

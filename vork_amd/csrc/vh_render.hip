@@ -1328,8 +1328,12 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, float* g
     vh_device* dev = sc->device;
     const uint32_t pixels = rp->width * rp->height;
     const uint32_t total = rp->sample_end - rp->sample_begin;
-    sc->stats = vh_render_stats{};
-    if (total == 0) return VH_OK;
+    vh_render_stats stats{};
+    if (total == 0) {
+        std::lock_guard<std::mutex> hs(sc->state_lock);
+        sc->stats = stats;
+        return VH_OK;
+    }
     uint32_t batch = rp->batch_samples;
     if (batch == 0) {
         // Up to 2^VH_BATCH_PATHS_LOG2 paths in flight (262 B each; 2^27 is
@@ -1360,21 +1364,27 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, float* g
                                  : std::max<uint32_t>(16u, (uint32_t)std::sqrt((float)rp->samples)),
         rp->samples);
     Moment* moments = nullptr;
+    vh_scene::Exposure* ex = nullptr;
     if (adaptive) {
-        if (rp->sample_begin == 0) {
-            VH_HIP(sc->moments.reserve((size_t)pixels * sizeof(Moment)));
-            sc->moments_width = rp->width;
-            sc->moments_height = rp->height;
-            sc->moments_seed = rp->seed;
-            sc->moments_samples = rp->samples;
-        } else if (sc->moments_next != rp->sample_begin || sc->moments_width != rp->width ||
-                   sc->moments_height != rp->height || sc->moments_seed != rp->seed ||
-                   sc->moments_samples != rp->samples) {
-            return fail(VH_ERR_USAGE, "vh_render: an adaptive sample range must continue the exposure rendered "
-                                      "before it on this scene");
+        {
+            std::lock_guard<std::mutex> hs(sc->state_lock);
+            std::unique_ptr<vh_scene::Exposure>& e = sc->exposures[s];
+            if (!e) e.reset(new vh_scene::Exposure());
+            ex = e.get();  // used only under the stream's lock, which this call holds
         }
-        sc->moments_next = UINT32_MAX;  // set again once this range is rendered
-        moments = sc->moments.as<Moment>();
+        if (rp->sample_begin == 0) {
+            VH_HIP(ex->moments.reserve((size_t)pixels * sizeof(Moment)));
+            ex->width = rp->width;
+            ex->height = rp->height;
+            ex->seed = rp->seed;
+            ex->samples = rp->samples;
+        } else if (ex->next != rp->sample_begin || ex->width != rp->width || ex->height != rp->height ||
+                   ex->seed != rp->seed || ex->samples != rp->samples) {
+            return fail(VH_ERR_USAGE, "vh_render: an adaptive sample range must continue the exposure rendered "
+                                      "before it on this scene and stream");
+        }
+        ex->next = UINT32_MAX;  // set again once this range is rendered
+        moments = ex->moments.as<Moment>();
     }
     const Paths P = path_layout(st, cap);
     const Scene S = scene_tables(sc);
@@ -1491,30 +1501,34 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, float* g
     std::memcpy(&crays, &h[kCameraRays], 8);
     v[0] += cv[0];
     v[1] += cv[1];
-    sc->stats.extend_camera_ms = stage_ms[3];
-    sc->stats.extend_camera_launches = stage_n[3];
-    sc->stats.extend_camera_rays = crays;
-    sc->stats.extend_camera_node_visits = cv[0];
-    sc->stats.extend_camera_triangle_tests = cv[1];
-    sc->stats.extend_ms = stage_ms[0];
-    sc->stats.shade_ms = stage_ms[1];
-    sc->stats.shadow_ms = stage_ms[2];
-    sc->stats.other_ms = profile ? ms - stage_ms[0] - stage_ms[1] - stage_ms[2] : 0.0;
-    sc->stats.extend_launches = stage_n[0];
-    sc->stats.shade_launches = stage_n[1];
-    sc->stats.shadow_launches = stage_n[2];
-    sc->stats.extend_node_visits = v[0];
-    sc->stats.extend_triangle_tests = v[1];
-    sc->stats.shadow_node_visits = v[2];
-    sc->stats.shadow_triangle_tests = v[3];
-    sc->stats.paths = (uint64_t)pixels * total - stopped;
-    sc->stats.stopped_samples = stopped;
-    sc->stats.extension_rays = ext;
-    sc->stats.shadow_rays = shd;
-    sc->stats.dropped_samples = h[9];
-    sc->stats.device_ms = ms;
+    stats.extend_camera_ms = stage_ms[3];
+    stats.extend_camera_launches = stage_n[3];
+    stats.extend_camera_rays = crays;
+    stats.extend_camera_node_visits = cv[0];
+    stats.extend_camera_triangle_tests = cv[1];
+    stats.extend_ms = stage_ms[0];
+    stats.shade_ms = stage_ms[1];
+    stats.shadow_ms = stage_ms[2];
+    stats.other_ms = profile ? ms - stage_ms[0] - stage_ms[1] - stage_ms[2] : 0.0;
+    stats.extend_launches = stage_n[0];
+    stats.shade_launches = stage_n[1];
+    stats.shadow_launches = stage_n[2];
+    stats.extend_node_visits = v[0];
+    stats.extend_triangle_tests = v[1];
+    stats.shadow_node_visits = v[2];
+    stats.shadow_triangle_tests = v[3];
+    stats.paths = (uint64_t)pixels * total - stopped;
+    stats.stopped_samples = stopped;
+    stats.extension_rays = ext;
+    stats.shadow_rays = shd;
+    stats.dropped_samples = h[9];
+    stats.device_ms = ms;
     if (h[8] & kStackOverflow) return fail(VH_ERR_DEVICE, "traversal stack overflow during render");
-    if (adaptive) sc->moments_next = rp->sample_end;
+    {
+        std::lock_guard<std::mutex> hs(sc->state_lock);
+        sc->stats = stats;
+    }
+    if (adaptive) ex->next = rp->sample_end;
     return VH_OK;
 }
 
@@ -1532,8 +1546,8 @@ int render_shared(const vh_render_job* jobs, uint32_t count, hipStream_t s) {
         const vh_render_params& rp = jobs[j].params;
         if ((size_t)rp.width * rp.height > cap)
             return fail(VH_ERR_USAGE, "vh_render_shared: job " + std::to_string(j) + " has more pixels than a batch");
-        jobs[j].scene->stats = vh_render_stats{};
     }
+    std::vector<vh_render_stats> stats(count);
     StreamState* st = stream_state(dev, s);
     std::lock_guard<std::mutex> hold(st->lock);
     VH_HIP(reserve_wavefront(st, cap));
@@ -1603,7 +1617,7 @@ int render_shared(const vh_render_job* jobs, uint32_t count, hipStream_t s) {
             used += pixels * nb;
             tiles += J.tiles;
             bounces = std::max(bounces, J.c.max_bounces + 1);
-            jobs[j].scene->stats.paths += pixels * nb;
+            stats[j].paths += pixels * nb;
             next[j] += nb;
         }
         tables.push_back(std::move(table));
@@ -1638,11 +1652,12 @@ int render_shared(const vh_render_job* jobs, uint32_t count, hipStream_t s) {
     std::memcpy(&shd, &h[6], 8);
     for (uint32_t j = 0; j < count; j++) {
         // rays, drops and time belong to the shared batches as a whole
-        vh_render_stats& stj = jobs[j].scene->stats;
-        stj.extension_rays = ext;
-        stj.shadow_rays = shd;
-        stj.dropped_samples = h[9];
-        stj.device_ms = ms;
+        stats[j].extension_rays = ext;
+        stats[j].shadow_rays = shd;
+        stats[j].dropped_samples = h[9];
+        stats[j].device_ms = ms;
+        std::lock_guard<std::mutex> hs(jobs[j].scene->state_lock);
+        jobs[j].scene->stats = stats[j];
     }
     if (h[8] & kStackOverflow) return fail(VH_ERR_DEVICE, "traversal stack overflow during render");
     return VH_OK;
