@@ -67,6 +67,9 @@ def main():
                          "config3: Poisson stream of requests through the render server; "
                          "config4: 4 spp + learned denoiser vs plain 64 spp at 1080p")
     ap.add_argument("--ref-spp", type=int, default=4096, help="config4: samples of the reference render")
+    ap.add_argument("--weights", default=None, help="config4: denoiser weight blob (default: the shipped one)")
+    ap.add_argument("--train-batch", type=int, default=8, help="config4: crops per timed training step")
+    ap.add_argument("--train-crop", type=int, default=256, help="config4: crop size of the timed training step")
     ap.add_argument("--requests", type=int, default=40, help="config3: requests per arrival rate")
     ap.add_argument("--rates", default="4,8,inf", help="config3: arrival rates (requests/s), inf = all at once")
     ap.add_argument("--slots", type=int, default=4, help="config3: requests rendered at once")
@@ -124,7 +127,7 @@ def main():
         frames = None
     else:
         sc = S.config2(width=args.width, height=args.height, spp=args.spp or 4, birds=args.birds)
-        frames = S.flock_frames(sc, args.frames)
+        frames = S.flock_frames(sc, args.frames, progress=lambda k: k % 32 == 0 and log(f"{k} frames made"))
     log(f"scene: {sc.triangle_count} triangles generated in {time.time() - t0:.1f}s")
 
     dev = native.Device(0)
@@ -239,27 +242,34 @@ def main():
         per_launch = algorithmic / launches
         avg_ms = prof["extend_ms"] / launches
         achieved = per_launch / (avg_ms * 1e-3) / 1e9
-        # the camera bounce (k_extend_packet: coherent camera rays as wave
-        # packets) and the later bounces (k_extend: incoherent rays, per lane)
-        # priced apart; both by the per-ray visits of the per-lane traversal
+        # the camera bounce and the later bounces (k_extend: incoherent rays,
+        # per lane) priced apart, both by the per-ray visits of the per-lane
+        # traversal; the camera bounce runs as wave packets (k_extend_packet)
+        # when a wave's 64 rays are the samples of at most 4 pixels (16 or more
+        # samples of a pixel per batch), else per lane in k_extend too
+        group = 1
+        while group < 64 and sc.spp % (2 * group) == 0:
+            group *= 2
+        camera_kernel = "k_extend_packet" if group >= 16 else "k_extend"
         cam_bytes = (cnt["extend_camera_node_visits"] * S_NODE + cnt["extend_camera_triangle_tests"] * S_TRI
                      + cnt["extend_camera_rays"] * (S_RAYIN + S_HITOUT))
         parts = {}
         for name, nbytes, ms, n, rays, nodes, tris in (
-                ("k_extend_packet", cam_bytes, prof["extend_camera_ms"], prof["extend_camera_launches"],
+                ("camera bounce", cam_bytes, prof["extend_camera_ms"], prof["extend_camera_launches"],
                  cnt["extend_camera_rays"], cnt["extend_camera_node_visits"], cnt["extend_camera_triangle_tests"]),
-                ("k_extend", algorithmic - cam_bytes, prof["extend_ms"] - prof["extend_camera_ms"],
+                ("later bounces", algorithmic - cam_bytes, prof["extend_ms"] - prof["extend_camera_ms"],
                  launches - prof["extend_camera_launches"], cnt["extension_rays"] - cnt["extend_camera_rays"],
                  cnt["extend_node_visits"] - cnt["extend_camera_node_visits"],
                  cnt["extend_triangle_tests"] - cnt["extend_camera_triangle_tests"])):
             n = max(n, 1)
             gbps = nbytes / n / (ms / n * 1e-3) / 1e9 if ms > 0 else None
-            parts[name] = {"launches": n, "avg_launch_ms": round(ms / n, 4), "bytes_per_launch": int(nbytes / n),
+            parts[name] = {"kernel": camera_kernel if name == "camera bounce" else "k_extend", "launches": n, "avg_launch_ms": round(ms / n, 4), "bytes_per_launch": int(nbytes / n),
                            "rays": rays, "node_visits_per_ray": round(nodes / max(rays, 1), 2),
                            "triangle_tests_per_ray": round(tris / max(rays, 1), 2),
                            "achieved": round(gbps, 1) if gbps else None,
                            "frac": round(gbps / HBM_PEAK_GBPS, 4) if gbps else None}
-        roofline = {"kernel": "extend stage: k_extend_packet (camera rays) + k_extend (later bounces), closest-hit traversal", "bound": "hbm",
+        roofline = {"kernel": f"extend stage: {camera_kernel} (camera rays) + k_extend (later bounces), closest-hit "
+                              "traversal", "bound": "hbm",
                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
                     "bytes_per_launch": int(per_launch), "avg_launch_ms": round(avg_ms, 4), "launches": launches,
@@ -272,16 +282,17 @@ def main():
             tr = measure_traffic(args)
             roofline["traffic_measured"] = tr
             ext = tr.get("kernels", {})
-            if "k_extend_packet" in ext and "k_extend" in ext:
-                a, b = ext["k_extend_packet"], ext["k_extend"]
-                stage_bytes = a["bytes_per_launch"] * a["launches"] + b["bytes_per_launch"] * b["launches"]
-                roofline["traffic"] = int(stage_bytes / (a["launches"] + b["launches"]))
+            if "k_extend" in ext and (camera_kernel == "k_extend" or "k_extend_packet" in ext):
+                launches_all = sum(ext[k]["launches"] for k in {camera_kernel, "k_extend"})
+                stage_bytes = sum(ext[k]["bytes_per_launch"] * ext[k]["launches"] for k in {camera_kernel, "k_extend"})
+                roofline["traffic"] = int(stage_bytes / launches_all)
                 roofline["traffic_source"] = ("rocprofv3 FETCH_SIZE + WRITE_SIZE of this run's own children, "
                                               "per launch of the extend stage, copy-calibrated")
-                for k in ("k_extend_packet", "k_extend"):
-                    t = ext[k]["bytes_per_launch"]
-                    parts[k]["traffic_bytes_per_launch"] = t
-                    parts[k]["traffic_gbps"] = round(t / (parts[k]["avg_launch_ms"] * 1e-3) / 1e9, 1)
+                if camera_kernel == "k_extend_packet":   # the two kernels' traffic apart
+                    for part in parts.values():
+                        t = ext[part["kernel"]]["bytes_per_launch"]
+                        part["traffic_bytes_per_launch"] = t
+                        part["traffic_gbps"] = round(t / (part["avg_launch_ms"] * 1e-3) / 1e9, 1)
 
     cpu = None
     if not args.no_cpu_baseline and args.workload == "config1":
@@ -317,6 +328,8 @@ def main():
     }
     if gpu_frames is not None:
         line["tlas_build_ms_p50"] = round(float(np.percentile(tlas_ms, 50)), 2)
+        line["tlas_build_ms_p99"] = round(float(np.percentile(tlas_ms, 99)), 2)
+        line["frames"] = args.frames
     print(json.dumps(line), flush=True)
 
 
@@ -330,7 +343,7 @@ def denoise_bench(args):
     log(f"scene: {sc.triangle_count} triangles generated in {time.time() - t0:.1f}s")
     r = native.Renderer(0)
     inst = r.install(sc)
-    den = D.Denoiser(r.device)
+    den = D.Denoiser(r.device, open(args.weights, "rb").read() if args.weights else None)
     H, W = sc.height, sc.width
     film = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
     guides = torch.zeros((H, W, 2, 4), dtype=torch.float32, device="cuda")
@@ -374,6 +387,23 @@ def denoise_bench(args):
     dn = float(np.median(dn_ms))
     achieved = flops / (dn * 1e-3) / 1e12
     e4, e64, eraw = rel(img4), rel(img64), rel(raw4)
+
+    # one step of the hand-written trainer (forward, loss, backward, Adam)
+    # over crops of the 4-spp render and its reference, all resident
+    B, Cr = args.train_batch, args.train_crop
+    corners = [((k * 389) % (H - Cr), (k * 677) % (W - Cr)) for k in range(B)]
+    ref_d = torch.from_numpy(np.ascontiguousarray(ref)).cuda()
+    tf = torch.stack([film[y:y + Cr, x:x + Cr] for y, x in corners]).contiguous()
+    tg = torch.stack([guides[y:y + Cr, x:x + Cr] for y, x in corners]).contiguous()
+    tt = torch.stack([ref_d[y:y + Cr, x:x + Cr] for y, x in corners]).contiguous()
+    tr = D.Trainer(r.device, None, seed=1)
+    for _ in range(2):
+        tr.step(tf, tg, tt, 4, 1e-4)
+    train_ms = float(np.median(timed(lambda: tr.step(tf, tg, tt, 4, 1e-4), 10)))
+    tr.close()
+    # forward + weight gradients + input gradients (all layers but the first)
+    train_flops = B * (3 * D.flops(Cr, Cr) - 2 * Cr * Cr * 9 * 32 * D.LEVELS[0])
+    train_tflops = train_flops / (train_ms * 1e-3) / 1e12
     p50_4, p50_64 = float(np.median(lat4)), float(np.median(lat64))
     equal = e4 <= e64
     return {
@@ -392,7 +422,15 @@ def denoise_bench(args):
         "roofline": {"kernel": "U-Net (19 k_conv3x3 bf16 MFMA launches + pool/feature/output passes)", "bound": "mfma",
                      "achieved": round(achieved, 1), "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": round(achieved / 2500.0, 4), "traffic": None, "flops_per_launch": flops,
-                     "denoise_ms_p50": round(dn, 3)}}
+                     "denoise_ms_p50": round(dn, 3),
+                     "speed_of_light_ms": round(D.speed_of_light_ms(W, H), 4),
+                     "frac_of_speed_of_light": round(D.speed_of_light_ms(W, H) / dn, 4),
+                     "speed_of_light": "sum over the 21 launches of max(flops / 2.5 PFLOP/s, least HBM bytes / "
+                                       "8 TB/s) (vork_amd.denoise.layer_costs)"},
+        "training": {"step": f"vh_trainer_step over {B} crops of {Cr}x{Cr} (forward, loss, backward, Adam)",
+                     "ms_per_step": round(train_ms, 3), "flops_per_step": train_flops,
+                     "achieved_tflops": round(train_tflops, 1), "frac_bf16_peak": round(train_tflops / 2500.0, 4)},
+        "weights": args.weights or "shipped"}
 
 
 def serve_bench(args):

@@ -437,7 +437,8 @@ int vh_trainer_create(vh_device* device, const void* weights, size_t bytes, uint
  * every activation, the loss (relative MSE of the radiance plus L1 of its
  * log1p, averaged over the batch, returned in *loss), the backward pass
  * (weight and bias gradients summed over the images) and, unless `flags` has
- * VH_TRAIN_NO_UPDATE, an Adam step with `learning_rate`. */
+ * VH_TRAIN_NO_UPDATE, an Adam step with `learning_rate`.  Every launch runs
+ * over all images of the batch (at most 2^25 pixels in all). */
 #define VH_TRAIN_NO_UPDATE 1u
 int vh_trainer_step(vh_denoiser_trainer* trainer, uint32_t images, uint32_t width, uint32_t height, float samples,
                     const float* films, const float* guides, const float* targets, float learning_rate,

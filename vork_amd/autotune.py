@@ -3,7 +3,8 @@
 The kernels' launch shapes and queue sizes are compile-time constants of the
 HIP sources (workgroup size, waves per SIMD each kernel is compiled for, how
 many idle lanes a traversal wave waits for before refilling, how many of a
-pixel's samples share a wave, persistent-grid workgroups per CU, wavefront
+pixel's samples share a wave, the persistent grid's workgroups per CU (its
+most, its least, and the paths per thread that pick between them), wavefront
 batch length, BVH leaf size and clustering radius, and whether hits are
 sorted into per-closure shading queues).  This module searches them
 by measurement and records the winners in vork_amd/tuning.json, which
@@ -52,6 +53,8 @@ KNOBS = {
     "VH_TRACE_REFILL_MIN": [32, 40, 48, 56],
     "VH_WAVE_SAMPLES": [16, 32, 64],
     "VH_GRID_BLOCKS_PER_CU": [4, 6, 8, 12, 16],
+    "VH_GRID_MIN_BLOCKS_PER_CU": [2, 4, 6],
+    "VH_GRID_PATHS_PER_THREAD": [8, 16, 32],
     "VH_BATCH_PATHS_LOG2": [25, 26, 27],
     "VH_LEAF_MAX": [1, 2, 4],
     "VH_PLOC_RADIUS": [8, 16, 24],

@@ -47,7 +47,24 @@ struct ConvArgs {
     int h, w, c1, c2, up1, relu;
     int cout;            // output channels of the layer (the block computes kCout of them)
     uint16_t* pool;      // [(h + 1) / 2][(w + 1) / 2][cout] 2x2 average of y, or null
+    // a batch of `images` (blockIdx.z; 0 means 1): each tensor holds its
+    // images one after the other, derived strides
+    int images;
 };
+
+// The tensors of image `img` of a batch: every pointer advanced by its
+// image's stride (x1 at its own resolution).
+__device__ __forceinline__ ConvArgs conv_image(ConvArgs a, int img) {
+    if (img == 0) return a;
+    const size_t px = (size_t)a.h * a.w;
+    const size_t px1 = a.up1 ? (size_t)((a.h + 1) / 2) * ((a.w + 1) / 2) : px;
+    const size_t pxp = (size_t)((a.h + 1) / 2) * ((a.w + 1) / 2);
+    a.x1 += img * px1 * a.c1;
+    if (a.x2) a.x2 += img * px * a.c2;
+    a.y += img * px * a.cout;
+    if (a.pool) a.pool += img * pxp * a.cout;
+    return a;
+}
 
 // One 16-byte chunk of channels [ch, ch + 8) of halo pixel (hy, hx) of the tile
 // at (y0, x0): zero outside the image.
@@ -62,12 +79,18 @@ __device__ __forceinline__ uint4 fetch_chunk(const ConvArgs& a, int y, int x, in
 }
 
 // kCout output channels from blockIdx.y * kCout; waves split as kMW (pixel
-// rows) x kNW (channel blocks).
-template <int kCout, int kMW>
-__global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
+// rows) x kNW (channel blocks); a wave holds the weight fragments of kGroup
+// taps at once (9: one L2 round trip per K block, issued before the halo
+// staging so the two latencies overlap; 3: fewer registers, more waves).
+template <int kCout, int kMW, int kGroup>
+__global__ __launch_bounds__(256) void k_conv3x3(ConvArgs args) {
+    const ConvArgs a = conv_image(args, blockIdx.z);
     constexpr int kNB = kCout / 16, kNW = 4 / kMW, kNBW = kNB / kNW, kRows = kTileH / kMW;
     static_assert(kNB % kNW == 0 && kTileH % kMW == 0, "wave split");
-    __shared__ __attribute__((aligned(16))) uint8_t halo[kHaloH * kHaloW * kPix];
+    constexpr int kOutPix = kCout + 8;  // bf16 per output-tile pixel (kCout + 16 B of pad)
+    constexpr int kLds = kHaloH * kHaloW * kPix > kTileH * kTileW * kOutPix * 2 ? kHaloH * kHaloW * kPix
+                                                                               : kTileH * kTileW * kOutPix * 2;
+    __shared__ __attribute__((aligned(16))) uint8_t halo[kLds];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave % kMW, wn = wave / kMW;
     const int tiles_x = (a.w + kTileW - 1) / kTileW;
@@ -80,7 +103,20 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
 #pragma unroll
         for (int n = 0; n < kNBW; n++) acc[r][n] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int kg = lane >> 4, col = lane & 15;  // k group (8 channels) and row/column of the fragment
+    static_assert(9 % kGroup == 0, "tap groups");
+    bf16x8 b[kGroup][kNBW];
+    auto load_taps = [&](int cb, int first) {
+#pragma unroll
+        for (int t = 0; t < kGroup; t++)
+#pragma unroll
+            for (int n = 0; n < kNBW; n++) {
+                const int co = co0 + (wn * kNBW + n) * 16 + col;
+                b[t][n] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(
+                                                         a.wt + ((size_t)(first + t) * a.cout + co) * cin + cb + kg * 8));
+            }
+    };
     for (int cb = 0; cb < cin; cb += kCB) {
+        load_taps(cb, 0);
         __syncthreads();
         for (int i = tid; i < kHaloH * kHaloW * 4; i += 256) {
             const int px = i >> 2, q = i & 3;
@@ -88,72 +124,95 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
             *reinterpret_cast<uint4*>(halo + px * kPix + q * 16) = fetch_chunk(a, y0 + hy - 1, x0 + hx - 1, cb + q * 8);
         }
         __syncthreads();
-#pragma unroll 1
-        for (int tap = 0; tap < 9; tap++) {
-            const int dy = tap / 3, dx = tap % 3;
-            bf16x8 b[kNBW];
 #pragma unroll
-            for (int n = 0; n < kNBW; n++) {
-                const int co = co0 + (wn * kNBW + n) * 16 + col;
-                b[n] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(
-                                                      a.wt + ((size_t)tap * a.cout + co) * cin + cb + kg * 8));
-            }
+        for (int g = 0; g < 9; g += kGroup) {
+            if (g > 0) load_taps(cb, g);
 #pragma unroll
-            for (int r = 0; r < kRows; r++) {
-                const int hy = wm * kRows + r + dy, hx = col + dx;
-                const bf16x8 av = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(
-                                                                 halo + (hy * kHaloW + hx) * kPix + kg * 16));
+            for (int t = 0; t < kGroup; t++) {
+                const int dy = (g + t) / 3, dx = (g + t) % 3;
 #pragma unroll
-                for (int n = 0; n < kNBW; n++) acc[r][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b[n], acc[r][n], 0, 0, 0);
+                for (int r = 0; r < kRows; r++) {
+                    const int hy = wm * kRows + r + dy, hx = col + dx;
+                    const bf16x8 av = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(
+                                                                     halo + (hy * kHaloW + hx) * kPix + kg * 16));
+#pragma unroll
+                    for (int n = 0; n < kNBW; n++)
+                        acc[r][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b[t][n], acc[r][n], 0, 0, 0);
+                }
+                // each tap's LDS reads stay with its MFMAs (hoisting every
+                // tap's A fragments would cost registers, hence blocks per CU)
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
     }
-    // D fragment: lane holds output channel (block + col) of pixels 4 kg + j of the row
-    static_assert(kRows % 2 == 0, "pooling pairs rows within a wave");
+    // Epilogue through LDS: bias, ReLU and bf16 rounding per D fragment (a
+    // lane holds output channel (block + col) of pixels 4 kg + j of a row)
+    // into an output tile of the block's kCout channels per pixel, then
+    // whole 16-byte chunks (8 channels) to memory, and the 2x2 pool from the
+    // rounded tile.
+    __syncthreads();  // every wave is done with the halo
+    uint16_t* tile = reinterpret_cast<uint16_t*>(halo);
 #pragma unroll
     for (int r = 0; r < kRows; r++) {
-        const int y = y0 + wm * kRows + r;
+        const int ty = wm * kRows + r;
 #pragma unroll
         for (int n = 0; n < kNBW; n++) {
-            const int co = co0 + (wn * kNBW + n) * 16 + col;
-            const float bias = a.bias[co];
+            const int c = (wn * kNBW + n) * 16 + col;
+            const float bias = a.bias[co0 + c];
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                const int x = x0 + kg * 4 + j;
                 float v = acc[r][n][j] + bias;
                 if (a.relu) v = fmaxf(v, 0.f);
-                const uint16_t h16 = f2bf(v);
-                acc[r][n][j] = bf2f(h16);  // the stored value, for the pool below
-                if (y < a.h && x < a.w) a.y[((size_t)y * a.w + x) * a.cout + co] = h16;
+                tile[(ty * kTileW + kg * 4 + j) * kOutPix + c] = f2bf(v);
             }
         }
+    }
+    __syncthreads();
+    constexpr int kChunks = kCout / 8;  // 16-byte chunks per pixel
+    for (int i = tid; i < kTileH * kTileW * kChunks; i += 256) {
+        const int px = i / kChunks, q = i % kChunks;
+        const int y = y0 + px / kTileW, x = x0 + px % kTileW;
+        if (y < a.h && x < a.w)
+            *reinterpret_cast<uint4*>(a.y + ((size_t)y * a.w + x) * a.cout + co0 + q * 8) =
+                *reinterpret_cast<const uint4*>(tile + px * kOutPix + q * 8);
     }
     if (a.pool) {
         const int ow = (a.w + 1) / 2;
+        for (int i = tid; i < (kTileH / 2) * (kTileW / 2) * kChunks; i += 256) {
+            const int g = i / kChunks, q = i % kChunks;
+            const int gy = g / (kTileW / 2), gx = g % (kTileW / 2);
+            const int y = y0 + 2 * gy, x = x0 + 2 * gx;
+            if (y >= a.h || x >= a.w) continue;
+            const bool right = x + 1 < a.w, down = y + 1 < a.h;
+            const int p00 = (2 * gy) * kTileW + 2 * gx;
+            const uint4 c00 = *reinterpret_cast<const uint4*>(tile + p00 * kOutPix + q * 8);
+            const uint4 c01 = *reinterpret_cast<const uint4*>(tile + (p00 + 1) * kOutPix + q * 8);
+            const uint4 c10 = *reinterpret_cast<const uint4*>(tile + (p00 + kTileW) * kOutPix + q * 8);
+            const uint4 c11 = *reinterpret_cast<const uint4*>(tile + (p00 + kTileW + 1) * kOutPix + q * 8);
+            const float cnt = (float)((1 + right) * (1 + down));
+            uint32_t o[4];
+            const uint32_t* w00 = &c00.x; const uint32_t* w01 = &c01.x;
+            const uint32_t* w10 = &c10.x; const uint32_t* w11 = &c11.x;
 #pragma unroll
-        for (int r = 0; r < kRows; r += 2) {
-            const int y = y0 + wm * kRows + r;
-            if (y >= a.h) continue;
+            for (int k = 0; k < 4; k++) {
+                uint32_t packed = 0;
 #pragma unroll
-            for (int n = 0; n < kNBW; n++) {
-                const int co = co0 + (wn * kNBW + n) * 16 + col;
-#pragma unroll
-                for (int j = 0; j < 4; j += 2) {
-                    const int x = x0 + kg * 4 + j;
-                    if (x >= a.w) continue;
+                for (int hlf = 0; hlf < 2; hlf++) {
+                    const int sh = 16 * hlf;
                     // sum in the order (0,0), (0,1), (1,0), (1,1); the last row / column of
                     // an odd size averages what exists
-                    float s = acc[r][n][j];
-                    int cnt = 1;
-                    if (x + 1 < a.w) { s += acc[r][n][j + 1]; cnt++; }
-                    if (y + 1 < a.h) {
-                        s += acc[r + 1][n][j];
-                        cnt++;
-                        if (x + 1 < a.w) { s += acc[r + 1][n][j + 1]; cnt++; }
+                    float sum = bf2f((uint16_t)(w00[k] >> sh));
+                    if (right) sum += bf2f((uint16_t)(w01[k] >> sh));
+                    if (down) {
+                        sum += bf2f((uint16_t)(w10[k] >> sh));
+                        if (right) sum += bf2f((uint16_t)(w11[k] >> sh));
                     }
-                    a.pool[((size_t)(y / 2) * ow + x / 2) * a.cout + co] = f2bf(s / (float)cnt);
+                    packed |= (uint32_t)f2bf(sum / cnt) << sh;
                 }
+                o[k] = packed;
             }
+            *reinterpret_cast<uint4*>(a.pool + ((size_t)(y / 2) * ow + x / 2) * a.cout + co0 + q * 8) =
+                make_uint4(o[0], o[1], o[2], o[3]);
         }
     }
 }
@@ -173,8 +232,16 @@ __global__ void k_features(const float4* film, const float4* guides, float sampl
     v[3] = ga.x * inv; v[4] = ga.y * inv; v[5] = ga.z * inv;
     v[6] = gn.x * inv; v[7] = gn.y * inv; v[8] = gn.z * inv;
     v[9] = gn.w * inv;
-    uint16_t* o = x + (size_t)p * 32;
-    for (int k = 0; k < 32; k++) o[k] = f2bf(v[k]);
+    // 64 bytes per pixel as four 16-byte stores
+    uint4* o = reinterpret_cast<uint4*>(x + (size_t)p * 32);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        uint32_t w[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            w[k] = (uint32_t)f2bf(v[8 * q + 2 * k]) | ((uint32_t)f2bf(v[8 * q + 2 * k + 1]) << 16);
+        o[q] = make_uint4(w[0], w[1], w[2], w[3]);
+    }
 }
 
 // Output: expm1 of the predicted log radiance (channels 0-2 of 16), alpha 1.
@@ -215,13 +282,16 @@ constexpr int kShape[kLayers][2] = {
 inline int launch_conv(const ConvArgs& a, uint32_t cu_count, hipStream_t s, bool force_split = false) {
     const uint32_t tiles = nblocks(a.w, kTileW) * nblocks(a.h, kTileH);
     const bool split = force_split || (a.cout > 32 && tiles < VH_DENOISE_SPLIT_BLOCKS * cu_count);
-    const dim3 grid(tiles, split ? a.cout / 32 : 1);
+    const dim3 grid(tiles, split ? a.cout / 32 : 1, a.images > 1 ? a.images : 1);
     switch (split ? 32 : a.cout) {
-        case 16: hipLaunchKernelGGL((k_conv3x3<16, 4>), grid, dim3(256), 0, s, a); break;
-        case 32: hipLaunchKernelGGL((k_conv3x3<32, 2>), grid, dim3(256), 0, s, a); break;
-        case 64: hipLaunchKernelGGL((k_conv3x3<64, 1>), grid, dim3(256), 0, s, a); break;
-        case 96: hipLaunchKernelGGL((k_conv3x3<96, 2>), grid, dim3(256), 0, s, a); break;
-        case 128: hipLaunchKernelGGL((k_conv3x3<128, 1>), grid, dim3(256), 0, s, a); break;
+        // wave splits and tap groups as measured fastest at 1920x1080: the
+        // kernels are latency bound (a block waits on its halo and weight
+        // loads), so fewer registers and more resident waves win
+        case 16: hipLaunchKernelGGL((k_conv3x3<16, 4, 3>), grid, dim3(256), 0, s, a); break;
+        case 32: hipLaunchKernelGGL((k_conv3x3<32, 2, 3>), grid, dim3(256), 0, s, a); break;
+        case 64: hipLaunchKernelGGL((k_conv3x3<64, 1, 9>), grid, dim3(256), 0, s, a); break;
+        case 96: hipLaunchKernelGGL((k_conv3x3<96, 2, 3>), grid, dim3(256), 0, s, a); break;
+        case 128: hipLaunchKernelGGL((k_conv3x3<128, 1, 3>), grid, dim3(256), 0, s, a); break;
         default: return fail(VH_ERR_USAGE, "denoiser: unsupported channel count");
     }
     return VH_OK;

@@ -52,6 +52,7 @@ int denoise_run(vh_denoiser* dn, int W, int H, float samples, const float* film,
         a.wt = wts + dn->w_off[layer]; a.bias = bs + dn->b_off[layer];
         a.y = y; a.h = h[l]; a.w = w[l]; a.relu = relu ? 1 : 0;
         a.cout = kShape[layer][1];
+        a.images = 1;
         layer++;
         return launch_conv(a, (uint32_t)dn->device->cu_count, s);
     };

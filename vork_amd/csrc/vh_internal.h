@@ -69,16 +69,14 @@ struct StreamState {
     size_t capacity = 0;  // paths the wavefront buffers hold
     BuildScratch build;
     // instantiated launch sequences of render batches by shape (hipGraphExec_t),
-    // recorded on `capture` and replayed on the calling stream
+    // built node by node and replayed on the calling stream
     using GraphKey = std::array<uint32_t, 8>;
     std::vector<std::pair<GraphKey, void*>> graphs;
-    hipStream_t capture = nullptr;
     StreamState() = default;
     StreamState(const StreamState&) = delete;
     StreamState& operator=(const StreamState&) = delete;
     ~StreamState() {
         for (auto& g : graphs) (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(g.second));
-        if (capture) (void)hipStreamDestroy(capture);
     }
 };
 

@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <tuple>
 #include <vector>
 
 #include "vh_internal.h"
@@ -190,10 +191,28 @@ constexpr uint32_t kClasses = 3;
 #ifndef VH_BATCH_PATHS_LOG2
 #define VH_BATCH_PATHS_LOG2 27
 #endif
-// Workgroups per CU of the persistent and grid-stride launches.
+// Workgroups per CU of the persistent and grid-stride launches: the grid
+// follows the batch, about VH_GRID_PATHS_PER_THREAD paths per thread, between
+// VH_GRID_MIN_BLOCKS_PER_CU and VH_GRID_BLOCKS_PER_CU workgroups per CU.  A
+// whole 1920x1080 x 64-spp exposure (2^27 paths) takes the most (later
+// workgroups start fresh while the first ones drain their slowest rays); a
+// 4-spp exposure is faster with a quarter of them, whose threads each claim
+// more rays instead of waiting to be scheduled.
 #ifndef VH_GRID_BLOCKS_PER_CU
 #define VH_GRID_BLOCKS_PER_CU 8
 #endif
+#ifndef VH_GRID_MIN_BLOCKS_PER_CU
+#define VH_GRID_MIN_BLOCKS_PER_CU 4
+#endif
+#ifndef VH_GRID_PATHS_PER_THREAD
+#define VH_GRID_PATHS_PER_THREAD 16
+#endif
+inline uint32_t wavefront_grid(size_t paths, uint32_t cu_count) {
+    const size_t per_cu = paths / ((size_t)cu_count * kBlock * VH_GRID_PATHS_PER_THREAD);
+    const uint32_t b = (uint32_t)std::min<size_t>(std::max<size_t>(per_cu, VH_GRID_MIN_BLOCKS_PER_CU),
+                                                  VH_GRID_BLOCKS_PER_CU);
+    return std::min<uint32_t>((uint32_t)((paths + kBlock - 1) / kBlock), cu_count * b);
+}
 
 // One job of a shared-queue batch (vh_render_shared): a scene's own
 // constants, tables and film, its range of the batch's path slots and of the
@@ -1186,46 +1205,97 @@ struct BatchShape {
 // and batch layouts in turn each find theirs.
 constexpr size_t kGraphsPerStream = 8;
 
-// Issues one batch's kernels on `s`.  `mark(stage, begin)` is called around
-// the extend (0; 3 for the camera bounce), shade (1) and shadow (2) launches.
-template <class Mark>
-int launch_batch(const BatchShape& b, const LaunchArgs* A, hipStream_t s, Mark&& mark) {
+// A kernel's arguments converted to its parameter types, and the array of
+// their addresses that hipLaunchKernel and kernel graph nodes take.
+template <class... P>
+struct KernelArgs {
+    std::tuple<P...> values;
+    void* ptrs[sizeof...(P) > 0 ? sizeof...(P) : 1];
+    template <class... A>
+    explicit KernelArgs(A... a) : values(static_cast<P>(a)...) {
+        std::apply([&](auto&... v) {
+            int i = 0;
+            ((ptrs[i++] = static_cast<void*>(&v)), ...);
+        }, values);
+    }
+    KernelArgs(const KernelArgs&) = delete;
+};
+
+// Launches a kernel directly on a stream.
+struct StreamLauncher {
+    hipStream_t s;
+    template <class... P, class... A>
+    void operator()(void (*k)(P...), dim3 grid, dim3 block, A... a) {
+        KernelArgs<P...> args(a...);
+        (void)hipLaunchKernel(reinterpret_cast<const void*>(k), grid, block, args.ptrs, 0, s);
+    }
+};
+
+// Appends a kernel node to a graph after the previous one: the batch's
+// graph is built node by node rather than captured from a stream, so other
+// threads' work on other streams (the serving loop's slots, torch's own
+// launches) cannot invalidate it while it is being recorded.
+struct GraphLauncher {
+    hipGraph_t graph = nullptr;
+    hipGraphNode_t last = nullptr;
+    hipError_t error = hipSuccess;
+    template <class... P, class... A>
+    void operator()(void (*k)(P...), dim3 grid, dim3 block, A... a) {
+        if (error != hipSuccess) return;
+        KernelArgs<P...> args(a...);  // copied into the node by hipGraphAddKernelNode
+        hipKernelNodeParams np{};
+        np.func = reinterpret_cast<void*>(k);
+        np.gridDim = grid;
+        np.blockDim = block;
+        np.sharedMemBytes = 0;
+        np.kernelParams = args.ptrs;
+        np.extra = nullptr;
+        hipGraphNode_t node = nullptr;
+        error = hipGraphAddKernelNode(&node, graph, last ? &last : nullptr, last ? 1 : 0, &np);
+        if (error == hipSuccess) last = node;
+    }
+};
+
+// Issues one batch's kernels through `launch` (a stream or a graph).
+// `mark(stage, begin)` is called around the extend (0; 3 for the camera
+// bounce), shade (1) and shadow (2) launches.
+template <class Launch, class Mark>
+int launch_batch(const BatchShape& b, const LaunchArgs* A, Launch& launch, Mark&& mark) {
     const dim3 grid(b.grid), block(kBlock), one(1), wave(64);
     const dim3 per_path(blocks(b.paths, kBlock)), per_pixel(blocks(b.pixels, kBlock));
-    if (b.mode == 2) hipLaunchKernelGGL(k_decide, per_pixel, block, 0, s, A);
+    if (b.mode == 2) launch(k_decide, per_pixel, block, A);
     if (b.mode != 0) {
-        hipLaunchKernelGGL(k_open_batch, one, wave, 0, s, A);
-        hipLaunchKernelGGL(k_camera<true>, per_path, block, 0, s, A);
+        launch(k_open_batch, one, wave, A);
+        launch(k_camera<true>, per_path, block, A);
     } else {
-        hipLaunchKernelGGL(k_camera<false>, per_path, block, 0, s, A);
+        launch(k_camera<false>, per_path, block, A);
     }
     int rc;
     for (uint32_t bounce = 0; bounce < b.bounces; bounce++) {
         const int stage = bounce == 0 ? 3 : 0;
         if ((rc = mark(stage, true))) return rc;
-        if (b.count) hipLaunchKernelGGL(k_extend<true>, grid, block, 0, s, A, bounce);
-        else if (b.packet && bounce == 0) hipLaunchKernelGGL(k_extend_packet, grid, block, 0, s, A);
-        else hipLaunchKernelGGL(k_extend<false>, grid, block, 0, s, A, bounce);
+        if (b.count) launch(k_extend<true>, grid, block, A, bounce);
+        else if (b.packet && bounce == 0) launch(k_extend_packet, grid, block, A);
+        else launch(k_extend<false>, grid, block, A, bounce);
         if ((rc = mark(stage, false))) return rc;
         if ((rc = mark(1, true))) return rc;
-        if (b.sort) hipLaunchKernelGGL(k_classify, grid, block, 0, s, A, bounce);
-        hipLaunchKernelGGL(k_shade<false>, grid, block, 0, s, A, bounce);
+        if (b.sort) launch(k_classify, grid, block, A, bounce);
+        launch(k_shade<false>, grid, block, A, bounce);
         if ((rc = mark(1, false))) return rc;
         if ((rc = mark(2, true))) return rc;
-        if (b.count) hipLaunchKernelGGL(k_shadow<true>, grid, block, 0, s, A);
-        else hipLaunchKernelGGL(k_shadow<false>, grid, block, 0, s, A);
+        if (b.count) launch(k_shadow<true>, grid, block, A);
+        else launch(k_shadow<false>, grid, block, A);
         if ((rc = mark(2, false))) return rc;
-        hipLaunchKernelGGL(k_resolve, grid, block, 0, s, A);
-        hipLaunchKernelGGL(k_swap_counts, one, wave, 0, s, A);
+        launch(k_resolve, grid, block, A);
+        launch(k_swap_counts, one, wave, A);
     }
-    if (b.mode != 0) hipLaunchKernelGGL(k_film<true>, per_pixel, block, 0, s, A);
-    else hipLaunchKernelGGL(k_film<false>, per_pixel, block, 0, s, A);
-    VH_HIP(hipGetLastError());
+    if (b.mode != 0) launch(k_film<true>, per_pixel, block, A);
+    else launch(k_film<false>, per_pixel, block, A);
     return VH_OK;
 }
 
-// Replays the stream's graph of this batch shape on `s`, recording it first
-// on the stream's private capture stream if the stream has none yet.
+// Replays the stream's graph of this batch shape on `s`, building it first
+// if the stream has none yet.
 int launch_batch_graph(StreamState* st, const BatchShape& b, const LaunchArgs* A, hipStream_t s) {
     const StreamState::GraphKey key = b.key();
     hipGraphExec_t exec = nullptr;
@@ -1236,19 +1306,14 @@ int launch_batch_graph(StreamState* st, const BatchShape& b, const LaunchArgs* A
             (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(st->graphs.front().second));
             st->graphs.erase(st->graphs.begin());
         }
-        if (!st->capture) VH_HIP(hipStreamCreateWithFlags(&st->capture, hipStreamNonBlocking));
-        VH_HIP(hipStreamBeginCapture(st->capture, hipStreamCaptureModeRelaxed));
-        int rc = launch_batch(b, A, st->capture, [](int, bool) { return VH_OK; });
-        hipGraph_t graph = nullptr;
-        hipError_t e = hipStreamEndCapture(st->capture, &graph);
-        if (rc) {
-            if (graph) (void)hipGraphDestroy(graph);
-            return rc;
-        }
-        if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+        GraphLauncher g;
+        VH_HIP(hipGraphCreate(&g.graph, 0));
+        int rc = launch_batch(b, A, g, [](int, bool) { return VH_OK; });
+        hipError_t e = g.error;
+        if (!rc && e == hipSuccess) e = hipGraphInstantiate(&exec, g.graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g.graph);
+        if (rc) return rc;
+        if (e != hipSuccess) return hip_fail(e, "vh_render: batch graph");
         st->graphs.emplace_back(key, exec);
     }
     VH_HIP(hipGraphLaunch(exec, s));
@@ -1390,7 +1455,7 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, float* g
     const Scene S = scene_tables(sc);
     Consts c = scene_consts(sc, rp, guides != nullptr);
     c.minimum = minimum;
-    const uint32_t grid = std::min<uint32_t>(blocks(cap, kBlock), (uint32_t)dev->cu_count * VH_GRID_BLOCKS_PER_CU);
+    const uint32_t grid = wavefront_grid(cap, (uint32_t)dev->cu_count);
     int rc = ensure_overflow(st->overflow, grid * kBlock);
     if (rc) return rc;
     VH_HIP(st->launch_args.reserve(sizeof(LaunchArgs)));
@@ -1452,7 +1517,8 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, float* g
         shape.sort = sort;
         if (profile) {
             // launched one by one, with events around the stage launches
-            rc = launch_batch(shape, dargs, s, [&](int stage, bool begin) -> int {
+            StreamLauncher direct{s};
+            rc = launch_batch(shape, dargs, direct, [&](int stage, bool begin) -> int {
                 if (begin) {
                     marks.push_back(Mark{stage, nullptr, nullptr});
                     VH_HIP(hipEventCreate(&marks.back().a));
@@ -1554,7 +1620,7 @@ int render_shared(const vh_render_job* jobs, uint32_t count, hipStream_t s) {
     VH_HIP(st->counters.reserve(kCounterWords * sizeof(uint32_t)));
     VH_HIP(st->launch_args.reserve(sizeof(LaunchArgs)));
     VH_HIP(st->jobs.reserve(kMaxJobs * sizeof(JobArgs)));
-    const uint32_t grid = std::min<uint32_t>(blocks(cap, kBlock), (uint32_t)dev->cu_count * VH_GRID_BLOCKS_PER_CU);
+    const uint32_t grid = wavefront_grid(cap, (uint32_t)dev->cu_count);
     int rc = ensure_overflow(st->overflow, grid * kBlock);
     if (rc) return rc;
     uint32_t* counters = st->counters.as<uint32_t>();

@@ -112,15 +112,16 @@ struct WgradArgs {
     float* dw;            // [tap][cout][cin] fp32, accumulated
     float* db;            // [cout] fp32, accumulated
     int h, w, c1, c2, up1, cout, cp;
+    int images;           // a batch: every tensor holds its images one after the other
 };
-__device__ __forceinline__ uint16_t input_at(const WgradArgs& a, int y, int x, int ch) {
+__device__ __forceinline__ uint16_t input_at(const WgradArgs& a, int img, int y, int x, int ch) {
     if (y < 0 || x < 0 || y >= a.h || x >= a.w) return 0;
     if (ch < a.c1) {
-        const int wh = a.up1 ? (a.w + 1) / 2 : a.w;
+        const int wh = a.up1 ? (a.w + 1) / 2 : a.w, hh = a.up1 ? (a.h + 1) / 2 : a.h;
         const int yy = a.up1 ? y >> 1 : y, xx = a.up1 ? x >> 1 : x;
-        return a.x1[((size_t)yy * wh + xx) * a.c1 + ch];
+        return a.x1[(((size_t)img * hh + yy) * wh + xx) * a.c1 + ch];
     }
-    return a.x2[((size_t)y * a.w + x) * a.c2 + (ch - a.c1)];
+    return a.x2[(((size_t)img * a.h + y) * a.w + x) * a.c2 + (ch - a.c1)];
 }
 
 // dW[tap][co][ci] += sum_p dz[p][co] * x[p + offset(tap)][ci] and
@@ -136,7 +137,7 @@ __global__ __launch_bounds__(256) void k_wgrad(WgradArgs a) {
     const int tap = blockIdx.z % 9, cb = (blockIdx.z / 9) * 32;
     const int dy = tap / 3 - 1, dx = tap % 3 - 1;
     const int cin = a.c1 + a.c2;
-    const int pixels = a.h * a.w;
+    const int pixels = a.h * a.w, total = pixels * a.images;  // dz holds the batch's pixels in order
     const int p0 = blockIdx.x * kWgradPixels;
     f32x4 acc[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
     float bsum = 0.f;
@@ -145,13 +146,14 @@ __global__ __launch_bounds__(256) void k_wgrad(WgradArgs a) {
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const int p = p0 + ks * 32 + kg * 8 + j;
-            const bool in = p < pixels;
+            const bool in = p < total;
             av[j] = in && co < a.cp ? a.dz[(size_t)p * a.cp + co] : (uint16_t)0;
-            const int y = in ? p / a.w : -10, x = in ? p % a.w : 0;
+            const int img = in ? p / pixels : 0, lp = p - img * pixels;
+            const int y = in ? lp / a.w : -10, x = in ? lp % a.w : 0;
 #pragma unroll
             for (int nb = 0; nb < 2; nb++) {
                 const int ci = cb + nb * 16 + col;
-                bv[nb][j] = in && ci < cin ? input_at(a, y + dy, x + dx, ci) : (uint16_t)0;
+                bv[nb][j] = in && ci < cin ? input_at(a, img, y + dy, x + dx, ci) : (uint16_t)0;
             }
             bsum += bf2f(av[j]);
         }
@@ -191,26 +193,29 @@ __global__ __launch_bounds__(256) void k_wgrad(WgradArgs a) {
 //   mode 2: x1 upsampled (channels [0, c1)) ++ x2 (channels [c1, cin)):
 //           g2 += the x2 part; each coarse pixel of g1 sums its children's
 //           x1 part
+//   Every tensor holds the batch's `images` one after the other.
 __global__ void k_route(const uint16_t* dx, int h, int w, int c1, int c2, int mode, float* g1, float* g2, int fh,
-                        int fw) {
+                        int fw, int images) {
     const int cin = c1 + c2;
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (mode == 0) {
-        if (i >= (size_t)h * w * c1) return;
+        if (i >= (size_t)images * h * w * c1) return;
         g1[i] += bf2f(dx[i]);
     } else if (mode == 1) {
         // one thread per fine pixel and channel
-        if (i >= (size_t)fh * fw * c1) return;
-        const size_t q = i / c1;
-        const int ch = (int)(i % c1);
+        const size_t fine = (size_t)fh * fw * c1;
+        if (i >= images * fine) return;
+        const size_t img = i / fine, r = i - img * fine;
+        const size_t q = r / c1;
+        const int ch = (int)(r % c1);
         const int y = (int)(q / fw), x = (int)(q % fw);
         const int py = y >> 1, px = x >> 1;
         const int ny = min(2, fh - 2 * py), nx = min(2, fw - 2 * px);
-        g1[i] += bf2f(dx[((size_t)py * w + px) * c1 + ch]) / (float)(ny * nx);
+        g1[i] += bf2f(dx[((img * h + py) * w + px) * c1 + ch]) / (float)(ny * nx);
     } else {
         // one thread per pixel and channel of the layer's input
-        if (i >= (size_t)h * w * cin) return;
-        const size_t p = i / cin;
+        if (i >= (size_t)images * h * w * cin) return;
+        const size_t p = i / cin;  // the pixel in the batch
         const int ch = (int)(i % cin);
         if (ch >= c1) {
             g2[p * c2 + (ch - c1)] += bf2f(dx[i]);
@@ -218,16 +223,17 @@ __global__ void k_route(const uint16_t* dx, int h, int w, int c1, int c2, int mo
         }
         // the coarse pixel sums its (up to 4) children once, from its
         // top-left child's thread
-        const int y = (int)(p / w), x = (int)(p % w);
+        const size_t img = p / ((size_t)h * w), lp = p - img * h * w;
+        const int y = (int)(lp / w), x = (int)(lp % w);
         if ((y & 1) || (x & 1)) return;
-        const int cw = (w + 1) / 2;
+        const int cw = (w + 1) / 2, chh = (h + 1) / 2;
         float s = bf2f(dx[i]);
         if (x + 1 < w) s += bf2f(dx[(p + 1) * cin + ch]);
         if (y + 1 < h) {
             s += bf2f(dx[(p + w) * cin + ch]);
             if (x + 1 < w) s += bf2f(dx[(p + w + 1) * cin + ch]);
         }
-        g1[((size_t)(y >> 1) * cw + (x >> 1)) * c1 + ch] += s;
+        g1[((img * chh + (y >> 1)) * cw + (x >> 1)) * c1 + ch] += s;
     }
 }
 
@@ -289,7 +295,7 @@ struct vh_denoiser_trainer {
     uint32_t b_off[vh::kLayers + 1] = {};
     vh::DeviceBuffer w, mw, vw, gw, wb, wt, b, mb, vb, gb, zero_bias, loss;
     vh::DeviceBuffer act, grad, tmp;    // activations, fp32 gradients, bf16 temporaries
-    int act_w = 0, act_h = 0;
+    int act_w = 0, act_h = 0, act_n = 0;  // size and images the buffers hold
     size_t y_off[vh::kLayers] = {}, pool_off[4] = {}, feat_off = 0;  // in `act` (bytes)
     size_t g_off[vh::kLayers] = {};                                  // in `grad` (bytes)
     size_t dz_off = 0, dx_off = 0;                                   // in `tmp` (bytes)
@@ -300,12 +306,12 @@ struct vh_denoiser_trainer {
 namespace vh {
 namespace {
 
-int trainer_buffers(vh_denoiser_trainer* tr, int W, int H) {
-    if (tr->act_w == W && tr->act_h == H) return VH_OK;
+int trainer_buffers(vh_denoiser_trainer* tr, int W, int H, int N) {
+    if (tr->act_w == W && tr->act_h == H && tr->act_n == N) return VH_OK;
     int h[5], w[5];
     h[0] = H; w[0] = W;
     for (int l = 1; l < 5; l++) { h[l] = (h[l - 1] + 1) / 2; w[l] = (w[l - 1] + 1) / 2; }
-    auto px = [&](int l) { return (size_t)h[l] * w[l]; };
+    auto px = [&](int l) { return (size_t)N * h[l] * w[l]; };  // pixels of the batch at level l
     size_t a = 0, g = 0, t_dz = 0, t_dx = 0;
     auto take = [](size_t& off, size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
     tr->feat_off = take(a, px(0) * 32 * 2);
@@ -325,10 +331,12 @@ int trainer_buffers(vh_denoiser_trainer* tr, int W, int H) {
     VH_HIP(tr->tmp.allocate(t));
     tr->act_w = W;
     tr->act_h = H;
+    tr->act_n = N;
     return VH_OK;
 }
 
-// Forward pass of one image, every activation kept.
+// Forward pass of the batch (every launch over all its images), every
+// activation kept.
 int trainer_forward(vh_denoiser_trainer* tr, const float* film, const float* guides, float samples, hipStream_t s) {
     const int W = tr->act_w, H = tr->act_h;
     int h[5], w[5];
@@ -336,7 +344,7 @@ int trainer_forward(vh_denoiser_trainer* tr, const float* film, const float* gui
     for (int l = 1; l < 5; l++) { h[l] = (h[l - 1] + 1) / 2; w[l] = (w[l - 1] + 1) / 2; }
     uint8_t* base = tr->act.as<uint8_t>();
     auto at = [&](size_t o) { return reinterpret_cast<uint16_t*>(base + o); };
-    const size_t px0 = (size_t)W * H;
+    const size_t px0 = (size_t)tr->act_n * W * H;
     hipLaunchKernelGGL(k_features, dim3(nblocks(px0, 256)), dim3(256), 0, s, reinterpret_cast<const float4*>(film),
                        reinterpret_cast<const float4*>(guides), samples, (int)px0, at(tr->feat_off));
     for (int L = 0; L < kLayers; L++) {
@@ -358,6 +366,7 @@ int trainer_forward(vh_denoiser_trainer* tr, const float* film, const float* gui
         a.cout = kShape[L][1];
         // the encoder's second convolutions of levels 0-3 also write their pool
         a.pool = (L % 2 == 1 && L < 8) ? at(tr->pool_off[L / 2]) : nullptr;
+        a.images = tr->act_n;
         int rc = launch_conv(a, (uint32_t)tr->device->cu_count, s);
         if (rc) return rc;
     }
@@ -365,7 +374,7 @@ int trainer_forward(vh_denoiser_trainer* tr, const float* film, const float* gui
     return VH_OK;
 }
 
-// Backward pass of one image (the output gradient is in g_off[18]); adds to
+// Backward pass of the batch (the output gradient is in g_off[18]); adds to
 // the weight and bias gradients.
 int trainer_backward(vh_denoiser_trainer* tr, hipStream_t s) {
     const int W = tr->act_w, H = tr->act_h;
@@ -384,7 +393,8 @@ int trainer_backward(vh_denoiser_trainer* tr, hipStream_t s) {
     for (int L = kLayers - 1; L >= 0; L--) {
         const LayerInput& in = kInput[L];
         const int l = kLevel[L], co = kShape[L][1], ci = kShape[L][0], cp = cpad32(co);
-        const size_t P = (size_t)h[l] * w[l];
+        const int N = tr->act_n;
+        const size_t P = (size_t)N * h[l] * w[l];  // pixels of the batch
         hipLaunchKernelGGL(k_relu_grad, dim3(nblocks(P * cp, 256)), dim3(256), 0, s, g_at(L), y_at(L), (int)P, co, cp,
                            L < kLayers - 1 ? 1 : 0, dz);
         WgradArgs wa;
@@ -401,6 +411,7 @@ int trainer_backward(vh_denoiser_trainer* tr, hipStream_t s) {
         wa.w = w[l];
         wa.cout = co;
         wa.cp = cp;
+        wa.images = N;
         const dim3 wg(nblocks(P, kWgradPixels), (co + 63) / 64, 9 * ((ci + 31) / 32));
         hipLaunchKernelGGL(k_wgrad, wg, dim3(256), 0, s, wa);
         if (L == 0) break;  // the features need no gradient
@@ -416,22 +427,23 @@ int trainer_backward(vh_denoiser_trainer* tr, hipStream_t s) {
         a.w = w[l];
         a.relu = 0;
         a.cout = ci;
+        a.images = N;
         const bool odd = ci != 16 && ci != 32 && ci != 64 && ci != 96 && ci != 128;
         int rc = launch_conv(a, (uint32_t)tr->device->cu_count, s, odd);
         if (rc) return rc;
         // route it to where the input came from
         if (in.x1 <= -2) {  // a pool of the finer level's encoder output (layer 2 (l - 1) + 1)
             const int src = 2 * (l - 1) + 1, fh = h[l - 1], fw = w[l - 1];
-            const size_t n = (size_t)fh * fw * ci;
+            const size_t n = (size_t)N * fh * fw * ci;
             hipLaunchKernelGGL(k_route, dim3(nblocks(n, 256)), dim3(256), 0, s, dx, h[l], w[l], ci, 0, 1, g_at(src),
-                               (float*)nullptr, fh, fw);
+                               (float*)nullptr, fh, fw, N);
         } else if (in.up1) {
             const int c1 = kShape[in.x1][1], c2 = kShape[in.x2][1];
             hipLaunchKernelGGL(k_route, dim3(nblocks(P * ci, 256)), dim3(256), 0, s, dx, h[l], w[l], c1, c2, 2,
-                               g_at(in.x1), g_at(in.x2), 0, 0);
+                               g_at(in.x1), g_at(in.x2), 0, 0, N);
         } else {
             hipLaunchKernelGGL(k_route, dim3(nblocks(P * ci, 256)), dim3(256), 0, s, dx, h[l], w[l], ci, 0, 0,
-                               g_at(in.x1), (float*)nullptr, 0, 0);
+                               g_at(in.x1), (float*)nullptr, 0, 0, N);
         }
     }
     VH_HIP(hipGetLastError());
@@ -565,27 +577,27 @@ int vh_trainer_step(vh_denoiser_trainer* tr, uint32_t images, uint32_t width, ui
     if (!tr || !films || !guides || !targets || images == 0) return fail(VH_ERR_USAGE, "vh_trainer_step: null argument");
     if (width < 2 || height < 2 || (uint64_t)width * height > (1ull << 22))
         return fail(VH_ERR_USAGE, "vh_trainer_step: image size out of range (2x2 to 4 Mpixels)");
+    if ((uint64_t)images * width * height > (1ull << 25))
+        return fail(VH_ERR_USAGE, "vh_trainer_step: a batch holds at most 2^25 pixels");
     if (!(samples > 0.f) || !std::isfinite(learning_rate) || learning_rate < 0.f)
         return fail(VH_ERR_USAGE, "vh_trainer_step: samples must be > 0 and the learning rate finite and >= 0");
     vh_device* dev = tr->device;
     VH_HIP(hipSetDevice(dev->ordinal));
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : dev->stream;
     std::lock_guard<std::mutex> hold(tr->lock);
-    int rc = trainer_buffers(tr, (int)width, (int)height);
+    int rc = trainer_buffers(tr, (int)width, (int)height, (int)images);
     if (rc) return rc;
-    const size_t P = (size_t)width * height;
+    const size_t P = (size_t)images * width * height;  // pixels of the batch
     hipLaunchKernelGGL(k_fill, dim3(grid_for(tr->nw)), dim3(256), 0, s, tr->gw.as<float>(), tr->nw, 0.f);
     hipLaunchKernelGGL(k_fill, dim3(grid_for(tr->nb)), dim3(256), 0, s, tr->gb.as<float>(), tr->nb, 0.f);
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, tr->loss.as<float>(), (size_t)1, 0.f);
-    const float inv_count = 1.f / (float)((double)images * P * 3.0);
-    for (uint32_t n = 0; n < images; n++) {
-        if ((rc = trainer_forward(tr, films + n * P * 4, guides + n * P * 8, samples, s))) return rc;
-        const uint16_t* y18 = reinterpret_cast<const uint16_t*>(tr->act.as<uint8_t>() + tr->y_off[kLayers - 1]);
-        float* g18 = reinterpret_cast<float*>(tr->grad.as<uint8_t>() + tr->g_off[kLayers - 1]);
-        hipLaunchKernelGGL(k_loss, dim3(nblocks(P, 256)), dim3(256), 0, s, y18, targets + n * P * 3, (int)P, inv_count,
-                           g18, tr->loss.as<float>());
-        if ((rc = trainer_backward(tr, s))) return rc;
-    }
+    const float inv_count = 1.f / (float)((double)P * 3.0);
+    if ((rc = trainer_forward(tr, films, guides, samples, s))) return rc;
+    const uint16_t* y18 = reinterpret_cast<const uint16_t*>(tr->act.as<uint8_t>() + tr->y_off[kLayers - 1]);
+    float* g18 = reinterpret_cast<float*>(tr->grad.as<uint8_t>() + tr->g_off[kLayers - 1]);
+    hipLaunchKernelGGL(k_loss, dim3(nblocks(P, 256)), dim3(256), 0, s, y18, targets, (int)P, inv_count, g18,
+                       tr->loss.as<float>());
+    if ((rc = trainer_backward(tr, s))) return rc;
     if (!(flags & VH_TRAIN_NO_UPDATE)) {
         tr->step++;
         const float b1 = 0.9f, b2 = 0.999f;

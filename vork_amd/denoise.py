@@ -283,6 +283,52 @@ def loss_fn(pred, target):
     return ((d - target) ** 2 / (target ** 2 + 1e-2)).mean() + (pred - torch.log1p(target)).abs().mean()
 
 
+HBM_BYTES_PER_S = 8.0e12      # MI355X HBM3E
+BF16_FLOPS_PER_S = 2.5e15     # MI355X dense bf16 MFMA
+
+
+def layer_costs(width, height):
+    """Per launch of one denoise at width x height: (name, flops, bytes) with
+    the multiply-adds x 2 each convolution runs (padded channel counts) and
+    the least HBM bytes it must move: its inputs read once (the decoder's
+    coarse input at its own resolution), its bf16 output written once (and
+    the encoder's 2x2 pool), its weights once.  The feature and output passes
+    move film, guides and radiance."""
+    h, w = [height], [width]
+    for _ in range(4):
+        h.append((h[-1] + 1) // 2)
+        w.append((w[-1] + 1) // 2)
+    px = [h[l] * w[l] for l in range(5)]
+    c = LEVELS
+    out = [("features", 0, px[0] * (16 + 32 + 64))]
+
+    def conv(name, l, c1, c2, cout, up=False, pool=False):
+        cin = c1 + c2
+        fl = 2 * px[l] * 9 * cin * cout
+        by = (px[l + 1] if up else px[l]) * c1 * 2 + px[l] * c2 * 2 + 9 * cin * cout * 2 + px[l] * cout * 2
+        if pool:
+            by += px[l + 1] * cout * 2
+        out.append((name, fl, by))
+
+    cin = 32
+    for l in range(5):
+        conv(f"enc{l}a", l, cin, 0, c[l])
+        conv(f"enc{l}b", l, c[l], 0, c[l], pool=l < 4)
+        cin = c[l]
+    for l in (3, 2, 1, 0):
+        conv(f"dec{l}a", l, c[l + 1], c[l], c[l], up=True)
+        conv(f"dec{l}b", l, c[l], 0, c[l])
+    conv("out", 0, c[0], 0, 16)
+    out.append(("output", 0, px[0] * (32 + 16)))
+    return out
+
+
+def speed_of_light_ms(width, height):
+    """Least time of one denoise: every launch at the larger of its MFMA time
+    and its HBM time."""
+    return sum(max(f / BF16_FLOPS_PER_S, b / HBM_BYTES_PER_S) for _, f, b in layer_costs(width, height)) * 1e3
+
+
 def flops(width, height):
     """Multiply-adds x 2 of one denoise (the padded channel counts the kernels run)."""
     h, w = [height], [width]

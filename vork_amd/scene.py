@@ -475,10 +475,11 @@ def flock_instances(positions, velocities, scales, seed=1):
     return inst
 
 
-def flock_frames(scene, frames, dt=1.0 / 24.0):
+def flock_frames(scene, frames, dt=1.0 / 24.0, progress=None):
     """Instance tables of `frames` successive animation frames of a config2
     scene (frame 0 is the scene's own table): each frame advects the birds by
-    the curl-noise velocity for `dt` and re-yaws them."""
+    the curl-noise velocity for `dt` and re-yaws them.  `progress(k)` is
+    called after every frame made."""
     base = scene.instances
     birds = base[1:]
     tr = birds["transform"].astype(np.float64)
@@ -492,6 +493,8 @@ def flock_frames(scene, frames, dt=1.0 / 24.0):
         f = base.copy()
         f[1:] = flock_instances(pos, curl_velocity(pos, t), scales)
         out.append(f)
+        if progress:
+            progress(len(out))
     return out
 
 
