@@ -103,85 +103,171 @@ __global__ void k_relu_grad(const float* g, const uint16_t* y, int pixels, int c
     dz[i] = f2bf(v);
 }
 
-// Input channel `ch` of pixel (y, x) of a layer: x1 (at half resolution when
-// up1) for ch < c1, else x2; zero outside the image.
+// Weight and bias gradients of one layer over the batch:
+//   dW[tap][co][ci] = sum_p dz[p][co] * x[p + offset(tap)][ci],  db[co] = sum_p dz[p][co]
+// as GEMMs with M = output channels, N = input channels and K = pixels.  A
+// workgroup owns 64 output channels (blockIdx.y), 32 input channels
+// (blockIdx.z) and every 8 x 16 pixel tile t = blockIdx.x + k gridDim.x of
+// the batch; per tile it stages the output gradient [128 pixels][64] and the
+// input's 10 x 18 halo [180 pixels][32] in LDS, and wave w computes taps
+// w, w + 4, w + 8 (its A fragments, the output gradient, are shared by its
+// taps; its B fragments are the input shifted by the tap), keeping all their
+// accumulators across the tiles.  Each workgroup writes its partial sums once;
+// k_wgrad_reduce adds the workgroups' partials to the gradients in a fixed
+// order, so the result does not depend on scheduling.
 struct WgradArgs {
-    const uint16_t* dz;   // [h][w][cp] output gradient (bf16, padded)
-    const uint16_t* x1;
-    const uint16_t* x2;
-    float* dw;            // [tap][cout][cin] fp32, accumulated
-    float* db;            // [cout] fp32, accumulated
-    int h, w, c1, c2, up1, cout, cp;
-    int images;           // a batch: every tensor holds its images one after the other
+    const uint16_t* dz;   // [images][h][w][cp] output gradient (bf16, padded)
+    const uint16_t* x1;   // [images][h or h/2][w or w/2][c1]
+    const uint16_t* x2;   // [images][h][w][c2] or null
+    float* part;          // [gridDim.x][gridDim.y][gridDim.z][kWgradPart] partial sums
+    int h, w, c1, c2, up1, cout, cp, images;
 };
-__device__ __forceinline__ uint16_t input_at(const WgradArgs& a, int img, int y, int x, int ch) {
-    if (y < 0 || x < 0 || y >= a.h || x >= a.w) return 0;
+constexpr int kWgradW = 9 * 64 * 32;          // one workgroup's weight partials
+constexpr int kWgradPart = kWgradW + 64;      // and its bias partials
+constexpr int kDzStride = 64 + 8;             // bf16 per staged pixel of dz
+constexpr int kXStride = 32 + 8;              // bf16 per staged halo pixel
+
+// 8 input channels [ch, ch + 8) of pixel (y, x) of image `img`: zero outside.
+__device__ __forceinline__ uint4 wgrad_fetch(const WgradArgs& a, int img, int y, int x, int ch) {
+    if (y < 0 || x < 0 || y >= a.h || x >= a.w) return make_uint4(0, 0, 0, 0);
     if (ch < a.c1) {
         const int wh = a.up1 ? (a.w + 1) / 2 : a.w, hh = a.up1 ? (a.h + 1) / 2 : a.h;
         const int yy = a.up1 ? y >> 1 : y, xx = a.up1 ? x >> 1 : x;
-        return a.x1[(((size_t)img * hh + yy) * wh + xx) * a.c1 + ch];
+        return *reinterpret_cast<const uint4*>(a.x1 + (((size_t)img * hh + yy) * wh + xx) * a.c1 + ch);
     }
-    return a.x2[(((size_t)img * a.h + y) * a.w + x) * a.c2 + (ch - a.c1)];
+    return *reinterpret_cast<const uint4*>(a.x2 + (((size_t)img * a.h + y) * a.w + x) * a.c2 + (ch - a.c1));
 }
 
-// dW[tap][co][ci] += sum_p dz[p][co] * x[p + offset(tap)][ci] and
-// db[co] += sum_p dz[p][co].  Block: 256 pixels (8 K steps of 32), 64 output
-// channels (16 per wave) from blockIdx.y, one tap and 32 input channels from
-// blockIdx.z; each wave keeps a 16 x 32 tile of dW in two MFMA accumulators
-// and adds it to the fp32 gradient once.
-constexpr int kWgradPixels = 256;
 __global__ __launch_bounds__(256) void k_wgrad(WgradArgs a) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __shared__ __attribute__((aligned(16))) uint16_t sdz[kTileH * kTileW * kDzStride];
+    __shared__ __attribute__((aligned(16))) uint16_t sx[kHaloH * kHaloW * kXStride];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int col = lane & 15, kg = lane >> 4;
-    const int co = blockIdx.y * 64 + wave * 16 + col;       // this lane's A row (output channel)
-    const int tap = blockIdx.z % 9, cb = (blockIdx.z / 9) * 32;
-    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-    const int cin = a.c1 + a.c2;
-    const int pixels = a.h * a.w, total = pixels * a.images;  // dz holds the batch's pixels in order
-    const int p0 = blockIdx.x * kWgradPixels;
-    f32x4 acc[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
-    float bsum = 0.f;
-    for (int ks = 0; ks < kWgradPixels / 32; ks++) {
-        uint16_t av[8], bv[2][8];
+    const int co0 = blockIdx.y * 64, cb = blockIdx.z * 32;
+    const int tiles_x = (a.w + kTileW - 1) / kTileW, per_image = tiles_x * ((a.h + kTileH - 1) / kTileH);
+    const int tiles = per_image * a.images;
+    f32x4 acc[3][4][2];
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const int p = p0 + ks * 32 + kg * 8 + j;
-            const bool in = p < total;
-            av[j] = in && co < a.cp ? a.dz[(size_t)p * a.cp + co] : (uint16_t)0;
-            const int img = in ? p / pixels : 0, lp = p - img * pixels;
-            const int y = in ? lp / a.w : -10, x = in ? lp % a.w : 0;
+    for (int t = 0; t < 3; t++)
 #pragma unroll
-            for (int nb = 0; nb < 2; nb++) {
-                const int ci = cb + nb * 16 + col;
-                bv[nb][j] = in && ci < cin ? input_at(a, img, y + dy, x + dx, ci) : (uint16_t)0;
+        for (int m = 0; m < 4; m++)
+#pragma unroll
+            for (int n = 0; n < 2; n++) acc[t][m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+    float bsum[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const int img = t / per_image, r = t - img * per_image;
+        const int y0 = (r / tiles_x) * kTileH, x0 = (r % tiles_x) * kTileW;
+        __syncthreads();
+        for (int i = tid; i < kTileH * kTileW * 8; i += 256) {
+            const int px = i >> 3, q = i & 7;
+            const int y = y0 + px / kTileW, x = x0 + px % kTileW, co = co0 + q * 8;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (y < a.h && x < a.w && co < a.cp)
+                v = *reinterpret_cast<const uint4*>(a.dz + (((size_t)img * a.h + y) * a.w + x) * a.cp + co);
+            *reinterpret_cast<uint4*>(sdz + px * kDzStride + q * 8) = v;
+        }
+        for (int i = tid; i < kHaloH * kHaloW * 4; i += 256) {
+            const int px = i >> 2, q = i & 3;
+            *reinterpret_cast<uint4*>(sx + px * kXStride + q * 8) =
+                wgrad_fetch(a, img, y0 + px / kHaloW - 1, x0 + px % kHaloW - 1, cb + q * 8);
+        }
+        __syncthreads();
+        // K step s: tile rows 2s and 2s + 1; lane group kg holds pixels
+        // (2s + kg / 2, 8 (kg % 2) .. 8 (kg % 2) + 7)
+#pragma unroll 1
+        for (int st = 0; st < kTileH / 2; st++) {
+            const int prow = 2 * st + (kg >> 1), pcol = (kg & 1) * 8;
+            bf16x8 A[4];
+#pragma unroll
+            for (int m = 0; m < 4; m++)
+#pragma unroll
+                for (int j = 0; j < 8; j++)
+                    A[m][j] = __builtin_bit_cast(__bf16, sdz[(prow * kTileW + pcol + j) * kDzStride + m * 16 + col]);
+            if (wave == 0 && blockIdx.z == 0) {
+#pragma unroll
+                for (int m = 0; m < 4; m++)
+#pragma unroll
+                    for (int j = 0; j < 8; j++) bsum[m] += (float)A[m][j];
             }
-            bsum += bf2f(av[j]);
-        }
-        bf16x8 A, B0, B1;
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-            A[j] = __builtin_bit_cast(__bf16, av[j]);
-            B0[j] = __builtin_bit_cast(__bf16, bv[0][j]);
-            B1[j] = __builtin_bit_cast(__bf16, bv[1][j]);
-        }
-        acc[0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B0, acc[0], 0, 0, 0);
-        acc[1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B1, acc[1], 0, 0, 0);
-    }
-    // D[m][n]: lane holds rows m = 4 kg + i (output channels) of column n = col
+            for (int ti = 0; ti < 3; ti++) {
+                const int tap = wave + 4 * ti;
+                if (tap >= 9) break;
+                const int dy = tap / 3, dx = tap % 3;
 #pragma unroll
-    for (int nb = 0; nb < 2; nb++) {
-        const int ci = cb + nb * 16 + col;
+                for (int n = 0; n < 2; n++) {
+                    bf16x8 B;
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int o = blockIdx.y * 64 + wave * 16 + kg * 4 + i;
-            if (o < a.cout && ci < cin && acc[nb][i] != 0.f)
-                atomicAdd(&a.dw[((size_t)tap * a.cout + o) * cin + ci], acc[nb][i]);
+                    for (int j = 0; j < 8; j++)
+                        B[j] = __builtin_bit_cast(
+                            __bf16, sx[((prow + dy) * kHaloW + pcol + j + dx) * kXStride + n * 16 + col]);
+#pragma unroll
+                    for (int m = 0; m < 4; m++)
+                        if (co0 + m * 16 < a.cout)
+                            acc[ti][m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[m], B, acc[ti][m][n], 0, 0, 0);
+                }
+            }
         }
     }
-    if (blockIdx.z == 0) {
-        bsum += __shfl_xor(bsum, 16);
-        bsum += __shfl_xor(bsum, 32);
-        if (kg == 0 && co < a.cout && bsum != 0.f) atomicAdd(&a.db[co], bsum);
+    // D fragment: lane holds output channels m * 16 + 4 kg + i of input channel n * 16 + col
+    float* part = a.part + ((size_t)(blockIdx.x * gridDim.y + blockIdx.y) * gridDim.z + blockIdx.z) * kWgradPart;
+#pragma unroll
+    for (int ti = 0; ti < 3; ti++) {
+        const int tap = wave + 4 * ti;
+        if (tap >= 9) break;
+#pragma unroll
+        for (int m = 0; m < 4; m++)
+#pragma unroll
+            for (int n = 0; n < 2; n++)
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    part[(tap * 64 + m * 16 + 4 * kg + i) * 32 + n * 16 + col] = acc[ti][m][n][i];
+    }
+    if (wave == 0) {
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            float v = bsum[m];
+            v += __shfl_xor(v, 16);
+            v += __shfl_xor(v, 32);
+            if (kg == 0) part[kWgradW + m * 16 + col] = v;  // zero unless blockIdx.z == 0
+        }
+    }
+}
+
+// dW[tap][co][ci] += the workgroups' partials; db[co] likewise from the
+// partials of input-channel block 0.  A block reduces 32 consecutive
+// gradient entries: its 8 thread rows each sum every 8th workgroup's
+// partials, then row 0 adds the 8 row sums, always in the same order.
+constexpr int kReduceRows = 8;
+__global__ __launch_bounds__(256) void k_wgrad_reduce(const float* part, int chunks, int ny, int nz, int cout,
+                                                      int cin, float* dw, float* db) {
+    __shared__ float rows[kReduceRows][33];
+    const int e = threadIdx.x & 31, g = threadIdx.x >> 5;
+    const size_t i = (size_t)blockIdx.x * 32 + e;
+    const size_t nw = (size_t)9 * cout * cin;
+    size_t at = 0, first = 0;  // the entry's offset in a partial block, and its first block
+    bool valid = true;
+    if (i < nw) {
+        const int ci = (int)(i % cin), co = (int)((i / cin) % cout), tap = (int)(i / ((size_t)cin * cout));
+        at = (size_t)(tap * 64 + co % 64) * 32 + ci % 32;
+        first = (size_t)(co / 64) * nz + ci / 32;
+    } else if (i < nw + cout) {
+        const int co = (int)(i - nw);
+        at = (size_t)kWgradW + co % 64;
+        first = (size_t)(co / 64) * nz;
+    } else {
+        valid = false;
+    }
+    float sum = 0.f;
+    if (valid)
+        for (int c = g; c < chunks; c += kReduceRows) sum += part[((size_t)c * ny * nz + first) * kWgradPart + at];
+    rows[g][e] = sum;
+    __syncthreads();
+    if (g == 0 && valid) {
+        float t = 0.f;
+        for (int k = 0; k < kReduceRows; k++) t += rows[k][e];
+        if (i < nw) dw[i] += t;
+        else db[i - nw] += t;
     }
 }
 
@@ -295,6 +381,7 @@ struct vh_denoiser_trainer {
     uint32_t b_off[vh::kLayers + 1] = {};
     vh::DeviceBuffer w, mw, vw, gw, wb, wt, b, mb, vb, gb, zero_bias, loss;
     vh::DeviceBuffer act, grad, tmp;    // activations, fp32 gradients, bf16 temporaries
+    vh::DeviceBuffer wpart;             // weight-gradient partial sums of k_wgrad's workgroups
     int act_w = 0, act_h = 0, act_n = 0;  // size and images the buffers hold
     size_t y_off[vh::kLayers] = {}, pool_off[4] = {}, feat_off = 0;  // in `act` (bytes)
     size_t g_off[vh::kLayers] = {};                                  // in `grad` (bytes)
@@ -305,6 +392,9 @@ struct vh_denoiser_trainer {
 
 namespace vh {
 namespace {
+
+// Workgroups of a k_wgrad launch (4 per CU), hence partial-sum slots.
+inline int wgrad_blocks(const vh_denoiser_trainer* tr) { return 4 * tr->device->cu_count; }
 
 int trainer_buffers(vh_denoiser_trainer* tr, int W, int H, int N) {
     if (tr->act_w == W && tr->act_h == H && tr->act_n == N) return VH_OK;
@@ -405,15 +495,19 @@ int trainer_backward(vh_denoiser_trainer* tr, hipStream_t s) {
         wa.up1 = in.up1;
         wa.x2 = in.x2 >= 0 ? y_at(in.x2) : nullptr;
         wa.c2 = in.x2 >= 0 ? kShape[in.x2][1] : 0;
-        wa.dw = tr->gw.as<float>() + tr->table.w_off[L];
-        wa.db = tr->gb.as<float>() + tr->b_off[L];
+        wa.part = tr->wpart.as<float>();
         wa.h = h[l];
         wa.w = w[l];
         wa.cout = co;
         wa.cp = cp;
         wa.images = N;
-        const dim3 wg(nblocks(P, kWgradPixels), (co + 63) / 64, 9 * ((ci + 31) / 32));
-        hipLaunchKernelGGL(k_wgrad, wg, dim3(256), 0, s, wa);
+        const int ny = (co + 63) / 64, nz = ci / 32;
+        const int tiles = (int)(nblocks(w[l], kTileW) * nblocks(h[l], kTileH)) * N;
+        const int chunks = std::max(1, std::min(tiles, wgrad_blocks(tr) / (ny * nz)));
+        hipLaunchKernelGGL(k_wgrad, dim3(chunks, ny, nz), dim3(256), 0, s, wa);
+        const size_t nred = (size_t)9 * co * ci + co;
+        hipLaunchKernelGGL(k_wgrad_reduce, dim3(nblocks(nred, 32)), dim3(256), 0, s, tr->wpart.as<float>(), chunks,
+                           ny, nz, co, ci, tr->gw.as<float>() + tr->table.w_off[L], tr->gb.as<float>() + tr->b_off[L]);
         if (L == 0) break;  // the features need no gradient
         // input gradient: the convolution of dz with the flipped, transposed weights
         ConvArgs a;
@@ -550,6 +644,7 @@ int vh_trainer_create(vh_device* dev, const void* blob, size_t bytes, uint32_t s
     if (e == hipSuccess) e = tr->wt.allocate(nwt * 2);
     if (e == hipSuccess) e = tr->zero_bias.allocate(256 * 4);
     if (e == hipSuccess) e = tr->loss.allocate(16);
+    if (e == hipSuccess) e = tr->wpart.allocate((size_t)wgrad_blocks(tr) * kWgradPart * sizeof(float));
     if (e != hipSuccess) return bail(e);
     hipStream_t s = dev->stream;
     VH_HIP(hipMemcpyAsync(tr->w.ptr, w.data(), nw * 4, hipMemcpyHostToDevice, s));
