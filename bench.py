@@ -524,7 +524,7 @@ def serve_bench(args):
         "evictions": server.evictions}
 
 
-CPU_SHARE = 16   # host cores a one-GPU box gives its jobs
+CPU_SHARE = 16   # host cores of one GPU's share of a multi-GPU host
 
 
 def cpu_baseline(sc, samples=8):
@@ -548,7 +548,7 @@ def cpu_baseline(sc, samples=8):
             "host_cpus_visible": visible,
             "sample": f"exposure samples 0-{samples - 1} of {sc.spp} for all {sc.width}x{sc.height} pixels "
                       f"({n} samples) in {dt:.1f}s on {threads} threads (all host cores of the run, at most the "
-                      f"one-GPU box's share of {CPU_SHARE}); oracle BVH build {build_s:.1f}s untimed"}
+                      f"one GPU's share of {CPU_SHARE}); oracle BVH build {build_s:.1f}s untimed"}
 
 
 def _rocprof_counter(counter, argv, outdir, timeout=300):

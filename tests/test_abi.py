@@ -47,6 +47,8 @@ def test_usage_errors_need_no_device():
 
 
 def test_rust_binding_in_the_integration_notes_covers_every_entry_point():
+    """The Rust binding written out for the reference side binds exactly the
+    header's entry points (one `pub fn` per symbol, none missing or extra)."""
     import os
     import re
     here = os.path.dirname(os.path.abspath(__file__))

@@ -14,11 +14,8 @@
 //     most `leaf_max` primitives become leaf children whose items are written
 //     contiguously per node; children are placed in octant-ordered slots
 //
-// rocprim's radix sort and prefix sum are the only library calls; all other
-// passes are kernels written here.
+// Every pass, the radix sort included, is a kernel written here.
 #include <cstring>
-#include <rocprim/device/device_radix_sort.hpp>
-#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -227,6 +224,87 @@ __global__ __launch_bounds__(1024) void k_scan_sums(uint64_t* sums, const uint32
         if (i < tiles) sums[i] = carry + incl - v;
         carry += sh[1023];
         __syncthreads();
+    }
+}
+
+// ---- LSD radix sort of (63-bit Morton key, id) pairs, 8 bits per pass.
+// Per pass: k_radix_hist counts each tile's digits; the counts, digit-major
+// ([digit][tile]), go through the exclusive scan above, so digit d of tile t
+// starts after every smaller digit and after digit d of the tiles before t;
+// k_radix_scatter then ranks each tile's keys stably (in index order) within
+// their digit and writes them there.
+constexpr int kRadixBlock = 256, kRadixItems = 16, kRadixTile = kRadixBlock * kRadixItems;
+constexpr int kRadixBins = 256;  // one per thread of a block
+static_assert(kRadixBins == kRadixBlock, "a thread per digit");
+
+__global__ __launch_bounds__(kRadixBlock) void k_radix_hist(const uint64_t* keys, uint32_t n, int shift,
+                                                           uint64_t* counts, uint32_t tiles) {
+    __shared__ uint32_t h[kRadixBins];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * kRadixTile;
+    for (int r = 0; r < kRadixItems; r++) {
+        const uint32_t i = base + r * kRadixBlock + threadIdx.x;
+        if (i < n) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & 0xffu], 1u);
+    }
+    __syncthreads();
+    counts[(size_t)threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// The tile's keys in rounds of one per thread (round r: keys base + 256 r +
+// thread), so index order is (round, wave, lane).  Within a wave the lanes of
+// one digit find each other by 8 ballots; a lane's rank is the count of its
+// peers below it; the waves' counts per digit are laid out in wave order in
+// LDS and the tile's running slot per digit advances by the round's total.
+__global__ __launch_bounds__(kRadixBlock) void k_radix_scatter(const uint64_t* keys, const uint32_t* vals, uint32_t n,
+                                                              int shift, const uint64_t* offsets,
+                                                              const uint64_t* tile_sums, uint32_t tiles,
+                                                              uint64_t* keys_out, uint32_t* vals_out) {
+    constexpr int kWaves = kRadixBlock / 64;
+    __shared__ uint32_t next_slot[kRadixBins];
+    __shared__ uint32_t wave_slot[kWaves][kRadixBins];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    {
+        const size_t j = (size_t)tid * tiles + blockIdx.x;
+        next_slot[tid] = (uint32_t)(offsets[j] + tile_sums[j / kScanTile]);
+    }
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint32_t base = blockIdx.x * kRadixTile;
+    for (int r = 0; r < kRadixItems; r++) {
+        const uint32_t i = base + r * kRadixBlock + tid;
+        const bool valid = i < n;
+        const uint64_t key = valid ? keys[i] : 0ull;
+        const uint32_t val = valid ? vals[i] : 0u;
+        const uint32_t d = (uint32_t)(key >> shift) & 0xffu;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const uint64_t ones = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? ones : ~ones;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & below);
+#pragma unroll
+        for (int w = 0; w < kWaves; w++) wave_slot[w][tid] = 0u;
+        __syncthreads();
+        if (valid && rank == 0) wave_slot[wave][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        {
+            uint32_t run = next_slot[tid];
+#pragma unroll
+            for (int w = 0; w < kWaves; w++) {
+                const uint32_t c = wave_slot[w][tid];
+                wave_slot[w][tid] = run;
+                run += c;
+            }
+            next_slot[tid] = run;
+        }
+        __syncthreads();
+        if (valid) {
+            const uint32_t pos = wave_slot[wave][d] + rank;
+            keys_out[pos] = key;
+            vals_out[pos] = val;
+        }
+        __syncthreads();  // wave_slot is cleared by the next round
     }
 }
 
@@ -697,13 +775,35 @@ int build_wide_bvh(vh_device* dev, hipStream_t s, BuildScratch& B, const float* 
     hipLaunchKernelGGL(k_centroid_bounds, dim3(rb), dim3(256), 0, s, d_lo, d_hi, n, bounds.as<uint32_t>());
     hipLaunchKernelGGL(k_morton, dim3(blocks(n, 256)), dim3(256), 0, s, d_lo, d_hi, n, bounds.as<uint32_t>(),
                        keys.as<uint64_t>(), ids.as<uint32_t>());
-    size_t tmp_bytes = 0;
-    VH_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, keys.as<uint64_t>(), keys_sorted.as<uint64_t>(),
-                                     ids.as<uint32_t>(), ids_sorted.as<uint32_t>(), n, 0, 63, s));
-    VH_HIP(tmp.reserve(tmp_bytes));
-    VH_HIP(rocprim::radix_sort_pairs(tmp.ptr, tmp_bytes, keys.as<uint64_t>(), keys_sorted.as<uint64_t>(),
-                                     ids.as<uint32_t>(), ids_sorted.as<uint32_t>(), n, 0, 63, s));
-    hipLaunchKernelGGL(k_ploc_init, dim3(blocks(n, 256)), dim3(256), 0, s, d_lo, d_hi, ids_sorted.as<uint32_t>(), n,
+    const uint32_t* sorted_ids = nullptr;
+    // 8 passes of 8 bits over the 63-bit keys, back and forth between the two
+    // buffers: the sorted ids end where the keys started
+    {
+        const uint32_t tiles = (n + kRadixTile - 1) / kRadixTile;
+        const size_t entries = (size_t)kRadixBins * tiles;
+        const size_t scan_tiles = (entries + kScanTile - 1) / kScanTile;
+        VH_HIP(tmp.reserve(sizeof(uint64_t) * (2 * entries + scan_tiles) + 64));
+        uint64_t* counts = tmp.as<uint64_t>();
+        uint64_t* offsets = counts + entries;
+        uint64_t* sums = offsets + entries;
+        uint32_t* sort_state = reinterpret_cast<uint32_t*>(sums + scan_tiles);
+        const uint32_t sort_init[8] = {(uint32_t)entries, 0, 0, 0, 0, 0, 0, 0};
+        VH_HIP(hipMemcpyAsync(sort_state, sort_init, sizeof(sort_init), hipMemcpyHostToDevice, s));
+        uint64_t *k_in = keys.as<uint64_t>(), *k_out = keys_sorted.as<uint64_t>();
+        uint32_t *v_in = ids.as<uint32_t>(), *v_out = ids_sorted.as<uint32_t>();
+        for (int shift = 0; shift < 64; shift += 8) {
+            hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(kRadixBlock), 0, s, k_in, n, shift, counts, tiles);
+            hipLaunchKernelGGL(k_scan_tiles, dim3((uint32_t)scan_tiles), dim3(kScanBlock), 0, s, counts, sort_state,
+                               offsets, sums);
+            hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, sums, sort_state);
+            hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(kRadixBlock), 0, s, k_in, v_in, n, shift, offsets,
+                               sums, tiles, k_out, v_out);
+            std::swap(k_in, k_out);
+            std::swap(v_in, v_out);
+        }
+        sorted_ids = v_in;
+    }
+    hipLaunchKernelGGL(k_ploc_init, dim3(blocks(n, 256)), dim3(256), 0, s, d_lo, d_hi, sorted_ids, n,
                        nb.as<float>(), size.as<uint32_t>(), ca.as<uint32_t>());
     // Clustering rounds run from device-resident state ([0] clusters, [1]
     // next node id, [2..3] the round's totals, [4] no-progress flag); the host
@@ -757,7 +857,7 @@ int build_wide_bvh(vh_device* dev, hipStream_t s, BuildScratch& B, const float* 
     VH_HIP(hipMemcpyAsync(counters.ptr, init, sizeof(init), hipMemcpyHostToDevice, s));
     QueueItem root = {root_id, 0u};
     VH_HIP(hipMemcpyAsync(qa.ptr, &root, sizeof(root), hipMemcpyHostToDevice, s));
-    Tree t{left.as<uint32_t>(), right.as<uint32_t>(), size.as<uint32_t>(), ids_sorted.as<uint32_t>(), nb.as<float>(),
+    Tree t{left.as<uint32_t>(), right.as<uint32_t>(), size.as<uint32_t>(), sorted_ids, nb.as<float>(),
            n, B.dp_dec.as<uint8_t>()};
     // Collapse level by level, reading the queue length back every kLevels
     // levels; a level's queue is at most kWidth times the last one.
