@@ -254,20 +254,37 @@ __global__ __launch_bounds__(kRadixBlock) void k_radix_hist(const uint64_t* keys
 // The tile's keys in rounds of one per thread (round r: keys base + 256 r +
 // thread), so index order is (round, wave, lane).  Within a wave the lanes of
 // one digit find each other by 8 ballots; a lane's rank is the count of its
-// peers below it; the waves' counts per digit are laid out in wave order in
-// LDS and the tile's running slot per digit advances by the round's total.
+// peers below it; the waves' counts per digit are laid out in wave order and
+// a running slot per digit advances by each round's total.  That places every
+// key at its stable position in the tile sorted by digit, in LDS; the sorted
+// tile then goes out in index order, so the keys of one digit, consecutive
+// in LDS, land in consecutive slots of the output (writes in runs).
 __global__ __launch_bounds__(kRadixBlock) void k_radix_scatter(const uint64_t* keys, const uint32_t* vals, uint32_t n,
-                                                              int shift, const uint64_t* offsets,
-                                                              const uint64_t* tile_sums, uint32_t tiles,
-                                                              uint64_t* keys_out, uint32_t* vals_out) {
+                                                              int shift, const uint64_t* counts,
+                                                              const uint64_t* offsets, const uint64_t* tile_sums,
+                                                              uint32_t tiles, uint64_t* keys_out, uint32_t* vals_out) {
     constexpr int kWaves = kRadixBlock / 64;
-    __shared__ uint32_t next_slot[kRadixBins];
+    __shared__ uint64_t tile_keys[kRadixTile];
+    __shared__ uint32_t tile_vals[kRadixTile];
+    __shared__ uint32_t out_base[kRadixBins];    // the digit's first output slot minus its first tile slot
+    __shared__ uint32_t next_slot[kRadixBins];   // the digit's next slot in the sorted tile
     __shared__ uint32_t wave_slot[kWaves][kRadixBins];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    {
-        const size_t j = (size_t)tid * tiles + blockIdx.x;
-        next_slot[tid] = (uint32_t)(offsets[j] + tile_sums[j / kScanTile]);
+    const size_t j = (size_t)tid * tiles + blockIdx.x;
+    const uint32_t count = (uint32_t)counts[j];
+    // exclusive scan of the tile's digit counts: each digit's first slot in the sorted tile
+    wave_slot[0][tid] = count;
+    __syncthreads();
+    for (int off = 1; off < kRadixBins; off <<= 1) {
+        const uint32_t x = tid >= off ? wave_slot[0][tid - off] : 0u;
+        __syncthreads();
+        wave_slot[0][tid] += x;
+        __syncthreads();
     }
+    const uint32_t first = wave_slot[0][tid] - count;
+    next_slot[tid] = first;
+    out_base[tid] = (uint32_t)(offsets[j] + tile_sums[j / kScanTile]) - first;
+    __syncthreads();
     const uint64_t below = (1ull << lane) - 1ull;
     const uint32_t base = blockIdx.x * kRadixTile;
     for (int r = 0; r < kRadixItems; r++) {
@@ -301,10 +318,17 @@ __global__ __launch_bounds__(kRadixBlock) void k_radix_scatter(const uint64_t* k
         __syncthreads();
         if (valid) {
             const uint32_t pos = wave_slot[wave][d] + rank;
-            keys_out[pos] = key;
-            vals_out[pos] = val;
+            tile_keys[pos] = key;
+            tile_vals[pos] = val;
         }
         __syncthreads();  // wave_slot is cleared by the next round
+    }
+    const uint32_t in_tile = n - base < (uint32_t)kRadixTile ? n - base : (uint32_t)kRadixTile;
+    for (uint32_t k = tid; k < in_tile; k += kRadixBlock) {
+        const uint64_t key = tile_keys[k];
+        const uint32_t pos = out_base[(uint32_t)(key >> shift) & 0xffu] + k;
+        keys_out[pos] = key;
+        vals_out[pos] = tile_vals[k];
     }
 }
 
@@ -796,8 +820,8 @@ int build_wide_bvh(vh_device* dev, hipStream_t s, BuildScratch& B, const float* 
             hipLaunchKernelGGL(k_scan_tiles, dim3((uint32_t)scan_tiles), dim3(kScanBlock), 0, s, counts, sort_state,
                                offsets, sums);
             hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, sums, sort_state);
-            hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(kRadixBlock), 0, s, k_in, v_in, n, shift, offsets,
-                               sums, tiles, k_out, v_out);
+            hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(kRadixBlock), 0, s, k_in, v_in, n, shift, counts,
+                               offsets, sums, tiles, k_out, v_out);
             std::swap(k_in, k_out);
             std::swap(v_in, v_out);
         }
