@@ -57,7 +57,7 @@ KNOBS = {
     "VH_GRID_PATHS_PER_THREAD": [8, 16, 32],
     "VH_BATCH_PATHS_LOG2": [25, 26, 27],
     "VH_LEAF_MAX": [1, 2, 4],
-    "VH_PLOC_RADIUS": [8, 16, 24],
+    "VH_PLOC_RADIUS": [8, 16, 24, 32, 48],
     "VH_SHADE_SORT": [0, 1],
 }
 # knobs swept when none are named (VH_SHADE_SORT is measured by bench.py --shade instead)
