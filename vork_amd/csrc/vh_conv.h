@@ -18,6 +18,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "vh_internal.h"
 
 namespace vh {
@@ -76,6 +78,86 @@ __device__ __forceinline__ uint4 fetch_chunk(const ConvArgs& a, int y, int x, in
         return *reinterpret_cast<const uint4*>(a.x1 + ((size_t)yy * wh + xx) * a.c1 + ch);
     }
     return *reinterpret_cast<const uint4*>(a.x2 + ((size_t)y * a.w + x) * a.c2 + (ch - a.c1));
+}
+
+// The epilogue of a tile through LDS: bias, ReLU and bf16 rounding per D
+// fragment (a lane holds output channel (block + col) of pixels 4 kg + j of a
+// row) into an output tile of the block's kCout channels per pixel at `tile`,
+// then whole 16-byte chunks (8 channels) to memory, and the 2x2 pool from the
+// rounded tile.  The caller makes sure no wave still reads what `tile`
+// overlays; the function ends with the tile read (no closing barrier).
+template <int kCout, int kMW>
+__device__ __forceinline__ void conv_epilogue(const ConvArgs& a, f32x4 (&acc)[kTileH / kMW][(kCout / 16) / (4 / kMW)],
+                                              uint16_t* tile, int y0, int x0, int co0) {
+    constexpr int kNB = kCout / 16, kNW = 4 / kMW, kNBW = kNB / kNW, kRows = kTileH / kMW;
+    constexpr int kOutPix = kCout + 8;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave % kMW, wn = wave / kMW;
+    const int kg = lane >> 4, col = lane & 15;
+#pragma unroll
+    for (int r = 0; r < kRows; r++) {
+        const int ty = wm * kRows + r;
+#pragma unroll
+        for (int n = 0; n < kNBW; n++) {
+            const int c = (wn * kNBW + n) * 16 + col;
+            const float bias = a.bias[co0 + c];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                float v = acc[r][n][j] + bias;
+                if (a.relu) v = fmaxf(v, 0.f);
+                tile[(ty * kTileW + kg * 4 + j) * kOutPix + c] = f2bf(v);
+            }
+        }
+    }
+    __syncthreads();
+    constexpr int kChunks = kCout / 8;  // 16-byte chunks per pixel
+    for (int i = tid; i < kTileH * kTileW * kChunks; i += 256) {
+        const int px = i / kChunks, q = i % kChunks;
+        const int y = y0 + px / kTileW, x = x0 + px % kTileW;
+        if (y < a.h && x < a.w)
+            *reinterpret_cast<uint4*>(a.y + ((size_t)y * a.w + x) * a.cout + co0 + q * 8) =
+                *reinterpret_cast<const uint4*>(tile + px * kOutPix + q * 8);
+    }
+    if (a.pool) {
+        const int ow = (a.w + 1) / 2;
+        for (int i = tid; i < (kTileH / 2) * (kTileW / 2) * kChunks; i += 256) {
+            const int g = i / kChunks, q = i % kChunks;
+            const int gy = g / (kTileW / 2), gx = g % (kTileW / 2);
+            const int y = y0 + 2 * gy, x = x0 + 2 * gx;
+            if (y >= a.h || x >= a.w) continue;
+            const bool right = x + 1 < a.w, down = y + 1 < a.h;
+            const int p00 = (2 * gy) * kTileW + 2 * gx;
+            const uint4 c00 = *reinterpret_cast<const uint4*>(tile + p00 * kOutPix + q * 8);
+            const uint4 c01 = *reinterpret_cast<const uint4*>(tile + (p00 + 1) * kOutPix + q * 8);
+            const uint4 c10 = *reinterpret_cast<const uint4*>(tile + (p00 + kTileW) * kOutPix + q * 8);
+            const uint4 c11 = *reinterpret_cast<const uint4*>(tile + (p00 + kTileW + 1) * kOutPix + q * 8);
+            const float cnt = (float)((1 + right) * (1 + down));
+            uint32_t o[4];
+            const uint32_t* w00 = &c00.x; const uint32_t* w01 = &c01.x;
+            const uint32_t* w10 = &c10.x; const uint32_t* w11 = &c11.x;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                uint32_t packed = 0;
+#pragma unroll
+                for (int hlf = 0; hlf < 2; hlf++) {
+                    const int sh = 16 * hlf;
+                    // sum in the order (0,0), (0,1), (1,0), (1,1); the last row / column of
+                    // an odd size averages what exists
+                    float sum = bf2f((uint16_t)(w00[k] >> sh));
+                    if (right) sum += bf2f((uint16_t)(w01[k] >> sh));
+                    if (down) {
+                        sum += bf2f((uint16_t)(w10[k] >> sh));
+                        if (right) sum += bf2f((uint16_t)(w11[k] >> sh));
+                    }
+                    packed |= (uint32_t)f2bf(sum / cnt) << sh;
+                }
+                o[k] = packed;
+            }
+            *reinterpret_cast<uint4*>(a.pool + ((size_t)(y / 2) * ow + x / 2) * a.cout + co0 + q * 8) =
+                make_uint4(o[0], o[1], o[2], o[3]);
+        }
+    }
+
 }
 
 // kCout output channels from blockIdx.y * kCout; waves split as kMW (pixel
@@ -145,75 +227,102 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs args) {
             }
         }
     }
-    // Epilogue through LDS: bias, ReLU and bf16 rounding per D fragment (a
-    // lane holds output channel (block + col) of pixels 4 kg + j of a row)
-    // into an output tile of the block's kCout channels per pixel, then
-    // whole 16-byte chunks (8 channels) to memory, and the 2x2 pool from the
-    // rounded tile.
     __syncthreads();  // every wave is done with the halo
-    uint16_t* tile = reinterpret_cast<uint16_t*>(halo);
+    conv_epilogue<kCout, kMW>(a, acc, reinterpret_cast<uint16_t*>(halo), y0, x0, co0);
+}
+
+// The same convolution with resident workgroups, for layers of at most 4
+// input-channel blocks of 32 (kKB): a workgroup loads its waves' weight fragments for
+// every tap and K block once, then walks tiles t = blockIdx.x + k gridDim.x
+// over the batch's images.  Each K block's halo is read into registers one
+// step ahead (3 chunks of 16 bytes per thread), so the loads of the next
+// step are in flight while the MFMAs of this one run; the weights are not
+// fetched again per tile.  The halo and the output tile have LDS regions of
+// their own, so the epilogue's global stores overlap the next step's loads.
+constexpr int kHaloChunks = kHaloH * kHaloW * 4;              // 16-byte chunks of one K block's halo
+constexpr int kChunksPerThread = (kHaloChunks + 255) / 256;
+
+template <int kCout, int kMW, int kKB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kKB == 1 ? 4 : kKB == 2 ? 3 : 2))) void k_conv3x3_resident(
+    ConvArgs args, int tiles_per_image, int tiles_total) {
+    constexpr int kNB = kCout / 16, kNW = 4 / kMW, kNBW = kNB / kNW, kRows = kTileH / kMW;
+    static_assert(kNB % kNW == 0 && kTileH % kMW == 0, "wave split");
+    constexpr int kOutPix = kCout + 8;
+    __shared__ __attribute__((aligned(16))) uint8_t halo[kHaloH * kHaloW * kPix];
+    __shared__ __attribute__((aligned(16))) uint16_t out_tile[kTileH * kTileW * kOutPix];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave % kMW, wn = wave / kMW;
+    const int kg = lane >> 4, col = lane & 15;
+    const int cin = args.c1 + args.c2;
+    const int tiles_x = (args.w + kTileW - 1) / kTileW;
+    // weights of every K block and tap (the layer's whole slice: co0 = 0)
+    bf16x8 b[kKB][9][kNBW];
 #pragma unroll
-    for (int r = 0; r < kRows; r++) {
-        const int ty = wm * kRows + r;
+    for (int kb = 0; kb < kKB; kb++)
 #pragma unroll
-        for (int n = 0; n < kNBW; n++) {
-            const int c = (wn * kNBW + n) * 16 + col;
-            const float bias = a.bias[co0 + c];
+        for (int tap = 0; tap < 9; tap++)
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                float v = acc[r][n][j] + bias;
-                if (a.relu) v = fmaxf(v, 0.f);
-                tile[(ty * kTileW + kg * 4 + j) * kOutPix + c] = f2bf(v);
+            for (int n = 0; n < kNBW; n++) {
+                const int co = (wn * kNBW + n) * 16 + col;
+                b[kb][tap][n] = __builtin_bit_cast(
+                    bf16x8, *reinterpret_cast<const uint4*>(args.wt + ((size_t)tap * args.cout + co) * cin + kb * kCB + kg * 8));
+            }
+    // one step = one K block of one tile; `pre` holds the next step's halo chunks
+    uint4 pre[kChunksPerThread];
+    auto fetch = [&](int t, int kb) {
+        const int img = t / tiles_per_image, r = t - img * tiles_per_image;
+        const ConvArgs a = conv_image(args, img);
+        const int y0 = (r / tiles_x) * kTileH, x0 = (r % tiles_x) * kTileW;
+#pragma unroll
+        for (int j = 0; j < kChunksPerThread; j++) {
+            const int i = tid + 256 * j;
+            pre[j] = make_uint4(0, 0, 0, 0);
+            if (i < kHaloChunks) {
+                const int px = i >> 2, q = i & 3;
+                pre[j] = fetch_chunk(a, y0 + px / kHaloW - 1, x0 + px % kHaloW - 1, kb * kCB + q * 8);
             }
         }
-    }
-    __syncthreads();
-    constexpr int kChunks = kCout / 8;  // 16-byte chunks per pixel
-    for (int i = tid; i < kTileH * kTileW * kChunks; i += 256) {
-        const int px = i / kChunks, q = i % kChunks;
-        const int y = y0 + px / kTileW, x = x0 + px % kTileW;
-        if (y < a.h && x < a.w)
-            *reinterpret_cast<uint4*>(a.y + ((size_t)y * a.w + x) * a.cout + co0 + q * 8) =
-                *reinterpret_cast<const uint4*>(tile + px * kOutPix + q * 8);
-    }
-    if (a.pool) {
-        const int ow = (a.w + 1) / 2;
-        for (int i = tid; i < (kTileH / 2) * (kTileW / 2) * kChunks; i += 256) {
-            const int g = i / kChunks, q = i % kChunks;
-            const int gy = g / (kTileW / 2), gx = g % (kTileW / 2);
-            const int y = y0 + 2 * gy, x = x0 + 2 * gx;
-            if (y >= a.h || x >= a.w) continue;
-            const bool right = x + 1 < a.w, down = y + 1 < a.h;
-            const int p00 = (2 * gy) * kTileW + 2 * gx;
-            const uint4 c00 = *reinterpret_cast<const uint4*>(tile + p00 * kOutPix + q * 8);
-            const uint4 c01 = *reinterpret_cast<const uint4*>(tile + (p00 + 1) * kOutPix + q * 8);
-            const uint4 c10 = *reinterpret_cast<const uint4*>(tile + (p00 + kTileW) * kOutPix + q * 8);
-            const uint4 c11 = *reinterpret_cast<const uint4*>(tile + (p00 + kTileW + 1) * kOutPix + q * 8);
-            const float cnt = (float)((1 + right) * (1 + down));
-            uint32_t o[4];
-            const uint32_t* w00 = &c00.x; const uint32_t* w01 = &c01.x;
-            const uint32_t* w10 = &c10.x; const uint32_t* w11 = &c11.x;
+    };
+    int t = blockIdx.x;
+    if (t < tiles_total) fetch(t, 0);
+    for (; t < tiles_total; t += gridDim.x) {
+        const int img = t / tiles_per_image, r = t - img * tiles_per_image;
+        const ConvArgs a = conv_image(args, img);
+        const int y0 = (r / tiles_x) * kTileH, x0 = (r % tiles_x) * kTileW;
+        f32x4 acc[kRows][kNBW];
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                uint32_t packed = 0;
+        for (int rr = 0; rr < kRows; rr++)
 #pragma unroll
-                for (int hlf = 0; hlf < 2; hlf++) {
-                    const int sh = 16 * hlf;
-                    // sum in the order (0,0), (0,1), (1,0), (1,1); the last row / column of
-                    // an odd size averages what exists
-                    float sum = bf2f((uint16_t)(w00[k] >> sh));
-                    if (right) sum += bf2f((uint16_t)(w01[k] >> sh));
-                    if (down) {
-                        sum += bf2f((uint16_t)(w10[k] >> sh));
-                        if (right) sum += bf2f((uint16_t)(w11[k] >> sh));
-                    }
-                    packed |= (uint32_t)f2bf(sum / cnt) << sh;
+            for (int n = 0; n < kNBW; n++) acc[rr][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < kKB; kb++) {
+            __syncthreads();  // the previous step's MFMAs are done with the halo
+#pragma unroll
+            for (int j = 0; j < kChunksPerThread; j++) {
+                const int i = tid + 256 * j;
+                if (i < kHaloChunks) *reinterpret_cast<uint4*>(halo + (i >> 2) * kPix + (i & 3) * 16) = pre[j];
+            }
+            // the next step's loads go out before this step's MFMAs
+            if (kb + 1 < kKB) fetch(t, kb + 1);
+            else if (t + (int)gridDim.x < tiles_total) fetch(t + gridDim.x, 0);
+            __syncthreads();
+#pragma unroll
+            for (int tap = 0; tap < 9; tap++) {
+                const int dy = tap / 3, dx = tap % 3;
+#pragma unroll
+                for (int rr = 0; rr < kRows; rr++) {
+                    const int hy = wm * kRows + rr + dy, hx = col + dx;
+                    const bf16x8 av = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(
+                                                                     halo + (hy * kHaloW + hx) * kPix + kg * 16));
+#pragma unroll
+                    for (int n = 0; n < kNBW; n++)
+                        acc[rr][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b[kb][tap][n], acc[rr][n], 0, 0, 0);
                 }
-                o[k] = packed;
+                __builtin_amdgcn_sched_barrier(0);
             }
-            *reinterpret_cast<uint4*>(a.pool + ((size_t)(y / 2) * ow + x / 2) * a.cout + co0 + q * 8) =
-                make_uint4(o[0], o[1], o[2], o[3]);
         }
+        conv_epilogue<kCout, kMW>(a, acc, out_tile, y0, x0, 0);
+        __syncthreads();  // the output tile is read before the next tile's epilogue writes it
     }
 }
 
@@ -271,6 +380,23 @@ constexpr int kShape[kLayers][2] = {
     {32, 16},                             // output: log radiance in channels 0-2
 };
 
+// A resident launch: as many workgroups as the device holds at once (the
+// occupancy the runtime reports for this kernel, asked once), since each
+// walks a fixed share of the tiles and a workgroup that had to wait for a
+// slot would start its share late.
+template <int kCout, int kMW, int kKB>
+inline int launch_resident(const ConvArgs& a, int tiles_per_image, int total, uint32_t cu_count, hipStream_t s) {
+    static const int per_cu = [] {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_conv3x3_resident<kCout, kMW, kKB>, 256, 0) != hipSuccess)
+            n = 1;
+        return n > 0 ? n : 1;
+    }();
+    const dim3 grid((uint32_t)std::min<int>(total, (int)cu_count * per_cu));
+    hipLaunchKernelGGL((k_conv3x3_resident<kCout, kMW, kKB>), grid, dim3(256), 0, s, a, tiles_per_image, total);
+    return VH_OK;
+}
+
 // Layers whose tiles give fewer than kSplitBlocks blocks per CU split their
 // output channels into 32-channel slices (more blocks, each staging the same
 // halo).
@@ -282,6 +408,22 @@ constexpr int kShape[kLayers][2] = {
 inline int launch_conv(const ConvArgs& a, uint32_t cu_count, hipStream_t s, bool force_split = false) {
     const uint32_t tiles = nblocks(a.w, kTileW) * nblocks(a.h, kTileH);
     const bool split = force_split || (a.cout > 32 && tiles < VH_DENOISE_SPLIT_BLOCKS * cu_count);
+    const int cin = a.c1 + a.c2, images = a.images > 1 ? a.images : 1;
+    if (!split && cin <= 4 * kCB && (a.cout == 16 || a.cout == 32 || a.cout == 64)) {
+        // resident workgroups: weights loaded once, the next halo prefetched
+        const int total = (int)tiles * images;
+        const int kb = cin / kCB;
+        if (a.cout == 16 && kb == 1) return launch_resident<16, 4, 1>(a, (int)tiles, total, cu_count, s);
+        if (a.cout == 16) return launch_resident<16, 4, 2>(a, (int)tiles, total, cu_count, s);
+        if (a.cout == 32 && kb == 1) return launch_resident<32, 2, 1>(a, (int)tiles, total, cu_count, s);
+        if (a.cout == 32 && kb == 2) return launch_resident<32, 2, 2>(a, (int)tiles, total, cu_count, s);
+        if (a.cout == 32 && kb == 3) return launch_resident<32, 2, 3>(a, (int)tiles, total, cu_count, s);
+        if (a.cout == 32) return launch_resident<32, 2, 4>(a, (int)tiles, total, cu_count, s);
+        if (kb == 1) return launch_resident<64, 1, 1>(a, (int)tiles, total, cu_count, s);
+        if (kb == 2) return launch_resident<64, 1, 2>(a, (int)tiles, total, cu_count, s);
+        if (kb == 3) return launch_resident<64, 1, 3>(a, (int)tiles, total, cu_count, s);
+        return launch_resident<64, 1, 4>(a, (int)tiles, total, cu_count, s);
+    }
     const dim3 grid(tiles, split ? a.cout / 32 : 1, a.images > 1 ? a.images : 1);
     switch (split ? 32 : a.cout) {
         // wave splits and tap groups as measured fastest at 1920x1080: the
