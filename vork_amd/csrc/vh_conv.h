@@ -255,7 +255,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kKB == 1 ? 
     const int kg = lane >> 4, col = lane & 15;
     const int cin = args.c1 + args.c2;
     const int tiles_x = (args.w + kTileW - 1) / kTileW;
-    // weights of every K block and tap (the layer's whole slice: co0 = 0)
+    const int co0 = blockIdx.y * kCout;  // this workgroup's output-channel slice
+    // weights of every K block and tap of the slice
     bf16x8 b[kKB][9][kNBW];
 #pragma unroll
     for (int kb = 0; kb < kKB; kb++)
@@ -263,7 +264,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kKB == 1 ? 
         for (int tap = 0; tap < 9; tap++)
 #pragma unroll
             for (int n = 0; n < kNBW; n++) {
-                const int co = (wn * kNBW + n) * 16 + col;
+                const int co = co0 + (wn * kNBW + n) * 16 + col;
                 b[kb][tap][n] = __builtin_bit_cast(
                     bf16x8, *reinterpret_cast<const uint4*>(args.wt + ((size_t)tap * args.cout + co) * cin + kb * kCB + kg * 8));
             }
@@ -321,7 +322,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kKB == 1 ? 
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
-        conv_epilogue<kCout, kMW>(a, acc, out_tile, y0, x0, 0);
+        conv_epilogue<kCout, kMW>(a, acc, out_tile, y0, x0, co0);
         __syncthreads();  // the output tile is read before the next tile's epilogue writes it
     }
 }
@@ -385,14 +386,15 @@ constexpr int kShape[kLayers][2] = {
 // walks a fixed share of the tiles and a workgroup that had to wait for a
 // slot would start its share late.
 template <int kCout, int kMW, int kKB>
-inline int launch_resident(const ConvArgs& a, int tiles_per_image, int total, uint32_t cu_count, hipStream_t s) {
+inline int launch_resident(const ConvArgs& a, int tiles_per_image, int total, uint32_t cu_count, hipStream_t s,
+                           int slices = 1) {
     static const int per_cu = [] {
         int n = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_conv3x3_resident<kCout, kMW, kKB>, 256, 0) != hipSuccess)
             n = 1;
         return n > 0 ? n : 1;
     }();
-    const dim3 grid((uint32_t)std::min<int>(total, (int)cu_count * per_cu));
+    const dim3 grid((uint32_t)std::max(1, std::min<int>(total, (int)cu_count * per_cu / slices)), (uint32_t)slices);
     hipLaunchKernelGGL((k_conv3x3_resident<kCout, kMW, kKB>), grid, dim3(256), 0, s, a, tiles_per_image, total);
     return VH_OK;
 }
@@ -409,6 +411,14 @@ inline int launch_conv(const ConvArgs& a, uint32_t cu_count, hipStream_t s, bool
     const uint32_t tiles = nblocks(a.w, kTileW) * nblocks(a.h, kTileH);
     const bool split = force_split || (a.cout > 32 && tiles < VH_DENOISE_SPLIT_BLOCKS * cu_count);
     const int cin = a.c1 + a.c2, images = a.images > 1 ? a.images : 1;
+    if (split && cin <= 4 * kCB && a.cout % 32 == 0) {
+        // 32-channel slices over the second grid dimension, resident likewise
+        const int total = (int)tiles * images, kb = cin / kCB, slices = a.cout / 32;
+        if (kb == 1) return launch_resident<32, 2, 1>(a, (int)tiles, total, cu_count, s, slices);
+        if (kb == 2) return launch_resident<32, 2, 2>(a, (int)tiles, total, cu_count, s, slices);
+        if (kb == 3) return launch_resident<32, 2, 3>(a, (int)tiles, total, cu_count, s, slices);
+        return launch_resident<32, 2, 4>(a, (int)tiles, total, cu_count, s, slices);
+    }
     if (!split && cin <= 4 * kCB && (a.cout == 16 || a.cout == 32 || a.cout == 64)) {
         // resident workgroups: weights loaded once, the next halo prefetched
         const int total = (int)tiles * images;
