@@ -645,6 +645,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VH_SHADE
     const uint32_t n = sorted ? c1 + counters[kClassCount + 2] : counters[0];
     uint32_t dropped = 0;
     uint32_t i = gid;
+    // the queue entry of the lane's next index, loaded one entry ahead (its
+    // load is in flight while the current entry is shaded)
+    auto entry = [&](uint32_t k) {
+        uint32_t e = k;
+        if (sorted) {
+            const uint32_t cls = k < c0 ? 0u : k < c1 ? 1u : 2u;
+            e = sorted[(size_t)cls * A->cap + (k - (cls == 0 ? 0u : cls == 1 ? c0 : c1))];
+        }
+        return rays[2 * e + 1];
+    };
+    float4 ahead = i < n ? entry(i) : make_float4(0.f, 0.f, 0.f, 0.f);
     while (true) {
         // A miss only adds the sky.  Each lane walks its entries (i, i + threads,
         // ...) past misses until it holds a hit, so the wave's shading steps run
@@ -655,13 +666,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VH_SHADE
         uint32_t p = 0;
         bool held = false;
         while (i < n) {
-            uint32_t e = i;
-            if (sorted) {
-                const uint32_t k = i < c0 ? 0u : i < c1 ? 1u : 2u;
-                e = sorted[(size_t)k * A->cap + (i - (k == 0 ? 0u : k == 1 ? c0 : c1))];
-            }
-            rd4 = rays[2 * e + 1];
+            rd4 = ahead;
             i += threads;
+            if (i < n) ahead = entry(i);
             p = f2u(rd4.w) & kPathMask;
             info = P.info[p];
             t4 = P.thr[p];
