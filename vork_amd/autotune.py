@@ -49,6 +49,7 @@ KNOBS = {
     "VH_TRACE_WAVES_PER_EU": [3, 4, 5],
     "VH_ANY_WAVES_PER_EU": [4, 5, 6, 7, 8],
     "VH_PACKET_WAVES_PER_EU": [4, 5, 6, 7],
+    "VH_PACKET_MIN_GROUP": [4, 16, 32],
     "VH_SHADE_WAVES_PER_EU": [3, 4, 5],
     "VH_TRACE_REFILL_MIN": [32, 40, 48, 56],
     "VH_WAVE_SAMPLES": [16, 32, 64],

@@ -447,6 +447,12 @@ VH_TRACE_KERNEL void k_extend(VH_ARGS A, uint32_t bounce) {
 #ifndef VH_PACKET_CAMERA
 #define VH_PACKET_CAMERA 1
 #endif
+// Packets when a pixel's samples fill at least this many consecutive slots
+// (a wave then holds the samples of at most 64 / VH_PACKET_MIN_GROUP nearby
+// pixels).
+#ifndef VH_PACKET_MIN_GROUP
+#define VH_PACKET_MIN_GROUP 16
+#endif
 // Waves per SIMD the packet kernel is compiled for: its node and triangle
 // data sit in SGPRs, so it fits 80 VGPRs and 6 waves.
 #ifndef VH_PACKET_WAVES_PER_EU
@@ -1520,7 +1526,7 @@ int render_scene(vh_scene* sc, const vh_render_params* rp, float* film, float* g
         shape.pixels = pixels;
         shape.mode = !adaptive ? 0u : c.decide ? 2u : 1u;
         shape.count = count;
-        shape.packet = VH_PACKET_CAMERA && !count && c.group >= 16;
+        shape.packet = VH_PACKET_CAMERA && !count && c.group >= VH_PACKET_MIN_GROUP;
         shape.sort = sort;
         if (profile) {
             // launched one by one, with events around the stage launches
