@@ -160,6 +160,34 @@ __device__ __forceinline__ void conv_epilogue(const ConvArgs& a, f32x4 (&acc)[kT
 
 }
 
+// The 9 taps of one K block for a wave's kRows output rows, walked by halo
+// row: each A fragment (halo row h, column shift dx) is read from LDS once and
+// feeds the up to three output rows h - dy it reaches, so a wave reads
+// 3 (kRows + 2) fragments instead of 9 kRows.  Every accumulator still takes
+// its taps in the order dy-major, dx-minor, so the sums are those of a
+// tap-by-tap loop, bit for bit.
+template <int kRows, int kNBW>
+__device__ __forceinline__ void halo_rows_mfma(const uint8_t* halo, int row0, int col, int kg,
+                                               const bf16x8 (&b)[9][kNBW], f32x4 (&acc)[kRows][kNBW]) {
+#pragma unroll
+    for (int h = 0; h < kRows + 2; h++) {
+#pragma unroll
+        for (int dx = 0; dx < 3; dx++) {
+            const bf16x8 av = __builtin_bit_cast(
+                bf16x8, *reinterpret_cast<const uint4*>(halo + ((row0 + h) * kHaloW + col + dx) * kPix + kg * 16));
+#pragma unroll
+            for (int dy = 0; dy < 3; dy++) {
+                const int rr = h - dy;
+                if (rr < 0 || rr >= kRows) continue;
+#pragma unroll
+                for (int n = 0; n < kNBW; n++)
+                    acc[rr][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b[dy * 3 + dx][n], acc[rr][n], 0, 0, 0);
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
 // kCout output channels from blockIdx.y * kCout; waves split as kMW (pixel
 // rows) x kNW (channel blocks); a wave holds the weight fragments of kGroup
 // taps at once (9: one L2 round trip per K block, issued before the halo
@@ -307,20 +335,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kKB == 1 ? 
             if (kb + 1 < kKB) fetch(t, kb + 1);
             else if (t + (int)gridDim.x < tiles_total) fetch(t + gridDim.x, 0);
             __syncthreads();
-#pragma unroll
-            for (int tap = 0; tap < 9; tap++) {
-                const int dy = tap / 3, dx = tap % 3;
-#pragma unroll
-                for (int rr = 0; rr < kRows; rr++) {
-                    const int hy = wm * kRows + rr + dy, hx = col + dx;
-                    const bf16x8 av = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(
-                                                                     halo + (hy * kHaloW + hx) * kPix + kg * 16));
-#pragma unroll
-                    for (int n = 0; n < kNBW; n++)
-                        acc[rr][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b[kb][tap][n], acc[rr][n], 0, 0, 0);
-                }
-                __builtin_amdgcn_sched_barrier(0);
-            }
+            halo_rows_mfma<kRows, kNBW>(halo, wm * kRows, col, kg, b[kb], acc);
         }
         conv_epilogue<kCout, kMW>(a, acc, out_tile, y0, x0, co0);
         __syncthreads();  // the output tile is read before the next tile's epilogue writes it
