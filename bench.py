@@ -425,7 +425,7 @@ def denoise_bench(args):
                      "denoise_ms_p50": round(dn, 3),
                      "speed_of_light_ms": round(D.speed_of_light_ms(W, H), 4),
                      "frac_of_speed_of_light": round(D.speed_of_light_ms(W, H) / dn, 4),
-                     "speed_of_light": "sum over the 21 launches of max(flops / 2.5 PFLOP/s, least HBM bytes / "
+                     "speed_of_light": "sum over the 19 launches of max(flops / 2.5 PFLOP/s, least HBM bytes / "
                                        "8 TB/s) (vork_amd.denoise.layer_costs)"},
         "training": {"step": f"vh_trainer_step over {B} crops of {Cr}x{Cr} (forward, loss, backward, Adam)",
                      "ms_per_step": round(train_ms, 3), "flops_per_step": train_flops,

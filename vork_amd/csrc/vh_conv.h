@@ -52,6 +52,15 @@ struct ConvArgs {
     // a batch of `images` (blockIdx.z; 0 means 1): each tensor holds its
     // images one after the other, derived strides
     int images;
+    // the output layer of inference: when set, the epilogue writes the clean
+    // radiance (expm1 of channels 0-2, alpha 1) here instead of y
+    float4* radiance = nullptr;
+    // the first layer of inference: when set, the halo is computed from the
+    // film and guide sums of `feat_samples` samples (the features of
+    // k_features) instead of read from x1
+    const float4* film = nullptr;
+    const float4* guides = nullptr;
+    float feat_samples = 1.f;
 };
 
 // The tensors of image `img` of a batch: every pointer advanced by its
@@ -65,6 +74,11 @@ __device__ __forceinline__ ConvArgs conv_image(ConvArgs a, int img) {
     if (a.x2) a.x2 += img * px * a.c2;
     a.y += img * px * a.cout;
     if (a.pool) a.pool += img * pxp * a.cout;
+    if (a.radiance) a.radiance += img * px;
+    if (a.film) {
+        a.film += img * px;
+        a.guides += img * px * 2;
+    }
     return a;
 }
 
@@ -110,6 +124,16 @@ __device__ __forceinline__ void conv_epilogue(const ConvArgs& a, f32x4 (&acc)[kT
         }
     }
     __syncthreads();
+    if (a.radiance) {
+        for (int px = tid; px < kTileH * kTileW; px += 256) {
+            const int y = y0 + px / kTileW, x = x0 + px % kTileW;
+            if (y >= a.h || x >= a.w) continue;
+            const uint16_t* v = tile + px * kOutPix;
+            a.radiance[(size_t)y * a.w + x] = make_float4(expm1f(fmaxf(bf2f(v[0]), 0.f)), expm1f(fmaxf(bf2f(v[1]), 0.f)),
+                                                          expm1f(fmaxf(bf2f(v[2]), 0.f)), 1.f);
+        }
+        return;
+    }
     constexpr int kChunks = kCout / 8;  // 16-byte chunks per pixel
     for (int i = tid; i < kTileH * kTileW * kChunks; i += 256) {
         const int px = i / kChunks, q = i % kChunks;
@@ -267,10 +291,32 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs args) {
 // step are in flight while the MFMAs of this one run; the weights are not
 // fetched again per tile.  The halo and the output tile have LDS regions of
 // their own, so the epilogue's global stores overlap the next step's loads.
+// The 32 input channels of one pixel from its film sum and guide sums (the
+// log1p of the mean radiance, mean albedo, mean normal, mean depth, then
+// zeros), as k_features computes them, in four 16-byte chunks.
+__device__ __forceinline__ void feature_chunks(float4 f, float4 ga, float4 gn, float samples, uint4 (&c)[4]) {
+    const float inv = 1.f / samples;
+    float v[16];
+    v[0] = log1pf(fmaxf(f.x * inv, 0.f));
+    v[1] = log1pf(fmaxf(f.y * inv, 0.f));
+    v[2] = log1pf(fmaxf(f.z * inv, 0.f));
+    v[3] = ga.x * inv; v[4] = ga.y * inv; v[5] = ga.z * inv;
+    v[6] = gn.x * inv; v[7] = gn.y * inv; v[8] = gn.z * inv;
+    v[9] = gn.w * inv;
+#pragma unroll
+    for (int k = 10; k < 16; k++) v[k] = 0.f;
+    uint32_t w[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) w[k] = (uint32_t)f2bf(v[2 * k]) | ((uint32_t)f2bf(v[2 * k + 1]) << 16);
+    c[0] = make_uint4(w[0], w[1], w[2], w[3]);
+    c[1] = make_uint4(w[4], w[5], w[6], w[7]);
+    c[2] = c[3] = make_uint4(0, 0, 0, 0);
+}
+
 constexpr int kHaloChunks = kHaloH * kHaloW * 4;              // 16-byte chunks of one K block's halo
 constexpr int kChunksPerThread = (kHaloChunks + 255) / 256;
 
-template <int kCout, int kMW, int kKB>
+template <int kCout, int kMW, int kKB, bool kFeat = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kKB == 1 ? 4 : kKB == 2 ? 3 : 2))) void k_conv3x3_resident(
     ConvArgs args, int tiles_per_image, int tiles_total) {
     constexpr int kNB = kCout / 16, kNW = 4 / kMW, kNBW = kNB / kNW, kRows = kTileH / kMW;
@@ -302,6 +348,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kKB == 1 ? 
         const int img = t / tiles_per_image, r = t - img * tiles_per_image;
         const ConvArgs a = conv_image(args, img);
         const int y0 = (r / tiles_x) * kTileH, x0 = (r % tiles_x) * kTileW;
+        if constexpr (kFeat) {
+            // thread tid < 180 holds halo pixel tid's film sum and guide sums
+            static_assert(kKB == 1 && kChunksPerThread == 3, "features fill one K block, three float4 per pixel");
+#pragma unroll
+            for (int j = 0; j < 3; j++) pre[j] = make_uint4(0, 0, 0, 0);
+            const int y = y0 + tid / kHaloW - 1, x = x0 + tid % kHaloW - 1;
+            if (tid < kHaloH * kHaloW && y >= 0 && x >= 0 && y < a.h && x < a.w) {
+                const size_t p = (size_t)y * a.w + x;
+                pre[0] = __builtin_bit_cast(uint4, a.film[p]);
+                pre[1] = __builtin_bit_cast(uint4, a.guides[2 * p]);
+                pre[2] = __builtin_bit_cast(uint4, a.guides[2 * p + 1]);
+            }
+            return;
+        }
 #pragma unroll
         for (int j = 0; j < kChunksPerThread; j++) {
             const int i = tid + 256 * j;
@@ -326,10 +386,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kKB == 1 ? 
 #pragma unroll
         for (int kb = 0; kb < kKB; kb++) {
             __syncthreads();  // the previous step's MFMAs are done with the halo
+            if constexpr (kFeat) {
+                // out-of-image pixels hold zero sums, whose features are zero
+                if (tid < kHaloH * kHaloW) {
+                    uint4 c[4];
+                    feature_chunks(__builtin_bit_cast(float4, pre[0]), __builtin_bit_cast(float4, pre[1]),
+                                   __builtin_bit_cast(float4, pre[2]), a.feat_samples, c);
 #pragma unroll
-            for (int j = 0; j < kChunksPerThread; j++) {
-                const int i = tid + 256 * j;
-                if (i < kHaloChunks) *reinterpret_cast<uint4*>(halo + (i >> 2) * kPix + (i & 3) * 16) = pre[j];
+                    for (int q = 0; q < 4; q++) *reinterpret_cast<uint4*>(halo + tid * kPix + q * 16) = c[q];
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < kChunksPerThread; j++) {
+                    const int i = tid + 256 * j;
+                    if (i < kHaloChunks) *reinterpret_cast<uint4*>(halo + (i >> 2) * kPix + (i & 3) * 16) = pre[j];
+                }
             }
             // the next step's loads go out before this step's MFMAs
             if (kb + 1 < kKB) fetch(t, kb + 1);
@@ -369,15 +440,6 @@ __global__ void k_features(const float4* film, const float4* guides, float sampl
     }
 }
 
-// Output: expm1 of the predicted log radiance (channels 0-2 of 16), alpha 1.
-__global__ void k_output(const uint16_t* y, int pixels, float4* out) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= pixels) return;
-    const uint16_t* v = y + (size_t)p * 16;
-    out[p] = make_float4(expm1f(fmaxf(bf2f(v[0]), 0.f)), expm1f(fmaxf(bf2f(v[1]), 0.f)),
-                         expm1f(fmaxf(bf2f(v[2]), 0.f)), 1.f);
-}
-
 inline uint32_t nblocks(size_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
 
 // The network: (input channels, output channels) of its 19 convolutions in
@@ -400,17 +462,18 @@ constexpr int kShape[kLayers][2] = {
 // occupancy the runtime reports for this kernel, asked once), since each
 // walks a fixed share of the tiles and a workgroup that had to wait for a
 // slot would start its share late.
-template <int kCout, int kMW, int kKB>
+template <int kCout, int kMW, int kKB, bool kFeat = false>
 inline int launch_resident(const ConvArgs& a, int tiles_per_image, int total, uint32_t cu_count, hipStream_t s,
                            int slices = 1) {
     static const int per_cu = [] {
         int n = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_conv3x3_resident<kCout, kMW, kKB>, 256, 0) != hipSuccess)
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_conv3x3_resident<kCout, kMW, kKB, kFeat>, 256, 0) !=
+            hipSuccess)
             n = 1;
         return n > 0 ? n : 1;
     }();
     const dim3 grid((uint32_t)std::max(1, std::min<int>(total, (int)cu_count * per_cu / slices)), (uint32_t)slices);
-    hipLaunchKernelGGL((k_conv3x3_resident<kCout, kMW, kKB>), grid, dim3(256), 0, s, a, tiles_per_image, total);
+    hipLaunchKernelGGL((k_conv3x3_resident<kCout, kMW, kKB, kFeat>), grid, dim3(256), 0, s, a, tiles_per_image, total);
     return VH_OK;
 }
 
@@ -426,6 +489,11 @@ inline int launch_conv(const ConvArgs& a, uint32_t cu_count, hipStream_t s, bool
     const uint32_t tiles = nblocks(a.w, kTileW) * nblocks(a.h, kTileH);
     const bool split = force_split || (a.cout > 32 && tiles < VH_DENOISE_SPLIT_BLOCKS * cu_count);
     const int cin = a.c1 + a.c2, images = a.images > 1 ? a.images : 1;
+    if (a.film) {
+        if (cin != kCB || a.cout != 32 || a.up1 || a.c2)
+            return fail(VH_ERR_USAGE, "denoiser: features feed a 32-to-32-channel layer only");
+        return launch_resident<32, 2, 1, true>(a, (int)tiles, (int)tiles * images, cu_count, s);
+    }
     if (split && cin <= 4 * kCB && a.cout % 32 == 0) {
         // 32-channel slices over the second grid dimension, resident likewise
         const int total = (int)tiles * images, kb = cin / kCB, slices = a.cout / 32;

@@ -26,16 +26,15 @@ int denoise_run(vh_denoiser* dn, int W, int H, float samples, const float* film,
     h[0] = H; w[0] = W;
     for (int l = 1; l < 5; l++) { h[l] = (h[l - 1] + 1) / 2; w[l] = (w[l - 1] + 1) / 2; }
     const int C[5] = {32, 64, 64, 96, 128};
-    // scratch: features, a temporary and a pooled input at the largest level
-    // used, every encoder output, every decoder output, and the output layer
+    // scratch: a temporary and a pooled input at the largest level used,
+    // every encoder output and every decoder output
     auto px = [&](int l) { return (size_t)h[l] * w[l]; };
     size_t off = 0;
     auto take = [&](size_t elems) { size_t o = off; off += (elems * 2 + 255) & ~(size_t)255; return o; };
-    const size_t o_feat = take(px(0) * 32), o_tmp = take(px(0) * 128), o_pool = take(px(1) * 96);
+    const size_t o_tmp = take(px(0) * 128), o_pool = take(px(1) * 96);
     size_t o_enc[5], o_dec[4];
     for (int l = 0; l < 5; l++) o_enc[l] = take(px(l) * C[l]);
     for (int l = 0; l < 4; l++) o_dec[l] = take(px(l) * C[l]);
-    const size_t o_out = take(px(0) * 16);
     if (dn->scratch_bytes < off) {
         VH_HIP(dn->scratch.allocate(off));
         dn->scratch_bytes = off;
@@ -46,8 +45,9 @@ int denoise_run(vh_denoiser* dn, int W, int H, float samples, const float* film,
     const float* bs = dn->biases.as<float>();
     int layer = 0, rc;
     auto conv = [&](int l, const uint16_t* x1, int c1, int up1, const uint16_t* x2, int c2, uint16_t* y, bool relu,
-                    uint16_t* pool = nullptr) {
+                    uint16_t* pool = nullptr, float4* radiance = nullptr) {
         ConvArgs a;
+        a.radiance = radiance;
         a.x1 = x1; a.x2 = x2; a.c1 = c1; a.c2 = c2; a.up1 = up1; a.pool = pool;
         a.wt = wts + dn->w_off[layer]; a.bias = bs + dn->b_off[layer];
         a.y = y; a.h = h[l]; a.w = w[l]; a.relu = relu ? 1 : 0;
@@ -56,13 +56,26 @@ int denoise_run(vh_denoiser* dn, int W, int H, float samples, const float* film,
         layer++;
         return launch_conv(a, (uint32_t)dn->device->cu_count, s);
     };
-    hipLaunchKernelGGL(k_features, dim3(nblocks(px(0), 256)), dim3(256), 0, s, reinterpret_cast<const float4*>(film),
-                       reinterpret_cast<const float4*>(guides), samples, (int)px(0), at(o_feat));
-    // encoder
-    const uint16_t* in = at(o_feat);
+    // encoder; the first layer computes its input features from the film and
+    // guides while it stages them
+    const uint16_t* in = nullptr;
     int cin = 32;
     for (int l = 0; l < 5; l++) {
-        if ((rc = conv(l, in, cin, 0, nullptr, 0, at(o_tmp), true))) return rc;
+        if (l == 0) {
+            ConvArgs a;
+            a.x1 = nullptr; a.x2 = nullptr; a.c1 = 32; a.c2 = 0; a.up1 = 0; a.pool = nullptr;
+            a.wt = wts + dn->w_off[layer]; a.bias = bs + dn->b_off[layer];
+            a.y = at(o_tmp); a.h = h[0]; a.w = w[0]; a.relu = 1;
+            a.cout = kShape[layer][1];
+            a.images = 1;
+            a.film = reinterpret_cast<const float4*>(film);
+            a.guides = reinterpret_cast<const float4*>(guides);
+            a.feat_samples = samples;
+            layer++;
+            if ((rc = launch_conv(a, (uint32_t)dn->device->cu_count, s))) return rc;
+        } else if ((rc = conv(l, in, cin, 0, nullptr, 0, at(o_tmp), true))) {
+            return rc;
+        }
         // levels 0-3 also write the next level's input, the 2x2 pool of their output
         if ((rc = conv(l, at(o_tmp), C[l], 0, nullptr, 0, at(o_enc[l]), true, l < 4 ? at(o_pool) : nullptr))) return rc;
         in = at(o_pool);
@@ -77,9 +90,9 @@ int denoise_run(vh_denoiser* dn, int W, int H, float samples, const float* film,
         coarse = at(o_dec[l]);
         cc = C[l];
     }
-    if ((rc = conv(0, at(o_dec[0]), 32, 0, nullptr, 0, at(o_out), false))) return rc;
-    hipLaunchKernelGGL(k_output, dim3(nblocks(px(0), 256)), dim3(256), 0, s, at(o_out), (int)px(0),
-                       reinterpret_cast<float4*>(out));
+    // the output layer writes the clean radiance from its epilogue
+    if ((rc = conv(0, at(o_dec[0]), 32, 0, nullptr, 0, nullptr, false, nullptr, reinterpret_cast<float4*>(out))))
+        return rc;
     VH_HIP(hipGetLastError());
     return VH_OK;
 }

@@ -292,20 +292,23 @@ def layer_costs(width, height):
     the multiply-adds x 2 each convolution runs (padded channel counts) and
     the least HBM bytes it must move: its inputs read once (the decoder's
     coarse input at its own resolution), its bf16 output written once (and
-    the encoder's 2x2 pool), its weights once.  The feature and output passes
-    move film, guides and radiance."""
+    the encoder's 2x2 pool), its weights once.  The first layer reads film and
+    guides; the output layer writes the fp32 radiance from its epilogue."""
     h, w = [height], [width]
     for _ in range(4):
         h.append((h[-1] + 1) // 2)
         w.append((w[-1] + 1) // 2)
     px = [h[l] * w[l] for l in range(5)]
     c = LEVELS
-    out = [("features", 0, px[0] * (16 + 32 + 64))]
+    out = []
 
     def conv(name, l, c1, c2, cout, up=False, pool=False):
         cin = c1 + c2
         fl = 2 * px[l] * 9 * cin * cout
-        by = (px[l + 1] if up else px[l]) * c1 * 2 + px[l] * c2 * 2 + 9 * cin * cout * 2 + px[l] * cout * 2
+        # the first layer reads the film and guide sums (48 B per pixel) and
+        # computes its input features while staging them
+        inp = px[0] * 48 if name == "enc0a" else (px[l + 1] if up else px[l]) * c1 * 2 + px[l] * c2 * 2
+        by = inp + 9 * cin * cout * 2 + px[l] * cout * 2
         if pool:
             by += px[l + 1] * cout * 2
         out.append((name, fl, by))
@@ -318,8 +321,9 @@ def layer_costs(width, height):
     for l in (3, 2, 1, 0):
         conv(f"dec{l}a", l, c[l + 1], c[l], c[l], up=True)
         conv(f"dec{l}b", l, c[l], 0, c[l])
-    conv("out", 0, c[0], 0, 16)
-    out.append(("output", 0, px[0] * (32 + 16)))
+    # the output layer writes the fp32 radiance (16 B per pixel) from its epilogue
+    cin = c[0]
+    out.append(("out", 2 * px[0] * 9 * cin * 16, px[0] * cin * 2 + 9 * cin * 16 * 2 + px[0] * 16))
     return out
 
 
