@@ -39,6 +39,13 @@ __device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
     return (uint16_t)(u >> 16);
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+// Two values rounded to bf16 (nearest even) by one v_cvt_pk_bf16_f32, a in the
+// low half: the same bits as f2bf for every non-NaN input.
+__device__ __forceinline__ uint32_t f2bf2(float a, float b) {
+    typedef __attribute__((ext_vector_type(2))) float f32x2;
+    typedef __attribute__((ext_vector_type(2))) __bf16 bf16x2;
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2{a, b}), bf16x2));
+}
 
 struct ConvArgs {
     const uint16_t* x1;  // channels [0, c1): at half resolution when up1
@@ -94,15 +101,15 @@ __device__ __forceinline__ uint4 fetch_chunk(const ConvArgs& a, int y, int x, in
     return *reinterpret_cast<const uint4*>(a.x2 + ((size_t)y * a.w + x) * a.c2 + (ch - a.c1));
 }
 
-// The epilogue of a tile through LDS: bias, ReLU and bf16 rounding per D
-// fragment (a lane holds output channel (block + col) of pixels 4 kg + j of a
-// row) into an output tile of the block's kCout channels per pixel at `tile`,
-// then whole 16-byte chunks (8 channels) to memory, and the 2x2 pool from the
-// rounded tile.  The caller makes sure no wave still reads what `tile`
-// overlays; the function ends with the tile read (no closing barrier).
+// The epilogue of a tile through LDS, in two halves.  conv_tile_to_lds:
+// bias, ReLU and bf16 rounding per D fragment (a lane holds output channel
+// (block + col) of pixels 4 kg + j of a row) into an output tile of the
+// block's kCout channels per pixel at `tile`.  conv_tile_store, after a
+// barrier: whole 16-byte chunks (8 channels) to memory, and the 2x2 pool from
+// the rounded tile (or, for the output layer, the radiance).
 template <int kCout, int kMW>
-__device__ __forceinline__ void conv_epilogue(const ConvArgs& a, f32x4 (&acc)[kTileH / kMW][(kCout / 16) / (4 / kMW)],
-                                              uint16_t* tile, int y0, int x0, int co0) {
+__device__ __forceinline__ void conv_tile_to_lds(const ConvArgs& a, f32x4 (&acc)[kTileH / kMW][(kCout / 16) / (4 / kMW)],
+                                                 uint16_t* tile, int co0) {
     constexpr int kNB = kCout / 16, kNW = 4 / kMW, kNBW = kNB / kNW, kRows = kTileH / kMW;
     constexpr int kOutPix = kCout + 8;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -116,14 +123,24 @@ __device__ __forceinline__ void conv_epilogue(const ConvArgs& a, f32x4 (&acc)[kT
             const int c = (wn * kNBW + n) * 16 + col;
             const float bias = a.bias[co0 + c];
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                float v = acc[r][n][j] + bias;
-                if (a.relu) v = fmaxf(v, 0.f);
-                tile[(ty * kTileW + kg * 4 + j) * kOutPix + c] = f2bf(v);
+            for (int j = 0; j < 4; j += 2) {
+                float v0 = acc[r][n][j] + bias, v1 = acc[r][n][j + 1] + bias;
+                if (a.relu) {
+                    v0 = fmaxf(v0, 0.f);
+                    v1 = fmaxf(v1, 0.f);
+                }
+                const uint32_t pk = f2bf2(v0, v1);
+                tile[(ty * kTileW + kg * 4 + j) * kOutPix + c] = (uint16_t)pk;
+                tile[(ty * kTileW + kg * 4 + j + 1) * kOutPix + c] = (uint16_t)(pk >> 16);
             }
         }
     }
-    __syncthreads();
+}
+
+template <int kCout>
+__device__ __forceinline__ void conv_tile_store(const ConvArgs& a, const uint16_t* tile, int y0, int x0, int co0) {
+    constexpr int kOutPix = kCout + 8;
+    const int tid = threadIdx.x;
     if (a.radiance) {
         for (int px = tid; px < kTileH * kTileW; px += 256) {
             const int y = y0 + px / kTileW, x = x0 + px % kTileW;
@@ -155,13 +172,14 @@ __device__ __forceinline__ void conv_epilogue(const ConvArgs& a, f32x4 (&acc)[kT
             const uint4 c01 = *reinterpret_cast<const uint4*>(tile + (p00 + 1) * kOutPix + q * 8);
             const uint4 c10 = *reinterpret_cast<const uint4*>(tile + (p00 + kTileW) * kOutPix + q * 8);
             const uint4 c11 = *reinterpret_cast<const uint4*>(tile + (p00 + kTileW + 1) * kOutPix + q * 8);
-            const float cnt = (float)((1 + right) * (1 + down));
+            // 1, 1/2 or 1/4: the product is the quotient, exactly
+            const float inv_cnt = right && down ? 0.25f : right || down ? 0.5f : 1.f;
             uint32_t o[4];
             const uint32_t* w00 = &c00.x; const uint32_t* w01 = &c01.x;
             const uint32_t* w10 = &c10.x; const uint32_t* w11 = &c11.x;
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                uint32_t packed = 0;
+                float avg[2];
 #pragma unroll
                 for (int hlf = 0; hlf < 2; hlf++) {
                     const int sh = 16 * hlf;
@@ -173,9 +191,9 @@ __device__ __forceinline__ void conv_epilogue(const ConvArgs& a, f32x4 (&acc)[kT
                         sum += bf2f((uint16_t)(w10[k] >> sh));
                         if (right) sum += bf2f((uint16_t)(w11[k] >> sh));
                     }
-                    packed |= (uint32_t)f2bf(sum / cnt) << sh;
+                    avg[hlf] = sum * inv_cnt;
                 }
-                o[k] = packed;
+                o[k] = f2bf2(avg[0], avg[1]);
             }
             *reinterpret_cast<uint4*>(a.pool + ((size_t)(y / 2) * ow + x / 2) * a.cout + co0 + q * 8) =
                 make_uint4(o[0], o[1], o[2], o[3]);
@@ -184,29 +202,59 @@ __device__ __forceinline__ void conv_epilogue(const ConvArgs& a, f32x4 (&acc)[kT
 
 }
 
+// Both halves of the epilogue.  The caller makes sure no wave still reads what
+// `tile` overlays; the function ends with the tile read (no closing barrier).
+template <int kCout, int kMW>
+__device__ __forceinline__ void conv_epilogue(const ConvArgs& a, f32x4 (&acc)[kTileH / kMW][(kCout / 16) / (4 / kMW)],
+                                              uint16_t* tile, int y0, int x0, int co0) {
+    conv_tile_to_lds<kCout, kMW>(a, acc, tile, co0);
+    __syncthreads();
+    conv_tile_store<kCout>(a, tile, y0, x0, co0);
+}
+
 // The 9 taps of one K block for a wave's kRows output rows, walked by halo
 // row: each A fragment (halo row h, column shift dx) is read from LDS once and
 // feeds the up to three output rows h - dy it reaches, so a wave reads
 // 3 (kRows + 2) fragments instead of 9 kRows.  Every accumulator still takes
 // its taps in the order dy-major, dx-minor, so the sums are those of a
 // tap-by-tap loop, bit for bit.
-template <int kRows, int kNBW>
+//
+// With kPipe the next halo row's fragments are read before this row's MFMAs
+// issue (three more fragments in registers), so a wave does not wait on each
+// row's LDS latency: for the kernels that run few waves per SIMD.
+template <int kRows, int kNBW, bool kPipe = false>
 __device__ __forceinline__ void halo_rows_mfma(const uint8_t* halo, int row0, int col, int kg,
                                                const bf16x8 (&b)[9][kNBW], f32x4 (&acc)[kRows][kNBW]) {
+    const uint8_t* base = halo + (row0 * kHaloW + col) * kPix + kg * 16;
+    auto read_row = [&](int h, bf16x8 (&av)[3]) {
+#pragma unroll
+        for (int dx = 0; dx < 3; dx++)
+            av[dx] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(base + (h * kHaloW + dx) * kPix));
+    };
+    bf16x8 cur[3];
+    if (kPipe) read_row(0, cur);
 #pragma unroll
     for (int h = 0; h < kRows + 2; h++) {
+        bf16x8 nxt[3];
+        if (kPipe) {
+            if (h + 1 < kRows + 2) read_row(h + 1, nxt);
+        } else {
+            read_row(h, cur);
+        }
 #pragma unroll
         for (int dx = 0; dx < 3; dx++) {
-            const bf16x8 av = __builtin_bit_cast(
-                bf16x8, *reinterpret_cast<const uint4*>(halo + ((row0 + h) * kHaloW + col + dx) * kPix + kg * 16));
 #pragma unroll
             for (int dy = 0; dy < 3; dy++) {
                 const int rr = h - dy;
                 if (rr < 0 || rr >= kRows) continue;
 #pragma unroll
                 for (int n = 0; n < kNBW; n++)
-                    acc[rr][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b[dy * 3 + dx][n], acc[rr][n], 0, 0, 0);
+                    acc[rr][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur[dx], b[dy * 3 + dx][n], acc[rr][n], 0, 0, 0);
             }
+        }
+        if (kPipe && h + 1 < kRows + 2) {
+#pragma unroll
+            for (int dx = 0; dx < 3; dx++) cur[dx] = nxt[dx];
         }
         __builtin_amdgcn_sched_barrier(0);
     }
@@ -290,7 +338,8 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs args) {
 // step ahead (3 chunks of 16 bytes per thread), so the loads of the next
 // step are in flight while the MFMAs of this one run; the weights are not
 // fetched again per tile.  The halo and the output tile have LDS regions of
-// their own, so the epilogue's global stores overlap the next step's loads.
+// their own: a tile's output waits in LDS and goes to memory during the next
+// tile's first step, right after that step's loads.
 // The 32 input channels of one pixel from its film sum and guide sums (the
 // log1p of the mean radiance, mean albedo, mean normal, mean depth, then
 // zeros), as k_features computes them, in four 16-byte chunks.
@@ -307,12 +356,17 @@ __device__ __forceinline__ void feature_chunks(float4 f, float4 ga, float4 gn, f
     for (int k = 10; k < 16; k++) v[k] = 0.f;
     uint32_t w[8];
 #pragma unroll
-    for (int k = 0; k < 8; k++) w[k] = (uint32_t)f2bf(v[2 * k]) | ((uint32_t)f2bf(v[2 * k + 1]) << 16);
+    for (int k = 0; k < 8; k++) w[k] = f2bf2(v[2 * k], v[2 * k + 1]);
     c[0] = make_uint4(w[0], w[1], w[2], w[3]);
     c[1] = make_uint4(w[4], w[5], w[6], w[7]);
     c[2] = c[3] = make_uint4(0, 0, 0, 0);
 }
 
+// Resident layers of at least this many K blocks read each halo row's
+// fragments one row ahead of its MFMAs.
+#ifndef VH_CONV_PIPE_KB
+#define VH_CONV_PIPE_KB 1
+#endif
 constexpr int kHaloChunks = kHaloH * kHaloW * 4;              // 16-byte chunks of one K block's halo
 constexpr int kChunksPerThread = (kHaloChunks + 255) / 256;
 
@@ -362,22 +416,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kKB == 1 ? 
             }
             return;
         }
+        // the K block comes from one input for every lane (c1 is a multiple of
+        // 32): x1 (at half resolution when up1) or the skip x2
+        const bool first = kb * kCB < a.c1;
+        const uint16_t* src = first ? a.x1 : a.x2;
+        const int cs = first ? a.c1 : a.c2, cb = first ? kb * kCB : kb * kCB - a.c1;
+        const int half = first && a.up1;
+        const int ws = half ? (a.w + 1) / 2 : a.w;
 #pragma unroll
         for (int j = 0; j < kChunksPerThread; j++) {
             const int i = tid + 256 * j;
+            const int px = i >> 2, q = i & 3;
+            const int y = y0 + px / kHaloW - 1, x = x0 + px % kHaloW - 1;
             pre[j] = make_uint4(0, 0, 0, 0);
-            if (i < kHaloChunks) {
-                const int px = i >> 2, q = i & 3;
-                pre[j] = fetch_chunk(a, y0 + px / kHaloW - 1, x0 + px % kHaloW - 1, kb * kCB + q * 8);
+            if (i < kHaloChunks && (unsigned)y < (unsigned)a.h && (unsigned)x < (unsigned)a.w) {
+                const uint32_t p = (uint32_t)(y >> half) * (uint32_t)ws + (uint32_t)(x >> half);
+                pre[j] = *reinterpret_cast<const uint4*>(src + (size_t)p * cs + cb + q * 8);
             }
         }
     };
-    int t = blockIdx.x;
+    int t = blockIdx.x, prev = -1;
     if (t < tiles_total) fetch(t, 0);
+    // the global stores of tile `prev`, whose output waits in out_tile
+    auto store_prev = [&]() {
+        const int img = prev / tiles_per_image, r = prev - img * tiles_per_image;
+        conv_tile_store<kCout>(conv_image(args, img), out_tile, (r / tiles_x) * kTileH, (r % tiles_x) * kTileW, co0);
+    };
     for (; t < tiles_total; t += gridDim.x) {
-        const int img = t / tiles_per_image, r = t - img * tiles_per_image;
+        const int img = t / tiles_per_image;
         const ConvArgs a = conv_image(args, img);
-        const int y0 = (r / tiles_x) * kTileH, x0 = (r % tiles_x) * kTileW;
         f32x4 acc[kRows][kNBW];
 #pragma unroll
         for (int rr = 0; rr < kRows; rr++)
@@ -402,14 +469,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kKB == 1 ? 
                     if (i < kHaloChunks) *reinterpret_cast<uint4*>(halo + (i >> 2) * kPix + (i & 3) * 16) = pre[j];
                 }
             }
-            // the next step's loads go out before this step's MFMAs
+            // the next step's loads go out before this step's MFMAs, and the
+            // previous tile's stores right after them: the wait for these
+            // loads at the next step also waits for the stores, and both have
+            // the MFMAs' time to complete
             if (kb + 1 < kKB) fetch(t, kb + 1);
             else if (t + (int)gridDim.x < tiles_total) fetch(t + gridDim.x, 0);
+            if (kb == 0 && prev >= 0) store_prev();
             __syncthreads();
-            halo_rows_mfma<kRows, kNBW>(halo, wm * kRows, col, kg, b[kb], acc);
+            halo_rows_mfma<kRows, kNBW, (kKB >= VH_CONV_PIPE_KB)>(halo, wm * kRows, col, kg, b[kb], acc);
         }
-        conv_epilogue<kCout, kMW>(a, acc, out_tile, y0, x0, co0);
-        __syncthreads();  // the output tile is read before the next tile's epilogue writes it
+        conv_tile_to_lds<kCout, kMW>(a, acc, out_tile, co0);
+        prev = t;
+    }
+    if (prev >= 0) {
+        __syncthreads();
+        store_prev();
     }
 }
 
