@@ -419,7 +419,8 @@ def denoise_bench(args):
         "equal_relmse": equal,
         "latency_ms": {"denoised_4spp_p50": round(p50_4, 2), "plain_64spp_p50": round(p50_64, 2),
                        "speedup": round(p50_64 / p50_4, 2)},
-        "roofline": {"kernel": "U-Net (19 k_conv3x3 bf16 MFMA launches + pool/feature/output passes)", "bound": "mfma",
+        "roofline": {"kernel": "U-Net (19 bf16 MFMA convolution launches; the first computes its input features, "
+                                "the last writes the radiance, encoder convolutions write their 2x2 pools)", "bound": "mfma",
                      "achieved": round(achieved, 1), "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": round(achieved / 2500.0, 4), "traffic": None, "flops_per_launch": flops,
                      "denoise_ms_p50": round(dn, 3),
