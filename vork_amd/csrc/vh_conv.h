@@ -89,18 +89,6 @@ __device__ __forceinline__ ConvArgs conv_image(ConvArgs a, int img) {
     return a;
 }
 
-// One 16-byte chunk of channels [ch, ch + 8) of halo pixel (hy, hx) of the tile
-// at (y0, x0): zero outside the image.
-__device__ __forceinline__ uint4 fetch_chunk(const ConvArgs& a, int y, int x, int ch) {
-    if (y < 0 || x < 0 || y >= a.h || x >= a.w) return make_uint4(0, 0, 0, 0);
-    if (ch < a.c1) {
-        const int wh = a.up1 ? (a.w + 1) / 2 : a.w;
-        const int yy = a.up1 ? y >> 1 : y, xx = a.up1 ? x >> 1 : x;
-        return *reinterpret_cast<const uint4*>(a.x1 + ((size_t)yy * wh + xx) * a.c1 + ch);
-    }
-    return *reinterpret_cast<const uint4*>(a.x2 + ((size_t)y * a.w + x) * a.c2 + (ch - a.c1));
-}
-
 // The epilogue of a tile through LDS, in two halves.  conv_tile_to_lds:
 // bias, ReLU and bf16 rounding per D fragment (a lane holds output channel
 // (block + col) of pixels 4 kg + j of a row) into an output tile of the
@@ -260,11 +248,46 @@ __device__ __forceinline__ void halo_rows_mfma(const uint8_t* halo, int row0, in
     }
 }
 
-// kCout output channels from blockIdx.y * kCout; waves split as kMW (pixel
-// rows) x kNW (channel blocks); a wave holds the weight fragments of kGroup
-// taps at once (9: one L2 round trip per K block, issued before the halo
-// staging so the two latencies overlap; 3: fewer registers, more waves).
-template <int kCout, int kMW, int kGroup>
+constexpr int kHaloChunks = kHaloH * kHaloW * 4;              // 16-byte chunks of one K block's halo
+constexpr int kChunksPerThread = (kHaloChunks + 255) / 256;
+
+// Thread tid's chunks (i = tid + 256 j) of K block kb's halo for the tile at
+// (y0, x0): zero outside the image.  The block comes from one input for
+// every lane (c1 is a multiple of 32): x1, at half resolution when up1, or
+// the skip x2.
+__device__ __forceinline__ void halo_chunks(const ConvArgs& a, int y0, int x0, int kb, uint4 (&pre)[kChunksPerThread]) {
+    const int tid = threadIdx.x;
+    const bool first = kb * kCB < a.c1;
+    const uint16_t* src = first ? a.x1 : a.x2;
+    const int cs = first ? a.c1 : a.c2, cb = first ? kb * kCB : kb * kCB - a.c1;
+    const int half = first && a.up1;
+    const int ws = half ? (a.w + 1) / 2 : a.w;
+#pragma unroll
+    for (int j = 0; j < kChunksPerThread; j++) {
+        const int i = tid + 256 * j;
+        const int px = i >> 2, q = i & 3;
+        const int y = y0 + px / kHaloW - 1, x = x0 + px % kHaloW - 1;
+        pre[j] = make_uint4(0, 0, 0, 0);
+        if (i < kHaloChunks && (unsigned)y < (unsigned)a.h && (unsigned)x < (unsigned)a.w) {
+            const uint32_t p = (uint32_t)(y >> half) * (uint32_t)ws + (uint32_t)(x >> half);
+            pre[j] = *reinterpret_cast<const uint4*>(src + (size_t)p * cs + cb + q * 8);
+        }
+    }
+}
+
+__device__ __forceinline__ void halo_store(uint8_t* halo, const uint4 (&pre)[kChunksPerThread]) {
+#pragma unroll
+    for (int j = 0; j < kChunksPerThread; j++) {
+        const int i = threadIdx.x + 256 * j;
+        if (i < kHaloChunks) *reinterpret_cast<uint4*>(halo + (i >> 2) * kPix + (i & 3) * 16) = pre[j];
+    }
+}
+
+// kCout output channels from blockIdx.y * kCout, one 8x16 tile per workgroup
+// (blockIdx.x) of image blockIdx.z; waves split as kMW (pixel rows) x kNW
+// (channel blocks).  Per K block the halo and all 9 taps' weight fragments
+// are loaded at once.
+template <int kCout, int kMW>
 __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs args) {
     const ConvArgs a = conv_image(args, blockIdx.z);
     constexpr int kNB = kCout / 16, kNW = 4 / kMW, kNBW = kNB / kNW, kRows = kTileH / kMW;
@@ -285,47 +308,22 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs args) {
 #pragma unroll
         for (int n = 0; n < kNBW; n++) acc[r][n] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int kg = lane >> 4, col = lane & 15;  // k group (8 channels) and row/column of the fragment
-    static_assert(9 % kGroup == 0, "tap groups");
-    bf16x8 b[kGroup][kNBW];
-    auto load_taps = [&](int cb, int first) {
+    for (int kb = 0; kb < cin / kCB; kb++) {
+        bf16x8 b[9][kNBW];
 #pragma unroll
-        for (int t = 0; t < kGroup; t++)
+        for (int t = 0; t < 9; t++)
 #pragma unroll
             for (int n = 0; n < kNBW; n++) {
                 const int co = co0 + (wn * kNBW + n) * 16 + col;
-                b[t][n] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(
-                                                         a.wt + ((size_t)(first + t) * a.cout + co) * cin + cb + kg * 8));
+                b[t][n] = __builtin_bit_cast(
+                    bf16x8, *reinterpret_cast<const uint4*>(a.wt + ((size_t)t * a.cout + co) * cin + kb * kCB + kg * 8));
             }
-    };
-    for (int cb = 0; cb < cin; cb += kCB) {
-        load_taps(cb, 0);
+        uint4 pre[kChunksPerThread];
+        halo_chunks(a, y0, x0, kb, pre);
+        if (kb > 0) __syncthreads();  // the previous K block's MFMAs are done with the halo
+        halo_store(halo, pre);
         __syncthreads();
-        for (int i = tid; i < kHaloH * kHaloW * 4; i += 256) {
-            const int px = i >> 2, q = i & 3;
-            const int hy = px / kHaloW, hx = px % kHaloW;
-            *reinterpret_cast<uint4*>(halo + px * kPix + q * 16) = fetch_chunk(a, y0 + hy - 1, x0 + hx - 1, cb + q * 8);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int g = 0; g < 9; g += kGroup) {
-            if (g > 0) load_taps(cb, g);
-#pragma unroll
-            for (int t = 0; t < kGroup; t++) {
-                const int dy = (g + t) / 3, dx = (g + t) % 3;
-#pragma unroll
-                for (int r = 0; r < kRows; r++) {
-                    const int hy = wm * kRows + r + dy, hx = col + dx;
-                    const bf16x8 av = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(
-                                                                     halo + (hy * kHaloW + hx) * kPix + kg * 16));
-#pragma unroll
-                    for (int n = 0; n < kNBW; n++)
-                        acc[r][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b[t][n], acc[r][n], 0, 0, 0);
-                }
-                // each tap's LDS reads stay with its MFMAs (hoisting every
-                // tap's A fragments would cost registers, hence blocks per CU)
-                __builtin_amdgcn_sched_barrier(0);
-            }
-        }
+        halo_rows_mfma<kRows, kNBW, true>(halo, wm * kRows, col, kg, b, acc);
     }
     __syncthreads();  // every wave is done with the halo
     conv_epilogue<kCout, kMW>(a, acc, reinterpret_cast<uint16_t*>(halo), y0, x0, co0);
@@ -367,8 +365,6 @@ __device__ __forceinline__ void feature_chunks(float4 f, float4 ga, float4 gn, f
 #ifndef VH_CONV_PIPE_KB
 #define VH_CONV_PIPE_KB 1
 #endif
-constexpr int kHaloChunks = kHaloH * kHaloW * 4;              // 16-byte chunks of one K block's halo
-constexpr int kChunksPerThread = (kHaloChunks + 255) / 256;
 
 template <int kCout, int kMW, int kKB, bool kFeat = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kKB == 1 ? 4 : kKB == 2 ? 3 : 2))) void k_conv3x3_resident(
@@ -416,24 +412,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kKB == 1 ? 
             }
             return;
         }
-        // the K block comes from one input for every lane (c1 is a multiple of
-        // 32): x1 (at half resolution when up1) or the skip x2
-        const bool first = kb * kCB < a.c1;
-        const uint16_t* src = first ? a.x1 : a.x2;
-        const int cs = first ? a.c1 : a.c2, cb = first ? kb * kCB : kb * kCB - a.c1;
-        const int half = first && a.up1;
-        const int ws = half ? (a.w + 1) / 2 : a.w;
-#pragma unroll
-        for (int j = 0; j < kChunksPerThread; j++) {
-            const int i = tid + 256 * j;
-            const int px = i >> 2, q = i & 3;
-            const int y = y0 + px / kHaloW - 1, x = x0 + px % kHaloW - 1;
-            pre[j] = make_uint4(0, 0, 0, 0);
-            if (i < kHaloChunks && (unsigned)y < (unsigned)a.h && (unsigned)x < (unsigned)a.w) {
-                const uint32_t p = (uint32_t)(y >> half) * (uint32_t)ws + (uint32_t)(x >> half);
-                pre[j] = *reinterpret_cast<const uint4*>(src + (size_t)p * cs + cb + q * 8);
-            }
-        }
+        halo_chunks(a, y0, x0, kb, pre);
     };
     int t = blockIdx.x, prev = -1;
     if (t < tiles_total) fetch(t, 0);
@@ -463,11 +442,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kKB == 1 ? 
                     for (int q = 0; q < 4; q++) *reinterpret_cast<uint4*>(halo + tid * kPix + q * 16) = c[q];
                 }
             } else {
-#pragma unroll
-                for (int j = 0; j < kChunksPerThread; j++) {
-                    const int i = tid + 256 * j;
-                    if (i < kHaloChunks) *reinterpret_cast<uint4*>(halo + (i >> 2) * kPix + (i & 3) * 16) = pre[j];
-                }
+                halo_store(halo, pre);
             }
             // the next step's loads go out before this step's MFMAs, and the
             // previous tile's stores right after them: the wait for these
@@ -556,7 +531,7 @@ inline int launch_resident(const ConvArgs& a, int tiles_per_image, int total, ui
 // output channels into 32-channel slices (more blocks, each staging the same
 // halo).
 #ifndef VH_DENOISE_SPLIT_BLOCKS
-#define VH_DENOISE_SPLIT_BLOCKS 4
+#define VH_DENOISE_SPLIT_BLOCKS 3
 #endif
 // `force_split` takes the sliced kernel for any width that is a multiple of
 // 32 (the backward pass's 160- and 224-channel outputs).
@@ -597,11 +572,11 @@ inline int launch_conv(const ConvArgs& a, uint32_t cu_count, hipStream_t s, bool
         // wave splits and tap groups as measured fastest at 1920x1080: the
         // kernels are latency bound (a block waits on its halo and weight
         // loads), so fewer registers and more resident waves win
-        case 16: hipLaunchKernelGGL((k_conv3x3<16, 4, 3>), grid, dim3(256), 0, s, a); break;
-        case 32: hipLaunchKernelGGL((k_conv3x3<32, 2, 3>), grid, dim3(256), 0, s, a); break;
-        case 64: hipLaunchKernelGGL((k_conv3x3<64, 1, 9>), grid, dim3(256), 0, s, a); break;
-        case 96: hipLaunchKernelGGL((k_conv3x3<96, 2, 3>), grid, dim3(256), 0, s, a); break;
-        case 128: hipLaunchKernelGGL((k_conv3x3<128, 1, 3>), grid, dim3(256), 0, s, a); break;
+        case 16: hipLaunchKernelGGL((k_conv3x3<16, 4>), grid, dim3(256), 0, s, a); break;
+        case 32: hipLaunchKernelGGL((k_conv3x3<32, 2>), grid, dim3(256), 0, s, a); break;
+        case 64: hipLaunchKernelGGL((k_conv3x3<64, 1>), grid, dim3(256), 0, s, a); break;
+        case 96: hipLaunchKernelGGL((k_conv3x3<96, 2>), grid, dim3(256), 0, s, a); break;
+        case 128: hipLaunchKernelGGL((k_conv3x3<128, 1>), grid, dim3(256), 0, s, a); break;
         default: return fail(VH_ERR_USAGE, "denoiser: unsupported channel count");
     }
     return VH_OK;
