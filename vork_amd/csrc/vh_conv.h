@@ -5,14 +5,19 @@
 // v_mfma_f32_16x16x32_bf16 is one tap over 32 input channels.  A workgroup
 // (4 waves) owns an 8 x 16 pixel tile and its output channels (all of them,
 // or a 32-channel slice per blockIdx.y on the coarse levels, whose tiles alone
-// would leave most CUs idle); for each
-// block of 32 input channels it stages the (8+2) x (16+2) halo of the tile in
-// LDS (fusing the nearest upsample and the skip concatenation of the decoder),
-// then each wave runs 9 taps of MFMAs over its share of the tile's 8 pixel
-// rows and of the output channels, with weights read as 16-byte fragments.
-// The epilogue adds the bias, applies ReLU and rounds to bf16; on the encoder
-// it also writes the 2x2 average pool of the rounded output (the next level's
-// input), whose 2x2 groups never straddle a tile or a wave's rows.
+// would leave most CUs idle).  For each block of 32 input channels it stages
+// the (8+2) x (16+2) halo of the tile in LDS (fusing the nearest upsample and
+// the skip concatenation of the decoder); each wave then walks the halo by
+// rows, every A fragment feeding the up to three output rows it reaches, over
+// its share of the tile's 8 pixel rows and of the output channels, with
+// weights held as 16-byte fragments in registers.  Layers of up to 4 K blocks
+// run in resident workgroups that keep their weights across tiles and
+// prefetch the next K block's halo.  The epilogue adds the bias, applies
+// ReLU and rounds to bf16; on the encoder it also writes the 2x2 average pool
+// of the rounded output (the next level's input), whose 2x2 groups never
+// straddle a tile or a wave's rows.  The first layer computes its input
+// features from the film and guides while staging, and the output layer
+// writes the radiance from its epilogue.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -527,9 +532,10 @@ inline int launch_resident(const ConvArgs& a, int tiles_per_image, int total, ui
     return VH_OK;
 }
 
-// Layers whose tiles give fewer than kSplitBlocks blocks per CU split their
-// output channels into 32-channel slices (more blocks, each staging the same
-// halo).
+// Layers whose tiles give fewer than VH_DENOISE_SPLIT_BLOCKS blocks per CU
+// split their output channels into 32-channel slices, one tile and slice per
+// workgroup (more blocks, each staging the same halo); the others run in
+// resident workgroups when they have at most 4 K blocks.
 #ifndef VH_DENOISE_SPLIT_BLOCKS
 #define VH_DENOISE_SPLIT_BLOCKS 3
 #endif
@@ -543,14 +549,6 @@ inline int launch_conv(const ConvArgs& a, uint32_t cu_count, hipStream_t s, bool
         if (cin != kCB || a.cout != 32 || a.up1 || a.c2)
             return fail(VH_ERR_USAGE, "denoiser: features feed a 32-to-32-channel layer only");
         return launch_resident<32, 2, 1, true>(a, (int)tiles, (int)tiles * images, cu_count, s);
-    }
-    if (split && cin <= 4 * kCB && a.cout % 32 == 0) {
-        // 32-channel slices over the second grid dimension, resident likewise
-        const int total = (int)tiles * images, kb = cin / kCB, slices = a.cout / 32;
-        if (kb == 1) return launch_resident<32, 2, 1>(a, (int)tiles, total, cu_count, s, slices);
-        if (kb == 2) return launch_resident<32, 2, 2>(a, (int)tiles, total, cu_count, s, slices);
-        if (kb == 3) return launch_resident<32, 2, 3>(a, (int)tiles, total, cu_count, s, slices);
-        return launch_resident<32, 2, 4>(a, (int)tiles, total, cu_count, s, slices);
     }
     if (!split && cin <= 4 * kCB && (a.cout == 16 || a.cout == 32 || a.cout == 64)) {
         // resident workgroups: weights loaded once, the next halo prefetched
