@@ -46,7 +46,8 @@ S_NODE, S_TRI, S_RAYIN, S_HITOUT = 80, 36, 32, 16   # bytes: wide node, triangle
 HBM_PEAK_GBPS = 8000.0
 # HBM-side traffic of the traced kernels, measured by this script's own
 # rocprofv3 children (FETCH_SIZE, then WRITE_SIZE, each over one render step
-# and over a plain device copy of known size that scales the counter to bytes)
+# and over a plain device copy of known size that scales the counter to bytes),
+# and their L2 hit rates (TCC_HIT_sum and TCC_MISS_sum over one render step)
 TRAFFIC_KERNELS = {"k_extend_packet": "k_extend_packet", "k_extend": "k_extend<false", "k_shade": "k_shade<",
                    "k_shadow": "k_shadow<false"}
 COPY_MIB = 1024
@@ -293,6 +294,12 @@ def main():
                         t = ext[part["kernel"]]["bytes_per_launch"]
                         part["traffic_bytes_per_launch"] = t
                         part["traffic_gbps"] = round(t / (part["avg_launch_ms"] * 1e-3) / 1e9, 1)
+                        if "l2_hit_rate" in ext[part["kernel"]]:
+                            part["l2_hit_rate"] = ext[part["kernel"]]["l2_hit_rate"]
+                kl = [ext[k] for k in {camera_kernel, "k_extend"} if "l2_hits" in ext[k]]
+                if kl:
+                    h = sum(k["l2_hits"] for k in kl)
+                    roofline["l2_hit_rate"] = round(h / max(h + sum(k["l2_misses"] for k in kl), 1), 4)
 
     cpu = None
     if not args.no_cpu_baseline and args.workload == "config1":
@@ -555,6 +562,12 @@ def cpu_baseline(sc, samples=8):
 def _rocprof_counter(counter, argv, outdir, timeout=300):
     """Runs `argv` (a python command) under rocprofv3 --pmc <counter> and
     returns {kernel name: [value per dispatch]} from its CSV."""
+    return _rocprof_counters([counter], argv, outdir, timeout)[counter]
+
+
+def _rocprof_counters(counters, argv, outdir, timeout=300):
+    """Runs `argv` under one rocprofv3 --pmc pass over `counters` and returns
+    {counter: {kernel name: [value per dispatch]}}."""
     import csv
     import glob
     import shutil
@@ -564,26 +577,27 @@ def _rocprof_counter(counter, argv, outdir, timeout=300):
         raise RuntimeError("rocprofv3 not found")
     if os.path.isdir(outdir):
         shutil.rmtree(outdir)
-    cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", outdir, "-o", "run", "--", *argv]
+    cmd = [exe, "--pmc", *counters, "--output-format", "csv", "-d", outdir, "-o", "run", "--", *argv]
     env = dict(os.environ, TMPDIR=os.environ.get("TMPDIR", "/tmp"))
     r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
                        timeout=timeout)
     files = glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True)
     if not files:
-        raise RuntimeError(f"rocprofv3 --pmc {counter} wrote no counter CSV (rc {r.returncode}): "
+        raise RuntimeError(f"rocprofv3 --pmc {' '.join(counters)} wrote no counter CSV (rc {r.returncode}): "
                            + r.stdout[-400:].replace("\n", " | "))
-    values = {}
+    values = {c: {} for c in counters}
     for f in files:
         with open(f) as fh:
             rows = list(csv.DictReader(fh))
         per = {}
         for row in rows:
-            if row.get("Counter_Name") != counter:
+            c = row.get("Counter_Name")
+            if c not in values:
                 continue
-            key = (row.get("Dispatch_Id"), row.get("Kernel_Name", ""))
+            key = (c, row.get("Dispatch_Id"), row.get("Kernel_Name", ""))
             per[key] = per.get(key, 0.0) + float(row["Counter_Value"])
-        for (_, name), v in per.items():
-            values.setdefault(name, []).append(v)
+        for (c, _, name), v in per.items():
+            values[c].setdefault(name, []).append(v)
     return values
 
 
@@ -622,6 +636,17 @@ def measure_traffic(args):
             k["bytes_per_launch"] = int(k.get("FETCH_SIZE", 0.0) + k.get("WRITE_SIZE", 0.0))
             k["read_bytes_per_launch"] = int(k.pop("FETCH_SIZE", 0.0))
             k["written_bytes_per_launch"] = int(k.pop("WRITE_SIZE", 0.0))
+        # L2 (TCC) hit rate of each kernel over the same step: hits / (hits + misses)
+        # summed over its launches and the L2 instances
+        hm = _rocprof_counters(["TCC_HIT_sum", "TCC_MISS_sum"], step, os.path.join(base, "step_TCC_HIT_MISS"))
+        for key, pattern in TRAFFIC_KERNELS.items():
+            hits = sum(v for name, vs in hm["TCC_HIT_sum"].items() if pattern in name for v in vs)
+            miss = sum(v for name, vs in hm["TCC_MISS_sum"].items() if pattern in name for v in vs)
+            if key in kernels and hits + miss > 0:
+                kernels[key]["l2_hits"] = int(hits)
+                kernels[key]["l2_misses"] = int(miss)
+                kernels[key]["l2_hit_rate"] = round(hits / (hits + miss), 4)
+        log("traffic: L2 hit/miss pass done")
         out = {"kernels": kernels, "fetch_scale": round(scale["FETCH_SIZE"], 4),
                "write_scale": round(scale["WRITE_SIZE"], 4), "seconds": round(time.time() - t, 1)}
     except Exception as e:  # reported, never required
