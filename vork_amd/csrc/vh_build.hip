@@ -1042,8 +1042,10 @@ __global__ void k_instance_records(const vh_instance* inst, uint32_t n, uint32_t
     r.blas = valid ? s.blas : 0u;
     r.custom_index = s.custom_index & 0xffffffu;
     r.material = s.material;
+    bool identity = valid;
+    for (int k = 0; k < 12; k++) identity = identity && r.w2o[k] == ((k == 0 || k == 5 || k == 10) ? 1.f : 0.f);
     r.mask_flags = (valid ? (uint32_t)s.mask : 0u) | ((s.flags & VH_INSTANCE_CULL_BACK_FACES) ? (1u << 8) : 0u) |
-                   (valid ? (1u << 9) : 0u);
+                   (valid ? (1u << 9) : 0u) | (identity ? kInstIdentity : 0u);
     out[i] = r;
 }
 }  // namespace

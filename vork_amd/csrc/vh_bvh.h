@@ -80,8 +80,19 @@ struct InstanceRecord {
     uint32_t blas;
     uint32_t custom_index;
     uint32_t material;
-    uint32_t mask_flags;  // bits 0-7 mask, bit 8 cull back faces, bit 9 valid
+    uint32_t mask_flags;  // bits 0-7 mask, bit 8 cull back faces, bit 9 valid, bit 10 identity (kInstIdentity)
 };
+// w2o is exactly the identity: traversal enters the instance with the world
+// ray itself whenever transforming it would leave it unchanged.
+constexpr uint32_t kInstIdentity = 1u << 10;
+
+// Whether the identity rows leave a ray bit for bit unchanged under
+// xform_point / xform_dir: they do unless a component is -0.0 (the sums
+// 1 x + 0 y + 0 z (+ 0) can turn a -0.0 into +0.0).
+VH_HD bool neg_zero(float x) { return f2u(x) == 0x80000000u; }
+VH_HD bool identity_exact(f3 o, f3 d) {
+    return !(neg_zero(o.x) || neg_zero(o.y) || neg_zero(o.z) || neg_zero(d.x) || neg_zero(d.y) || neg_zero(d.z));
+}
 
 struct TlasView {
     const WideNode* nodes;

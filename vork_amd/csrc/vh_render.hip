@@ -479,6 +479,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VH_PACKE
         r.wo = r.o = mk(a.x, a.y, a.z);
         r.wd = r.ld = mk(b.x, b.y, b.z);
         r.inv = box_inv(r.ld);
+        r.winv = r.inv;
         r.oct = octant(r.inv);
         r.lsh = make_shear(r.ld);
         r.tmin = 0.f;

@@ -209,6 +209,8 @@ struct Tracer {
     f3 wo, wd;              // world ray
     f3 o, inv;              // ray of the current space (world or instance object space)
     uint32_t oct;
+    f3 winv;                // the world ray's inv and oct, restored on leaving an instance
+    uint32_t woct;
     f3 ld;                  // object-space direction of the triangle tests
     Shear lsh;
     float tmin, tmax;
@@ -236,6 +238,7 @@ struct Tracer {
                                           Stack& st) {
         wo = o_; wd = d_;
         o = o_; inv = box_inv(d_); oct = octant(inv);
+        winv = inv; woct = oct;
         ld = d_;
         lsh = make_shear(d_);
         tmin = tmin_; tmax = tmax_; mask = mask_;
@@ -250,8 +253,8 @@ struct Tracer {
 
     __device__ __forceinline__ void leave_instance() {
         o = wo;
-        inv = box_inv(wd);
-        oct = octant(inv);
+        inv = winv;
+        oct = woct;
         in_blas = false;
     }
 
@@ -263,13 +266,21 @@ struct Tracer {
         if ((ngroup & 0xffu) && !st.push(nbase, ngroup)) return false;
         if (items && !st.push(ibase | kItemGroup, items)) return false;
         blas_sp = st.sp;
-        float w2o[12];
+        if ((mf & kInstIdentity) && identity_exact(wo, wd)) {
+            // the transformed ray is the world ray: no transform, no reciprocals
+            o = wo;
+            ld = wd;
+            inv = winv;
+            oct = woct;
+        } else {
+            float w2o[12];
 #pragma unroll
-        for (int k = 0; k < 12; k++) w2o[k] = r.w2o[k];
-        o = xform_point(w2o, wo);
-        ld = xform_dir(w2o, wd);
-        inv = box_inv(ld);
-        oct = octant(inv);
+            for (int k = 0; k < 12; k++) w2o[k] = r.w2o[k];
+            o = xform_point(w2o, wo);
+            ld = xform_dir(w2o, wd);
+            inv = box_inv(ld);
+            oct = octant(inv);
+        }
         lsh = make_shear(ld);
         cull = (mf >> 8) & 1u;
         const VH_GLOBAL BlasRecord& b = as_global(tv.blases)[r.blas];
@@ -496,6 +507,7 @@ __device__ __forceinline__ void load_node_scalar(const WideNode* base, uint32_t 
 
 struct PacketRay {
     f3 wo, wd;       // world ray
+    f3 winv;         // the world ray's inv, restored on leaving an instance
     f3 o, inv, ld;   // ray of the current space
     Shear lsh;
     uint32_t oct, mask;
@@ -610,12 +622,18 @@ __device__ inline bool trace_packet(const TlasView& tv, PacketRay& r, uint64_t* 
                 sp++;
             }
             blas_sp = sp;
-            float w2o[12];
+            if ((mf & kInstIdentity) && identity_exact(r.wo, r.wd)) {
+                r.o = r.wo;
+                r.ld = r.wd;
+                r.inv = r.winv;
+            } else {
+                float w2o[12];
 #pragma unroll
-            for (int k = 0; k < 12; k++) w2o[k] = rec.w2o[k];
-            r.o = xform_point(w2o, r.wo);
-            r.ld = xform_dir(w2o, r.wd);
-            r.inv = box_inv(r.ld);
+                for (int k = 0; k < 12; k++) w2o[k] = rec.w2o[k];
+                r.o = xform_point(w2o, r.wo);
+                r.ld = xform_dir(w2o, r.wd);
+                r.inv = box_inv(r.ld);
+            }
             r.oct = octant(r.inv);
             r.lsh = make_shear(r.ld);
             r.in_inst = enter;
@@ -655,7 +673,7 @@ __device__ inline bool trace_packet(const TlasView& tv, PacketRay& r, uint64_t* 
         // the group is used up: leave the instance, take the next group, or stop
         if (in_blas && sp == blas_sp) {
             r.o = r.wo;
-            r.inv = box_inv(r.wd);
+            r.inv = r.winv;
             r.oct = octant(r.inv);
             in_blas = false;
             order = (uint32_t)__builtin_amdgcn_readfirstlane((int)r.oct);
