@@ -379,23 +379,24 @@ __device__ __forceinline__ void feature_chunks(float4 f, float4 ga, float4 gn, f
 // The same convolution with resident workgroups, for layers of at most 4
 // input-channel blocks of 32 (kKB): a workgroup loads its waves' weight fragments for
 // every tap and K block once, then walks tiles t = blockIdx.x + k gridDim.x
-// over the batch's images.  Each K block's halo is read into registers one
-// step ahead (3 chunks of 16 bytes per thread), so the loads of the next
-// step are in flight while the MFMAs of this one run; the weights are not
-// fetched again per tile.  The halo and the output tile have LDS regions of
-// their own: a tile's output waits in LDS and goes to memory during the next
-// tile's first step, right after that step's loads.
+// over the batch's images; the weights are not fetched again per tile.  A
+// layer of one K block reads the next tile's halo into registers one tile
+// ahead (3 chunks of 16 bytes per thread), so its loads are in flight while
+// the MFMAs of this one run; a layer of 2-4 K blocks stages all of a tile's
+// K blocks at once after issuing their loads at the tile's start.  The halos
+// and the output tile have LDS regions of their own: a tile's output waits in
+// LDS and goes to memory during the next tile's staging.
 template <int kCout, int kMW, int kKB, bool kFeat = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kKB == 1 ? 4 : kKB == 2 ? 3 : 2))) void k_conv3x3_resident(
     ConvArgs args, int tiles_per_image, int tiles_total) {
     constexpr int kNB = kCout / 16, kNW = 4 / kMW, kNBW = kNB / kNW, kRows = kTileH / kMW;
     static_assert(kNB % kNW == 0 && kTileH % kMW == 0, "wave split");
     constexpr int kOutPix = kCout + 8;
-    // 32-channel layers of 3 K blocks (the decoder's first full-resolution
-    // layer) stage all of a tile's K blocks at once, an LDS halo per block:
-    // two barriers per tile instead of two per K block (measured 142 -> 134 us
-    // at 1920x1080; the 64-channel layers would spill their registers)
-    constexpr bool kAll = kKB == 3 && kCout == 32;
+    // Layers of 2-4 K blocks stage all of a tile's K blocks at once, an LDS
+    // halo per block: two barriers per tile instead of two per K block (at
+    // 1920x1080 the decoder's first full-resolution layer, 3 K blocks, went
+    // 142 -> 130 us and the 4-K-block layer 73 -> 70 us)
+    constexpr bool kAll = kKB >= 2;
     constexpr int kHalos = kAll ? kKB : 1;             // halos staged per barrier
     constexpr int kHaloBytes = kHaloH * kHaloW * kPix;
     __shared__ __attribute__((aligned(16))) uint8_t halo[kHalos * kHaloBytes];
@@ -442,10 +443,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kKB == 1 ? 
         halo_chunks(a, y0, x0, kb, dst);
     };
     int t = blockIdx.x, prev = -1;
-    if (t < tiles_total) {
-#pragma unroll
-        for (int kb = 0; kb < kHalos; kb++) fetch(t, kb, pre[kb]);
-    }
+    if (!kAll && t < tiles_total) fetch(t, 0, pre[0]);
     // the global stores of tile `prev`, whose output waits in out_tile
     auto store_prev = [&]() {
         const int img = prev / tiles_per_image, r = prev - img * tiles_per_image;
@@ -460,16 +458,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kKB == 1 ? 
 #pragma unroll
             for (int n = 0; n < kNBW; n++) acc[rr][n] = f32x4{0.f, 0.f, 0.f, 0.f};
         if constexpr (kAll) {
+            // this tile's loads go out first and the previous tile's stores
+            // run while they are in flight (holding them a tile ahead, as the
+            // other layers do, measured no faster here: 133 vs 132 us)
+#pragma unroll
+            for (int kb = 0; kb < kKB; kb++) fetch(t, kb, pre[kb]);
             __syncthreads();  // the previous tile's MFMAs are done with the halos
+            if (prev >= 0) store_prev();
 #pragma unroll
             for (int kb = 0; kb < kKB; kb++) halo_store(halo + kb * kHaloBytes, pre[kb]);
-            // the next tile's loads go out before this tile's MFMAs, and the
-            // previous tile's stores right after them
-            if (t + (int)gridDim.x < tiles_total) {
-#pragma unroll
-                for (int kb = 0; kb < kKB; kb++) fetch(t + gridDim.x, kb, pre[kb]);
-            }
-            if (prev >= 0) store_prev();
             __syncthreads();
 #pragma unroll
             for (int kb = 0; kb < kKB; kb++)
