@@ -51,7 +51,10 @@ KNOBS = {
     "VH_PACKET_WAVES_PER_EU": [4, 5, 6, 7],
     "VH_PACKET_MIN_GROUP": [4, 16, 32],
     "VH_SHADE_WAVES_PER_EU": [3, 4, 5],
-    "VH_TRACE_REFILL_MIN": [32, 40, 48, 56],
+    "VH_TRACE_REFILL_MIN": [8, 16, 24, 32, 48],
+    "VH_TRACE_CLAIM_CHUNK": [0, 1],
+    "VH_ANY_REFILL_MIN": [16, 32, 48, 56],
+    "VH_ANY_CLAIM_CHUNK": [0, 1],
     "VH_WAVE_SAMPLES": [16, 32, 64],
     "VH_GRID_BLOCKS_PER_CU": [4, 6, 8, 12, 16],
     "VH_GRID_MIN_BLOCKS_PER_CU": [2, 4, 6],
