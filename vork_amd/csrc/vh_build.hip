@@ -644,12 +644,37 @@ __device__ __forceinline__ int dp_children(const Tree& t, uint32_t r, uint32_t* 
     return m;
 }
 
+// The exclusive prefix of v (< 2^kBits) over the wave's lanes plus `counter`'s
+// value before the wave's total was added to it: one atomic per wave, so the
+// lanes of a level do not serialise on the counter.  Every lane of the wave
+// takes part.
+template <int kBits>
+__device__ __forceinline__ uint32_t wave_alloc(uint32_t v, uint32_t* counter) {
+    const uint64_t lt = (1ull << __lane_id()) - 1ull;
+    uint32_t pre = 0, tot = 0;
+#pragma unroll
+    for (int b = 0; b < kBits; b++) {
+        const uint64_t m = __ballot((v >> b) & 1u);
+        pre += (uint32_t)__popcll(m & lt) << b;
+        tot += (uint32_t)__popcll(m) << b;
+    }
+    uint32_t base = 0;
+    if (tot) {
+        const int leader = __ffsll((unsigned long long)__ballot(1)) - 1;
+        if ((int)__lane_id() == leader) base = atomicAdd(counter, tot);
+        base = __shfl(base, leader);
+    }
+    return base + pre;
+}
+
 __global__ void k_collapse(Tree t, const QueueItem* in, const uint32_t* in_count, QueueItem* out, uint32_t* out_count,
                            uint32_t* node_counter, uint32_t* item_counter, WideNode* nodes, uint32_t* order,
                            uint32_t leaf_max, uint32_t n_prims, uint32_t* max_level, uint32_t level) {
     uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= *in_count) return;
-    QueueItem it = in[idx];
+    // lanes past the queue stay for the wave's allocations
+    const bool valid = idx < *in_count;
+    QueueItem it = valid ? in[idx] : QueueItem{0u, 0u};
+    if (!valid) it.binary = 0;  // a leaf: one child, no allocation
     uint32_t c[kWidth];
     int m = 0;
     if (is_leaf(t, it.binary)) {
@@ -710,13 +735,22 @@ __global__ void k_collapse(Tree t, const QueueItem* in, const uint32_t* in_count
     }
     node.ex = e[0]; node.ey = e[1]; node.ez = e[2];
     uint32_t n_internal = 0, n_items = 0;
-    for (int k = 0; k < m; k++) {
-        if (dp_internal(t, c[k], leaf_max)) n_internal++; else n_items += t.size[c[k]];
+    if (valid) {
+        for (int k = 0; k < m; k++) {
+            if (dp_internal(t, c[k], leaf_max)) n_internal++; else n_items += t.size[c[k]];
+        }
     }
-    uint32_t child_base = n_internal ? atomicAdd(node_counter, n_internal) : 0u;
-    uint32_t item_base = n_items ? atomicAdd(item_counter, n_items) : 0u;
-    node.child_base = child_base;
-    node.item_base = item_base;
+    // the wave's wide children, primitives and queue entries in one atomic
+    // each (a node's internal children take consecutive node ids and queue
+    // slots, in slot order)
+    const uint32_t child_base = wave_alloc<4>(n_internal, node_counter);
+    const uint32_t item_base = wave_alloc<7>(n_items, item_counter);
+    const uint32_t slot_base = wave_alloc<4>(n_internal, out_count);
+    if (__ballot(valid && n_internal) && __lane_id() == (uint32_t)(__ffsll((unsigned long long)__ballot(1)) - 1))
+        atomicMax(max_level, level + 1);
+    if (!valid) return;
+    node.child_base = n_internal ? child_base : 0u;
+    node.item_base = n_items ? item_base : 0u;
     uint32_t ki = 0, off = 0, imask = 0;
     for (int q = 0; q < kWidth; q++) {
         node.meta[q] = (uint8_t)kMetaNoItems;
@@ -724,9 +758,8 @@ __global__ void k_collapse(Tree t, const QueueItem* in, const uint32_t* in_count
         if (k < 0) continue;
         if (dp_internal(t, c[k], leaf_max)) {
             imask |= 1u << q;
-            uint32_t slot = atomicAdd(out_count, 1u);
-            out[slot].binary = c[k];
-            out[slot].wide = child_base + ki;
+            out[slot_base + ki].binary = c[k];
+            out[slot_base + ki].wide = child_base + ki;
             ki++;
         } else {
             uint32_t prims[4];
@@ -738,7 +771,6 @@ __global__ void k_collapse(Tree t, const QueueItem* in, const uint32_t* in_count
     }
     node.imask = (uint8_t)imask;
     nodes[it.wide] = node;
-    if (ki) atomicMax(max_level, level + 1);
 }
 
 // SAH cost of the binary tree relative to its root, with a cost of 1 per
