@@ -37,7 +37,92 @@ void device_drop(vh_device* d) {
     d->streams.clear();
     if (d->static_host) (void)hipHostFree(d->static_host);
     if (d->stream) (void)hipStreamDestroy(d->stream);
+    // every object of the device has released its blocks into the cache
+    pool_trim(d->ordinal);
     delete d;
+}
+
+// ------------------------------------------------------------ block cache
+namespace {
+struct BlockPool {
+    std::mutex lock;
+    std::multimap<size_t, void*> blocks;  // size -> block
+    size_t cached = 0;
+};
+constexpr int kPoolDevices = 64;
+constexpr size_t kPoolCap = size_t(16) << 30;
+// Never destroyed: blocks may be released during static destruction.
+BlockPool* pools() {
+    static BlockPool* p = new BlockPool[kPoolDevices];
+    return p;
+}
+// Sizes in 256-byte steps, and in 2 MiB steps from 2 MiB on.
+size_t pool_round(size_t n) {
+    const size_t g = n >= (size_t(2) << 20) ? (size_t(2) << 20) : size_t(256);
+    return (n + g - 1) / g * g;
+}
+}  // namespace
+
+hipError_t pool_alloc(void** p, size_t* bytes, int* device) {
+    int d = 0;
+    hipError_t e = hipGetDevice(&d);
+    if (e != hipSuccess) return e;
+    const size_t n = pool_round(*bytes);
+    if (d >= 0 && d < kPoolDevices) {
+        BlockPool& bp = pools()[d];
+        std::lock_guard<std::mutex> hold(bp.lock);
+        auto it = bp.blocks.lower_bound(n);
+        if (it != bp.blocks.end() && it->first <= n + n / 4) {
+            *p = it->second;
+            *bytes = it->first;
+            *device = d;
+            bp.cached -= it->first;
+            bp.blocks.erase(it);
+            return hipSuccess;
+        }
+    }
+    e = hipMalloc(p, n);
+    if (e == hipErrorOutOfMemory) {
+        // the cached blocks first, then once more
+        (void)hipGetLastError();
+        pool_trim(d);
+        e = hipMalloc(p, n);
+    }
+    if (e == hipSuccess) {
+        *bytes = n;
+        *device = d;
+    }
+    return e;
+}
+
+void pool_free(void* p, size_t bytes, int device) {
+    if (device >= 0 && device < kPoolDevices) {
+        BlockPool& bp = pools()[device];
+        std::lock_guard<std::mutex> hold(bp.lock);
+        if (bp.cached + bytes <= kPoolCap) {
+            bp.blocks.emplace(bytes, p);
+            bp.cached += bytes;
+            return;
+        }
+    }
+    (void)hipFree(p);
+}
+
+void pool_trim(int device) {
+    if (device < 0 || device >= kPoolDevices) return;
+    std::multimap<size_t, void*> blocks;
+    {
+        BlockPool& bp = pools()[device];
+        std::lock_guard<std::mutex> hold(bp.lock);
+        blocks.swap(bp.blocks);
+        bp.cached = 0;
+    }
+    if (blocks.empty()) return;
+    int cur = 0;
+    const bool restore = hipGetDevice(&cur) == hipSuccess && cur != device;
+    if (restore) (void)hipSetDevice(device);
+    for (auto& b : blocks) (void)hipFree(b.second);
+    if (restore) (void)hipSetDevice(cur);
 }
 
 StreamState* stream_state(vh_device* dev, hipStream_t s) {

@@ -27,16 +27,37 @@ int hip_fail(hipError_t e, const char* what);
     } while (0)
 
 // A device allocation that frees itself.
+// Device blocks of released BLAS, TLAS and scene objects, kept per device for
+// the next object's buffers of about the same size (a scene rebuilt every
+// frame otherwise pays hipMalloc and hipFree of its whole storage each time).
+// A block enters the cache only once no work reads it: those objects are
+// released after their calls, which all return synchronised.  pool_alloc
+// returns a cached block of n to n + n/4 bytes (the size in *bytes) or a new
+// one; pool_free keeps up to 16 GiB per device and frees the rest; pool_trim
+// frees a device's cached blocks.
+hipError_t pool_alloc(void** p, size_t* bytes, int* device);
+void pool_free(void* p, size_t bytes, int device);
+void pool_trim(int device);
+
 struct DeviceBuffer {
     void* ptr = nullptr;
     size_t bytes = 0;
+    bool pooled = false;  // allocated from and released to the block cache
+    int device = 0;
     DeviceBuffer() = default;
+    explicit DeviceBuffer(bool pooled_) : pooled(pooled_) {}
     DeviceBuffer(const DeviceBuffer&) = delete;
     DeviceBuffer& operator=(const DeviceBuffer&) = delete;
     ~DeviceBuffer() { release(); }
     hipError_t allocate(size_t n) {
         release();
         if (n == 0) n = 16;
+        if (pooled) {
+            size_t b = n;
+            const hipError_t e = pool_alloc(&ptr, &b, &device);
+            if (e == hipSuccess) bytes = b; else ptr = nullptr;
+            return e;
+        }
         hipError_t e = hipMalloc(&ptr, n);
         if (e == hipSuccess) bytes = n;
         return e;
@@ -44,7 +65,10 @@ struct DeviceBuffer {
     // Grows to at least n bytes; contents are not preserved.
     hipError_t reserve(size_t n) { return n <= bytes ? hipSuccess : allocate(n); }
     void release() {
-        if (ptr) (void)hipFree(ptr);
+        if (ptr) {
+            if (pooled) pool_free(ptr, bytes, device);
+            else (void)hipFree(ptr);
+        }
         ptr = nullptr;
         bytes = 0;
     }
@@ -129,14 +153,15 @@ class DeviceRef {
 
 struct vh_blas {
     vh::DeviceRef device;  // first: dropped after the buffers below are freed
-    vh::DeviceBuffer nodes, tris, tri_ids, positions, normals, indices, materials, surf;
+    vh::DeviceBuffer nodes{true}, tris{true}, tri_ids{true}, positions{true}, normals{true}, indices{true},
+        materials{true}, surf{true};
     vh::BlasRecord record{};
     vh_blas_info info{};
 };
 
 struct vh_tlas {
     vh::DeviceRef device;  // first: dropped after the buffers below are freed
-    vh::DeviceBuffer nodes, inst_ids, instances, blas_records;
+    vh::DeviceBuffer nodes{true}, inst_ids{true}, instances{true}, blas_records{true};
     std::vector<vh_blas*> blases;
     uint32_t instance_count = 0;
     uint32_t node_count = 0;
@@ -163,7 +188,7 @@ struct vh_light_node {
 struct vh_scene {
     vh::DeviceRef device;  // first: dropped after the buffers below are freed
     const vh_tlas* tlas = nullptr;
-    vh::DeviceBuffer materials, lights, light_tree, filter;
+    vh::DeviceBuffer materials{true}, lights{true}, light_tree{true}, filter{true};
     uint32_t material_count = 0, light_count = 0, light_nodes = 0;
     vh_camera camera{};
     vh_sky sky{};
