@@ -646,9 +646,14 @@ int vh_scene_create(vh_device* dev, const vh_tlas* tlas, const vh_scene_desc* de
         VH_HIP(sc->lights.allocate(sizeof(vh_rect_light)));
         VH_HIP(sc->light_tree.allocate(sizeof(vh_light_node)));
     }
-    float table[1025];
-    filter_table(table);
-    if ((rc = upload(sc->filter, table, sizeof(table), false, s))) return bail(rc);
+    // the same table for every scene: computed once (about 49,000 evaluations
+    // of the filter's CDF)
+    static const std::array<float, 1025> table = [] {
+        std::array<float, 1025> t;
+        filter_table(t.data());
+        return t;
+    }();
+    if ((rc = upload(sc->filter, table.data(), sizeof(float) * table.size(), false, s))) return bail(rc);
     sc->camera = desc->camera;
     sc->sky = desc->sky;
     VH_HIP(hipStreamSynchronize(s));
