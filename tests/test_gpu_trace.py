@@ -38,6 +38,27 @@ def test_closest_and_any_match_brute_force_config0(renderer):
     assert np.array_equal(occ, ora.trace_any(rays, brute_force=True))
 
 
+def test_rebuilt_scenes_reuse_device_blocks(renderer):
+    """A released install's BLAS, TLAS and scene storage goes back to the
+    device block cache and the next install of the same size takes it: each
+    rebuilt scene, with other vertex positions written over the reused
+    blocks, still traces like brute force."""
+    rng = np.random.default_rng(5)
+    for k in range(3):
+        sc = S.config0(width=64, height=64)
+        for m in sc.meshes:
+            m.positions = (m.positions + rng.normal(0.0, 0.02, m.positions.shape)).astype(np.float32)
+        inst = renderer.install(sc)
+        ora = oracle.OracleScene(sc)
+        rays = random_rays(sc, 3000, seed=10 + k)
+        _compare(inst.tlas.trace_closest(rays), ora.trace_closest(rays, brute_force=True))
+        assert np.array_equal(inst.tlas.trace_any(rays), ora.trace_any(rays, brute_force=True))
+        inst.close()
+        inst.tlas.close()
+        for b in inst.blases:
+            b.close()
+
+
 def test_camera_rays_match_oracle_bvh(renderer):
     sc = S.config0(width=96, height=96)
     inst = renderer.install(sc)
