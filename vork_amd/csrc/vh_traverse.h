@@ -535,6 +535,20 @@ __device__ __forceinline__ uint64_t uniform_address(const void* p, uint64_t offs
     const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(a >> 32));
     return ((uint64_t)hi << 32) | lo;
 }
+// Lanes whose `pred` holds, straight from the compare mask: HIP's
+// __ballot(int) widens the bool to an int and compares it again, two more
+// vector instructions per vote.
+__device__ __forceinline__ uint64_t wave_vote(bool pred) { return __builtin_amdgcn_ballot_w64(pred); }
+
+// A value every lane holds alike (read at a wave-uniform address from LDS or
+// through the vector unit), moved into an SGPR so that the compiler treats
+// what depends on it as uniform: branches on it stay scalar.
+__device__ __forceinline__ uint32_t wave_uniform(uint32_t x) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)x); }
+template <class T>
+__device__ __forceinline__ const T* wave_uniform(const T* p) {
+    const uint64_t a = (uint64_t)p;
+    return (const T*)(((uint64_t)wave_uniform((uint32_t)(a >> 32)) << 32) | wave_uniform((uint32_t)a));
+}
 __device__ __forceinline__ void load_node_scalar(const WideNode* base, uint32_t i, NodeBits& nb) {
     const VH_CONST vu4* p = (const VH_CONST vu4*)uniform_address(base, (uint64_t)i * sizeof(WideNode));
 #pragma unroll
@@ -554,7 +568,11 @@ struct PacketRay {
 
 // Box tests of a node's children for every lane; the union over the wave of
 // the hit internal children (bits at slot ^ order_oct) and of the hit leaf
-// items.
+// items.  With kUniform every lane taking part has order_oct as its own
+// octant (the usual case for a packet): the near and far planes are then
+// picked once for the wave, by scalar selects of the node's dwords, instead
+// of per lane.
+template <bool kUniform>
 __device__ __forceinline__ void packet_children(const NodeBits& nb, const PacketRay& r, bool take_part,
                                                 uint32_t order_oct, uint32_t& hits, uint32_t& items) {
     const WideNode& n = nb.n;
@@ -567,7 +585,7 @@ __device__ __forceinline__ void packet_children(const NodeBits& nb, const Packet
     const float ey = ((255.f * sy + fabsf(n.origin[1])) + fabsf(o.y)) * fabsf(inv.y) * kBoxMargin;
     const float ez = ((255.f * sz + fabsf(n.origin[2])) + fabsf(o.z)) * fabsf(inv.z) * kBoxMargin;
     const float bnx = bx - ex, bfx = bx + ex, bny = by - ey, bfy = by + ey, bnz = bz - ez, bfz = bz + ez;
-    const uint32_t oct = r.oct;
+    const uint32_t oct = kUniform ? order_oct : r.oct;
     const bool px = !(oct & 1u), py = !(oct & 2u), pz = !(oct & 4u);
     const uint32_t nx0 = px ? nb.v[2].x : nb.v[3].z, nx1 = px ? nb.v[2].y : nb.v[3].w;
     const uint32_t fx0 = px ? nb.v[3].z : nb.v[2].x, fx1 = px ? nb.v[3].w : nb.v[2].y;
@@ -576,6 +594,9 @@ __device__ __forceinline__ void packet_children(const NodeBits& nb, const Packet
     const uint32_t nz0 = pz ? nb.v[3].x : nb.v[4].z, nz1 = pz ? nb.v[3].y : nb.v[4].w;
     const uint32_t fz0 = pz ? nb.v[4].z : nb.v[3].x, fz1 = pz ? nb.v[4].w : nb.v[3].y;
     const uint32_t imask = n.imask;
+    // the vote of a single compare is its lane mask; the lanes taking part are
+    // masked in with one scalar AND per child
+    const uint64_t part = wave_vote(take_part);
     uint32_t h = 0, it = 0;
 #pragma unroll
     for (int c = 0; c < 8; c++) {
@@ -591,7 +612,7 @@ __device__ __forceinline__ void packet_children(const NodeBits& nb, const Packet
         const float tfz = __builtin_fmaf((float)((fzw >> sh) & 0xffu), az, bfz);
         const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, r.tmin));
         const float tf = fminf(fminf(tfx, tfy), fminf(tfz, tmax));
-        if (__ballot(take_part && tn <= tf)) {
+        if (wave_vote(tn <= tf) & part) {
             const uint32_t meta = ((c < 4 ? nb.v[1].z : nb.v[1].w) >> sh) & 0xffu;
             if ((imask >> c) & 1u) {
                 h |= 1u << ((uint32_t)c ^ order_oct);
@@ -643,11 +664,11 @@ __device__ inline bool trace_packet(const TlasView& tv, PacketRay& r, uint64_t* 
                 continue;
             }
             // an instance of the TLAS: enter it if some lane's mask takes it
-            const uint32_t inst = as_global(tv.inst_ids)[item];
+            const uint32_t inst = wave_uniform(as_global(tv.inst_ids)[item]);
             const VH_GLOBAL InstanceRecord& rec = as_global(tv.instances)[inst];
-            const uint32_t mf = rec.mask_flags;
+            const uint32_t mf = wave_uniform(rec.mask_flags);
             const bool enter = r.live && (mf & (1u << 9)) && (mf & r.mask & 0xffu);
-            if (!__ballot(enter)) continue;
+            if (!wave_vote(enter)) continue;
             if (sp + 2 > kPacketStack) return false;
             if (ngroup & 0xffu) {
                 if (lane == 0) stack[sp] = (uint64_t)nbase | ((uint64_t)ngroup << 32);
@@ -675,9 +696,9 @@ __device__ inline bool trace_packet(const TlasView& tv, PacketRay& r, uint64_t* 
             r.in_inst = enter;
             cull = (mf >> 8) & 1u;
             const VH_GLOBAL BlasRecord& br = as_global(tv.blases)[rec.blas];
-            bnodes = br.nodes;
-            btris = br.tris;
-            bids = br.tri_ids;
+            bnodes = wave_uniform(br.nodes);
+            btris = wave_uniform(br.tris);
+            bids = wave_uniform(br.tri_ids);
             cur_inst = inst;
             in_blas = true;
             order = (uint32_t)__builtin_amdgcn_readfirstlane((int)r.oct);
@@ -699,7 +720,11 @@ __device__ inline bool trace_packet(const TlasView& tv, PacketRay& r, uint64_t* 
             NodeBits nb;
             load_node_scalar(in_blas ? bnodes : tv.nodes, node, nb);
             uint32_t h, it;
-            packet_children(nb, r, in_blas ? r.in_inst : r.live, order, h, it);
+            const bool take_part = in_blas ? r.in_inst : r.live;
+            if (!wave_vote(take_part && r.oct != order))
+                packet_children<true>(nb, r, take_part, order, h, it);
+            else
+                packet_children<false>(nb, r, take_part, order, h, it);
             nbase = nb.n.child_base;
             ngroup = h | ((uint32_t)nb.n.imask << 8);
             ibase = nb.n.item_base;
@@ -717,7 +742,7 @@ __device__ inline bool trace_packet(const TlasView& tv, PacketRay& r, uint64_t* 
         if (sp == 0) break;
         sp--;
         const uint64_t e = stack[sp];
-        const uint32_t e0 = (uint32_t)e, e1 = (uint32_t)(e >> 32);
+        const uint32_t e0 = wave_uniform((uint32_t)e), e1 = wave_uniform((uint32_t)(e >> 32));
         if (e0 & kItemGroup) {
             ibase = e0 & ~kItemGroup;
             items = e1;
