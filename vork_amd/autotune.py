@@ -48,7 +48,7 @@ KNOBS = {
     "VH_BLOCK": [128, 256, 512],
     "VH_TRACE_WAVES_PER_EU": [3, 4, 5],
     "VH_ANY_WAVES_PER_EU": [4, 5, 6, 7, 8],
-    "VH_PACKET_WAVES_PER_EU": [4, 5, 6, 7],
+    "VH_PACKET_WAVES_PER_EU": [5, 6, 7, 8],
     "VH_PACKET_MIN_GROUP": [4, 16, 32],
     "VH_SHADE_WAVES_PER_EU": [3, 4, 5],
     "VH_TRACE_REFILL_MIN": [8, 16, 24, 32, 48],

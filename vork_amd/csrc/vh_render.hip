@@ -454,7 +454,8 @@ VH_TRACE_KERNEL void k_extend(VH_ARGS A, uint32_t bounce) {
 #define VH_PACKET_MIN_GROUP 16
 #endif
 // Waves per SIMD the packet kernel is compiled for: its node and triangle
-// data sit in SGPRs, so it fits 80 VGPRs and 6 waves.
+// data and its walk state sit in SGPRs, so it fits 72 VGPRs at 7 waves (8
+// spill a few); the tuned table sets 8.
 #ifndef VH_PACKET_WAVES_PER_EU
 #define VH_PACKET_WAVES_PER_EU 6
 #endif
