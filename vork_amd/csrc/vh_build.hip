@@ -671,13 +671,20 @@ __global__ void k_collapse(Tree t, const QueueItem* in, const uint32_t* in_count
                            uint32_t* node_counter, uint32_t* item_counter, WideNode* nodes, uint32_t* order,
                            uint32_t leaf_max, uint32_t n_prims, uint32_t* max_level, uint32_t level) {
     uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-    // lanes past the queue stay for the wave's allocations
-    const bool valid = idx < *in_count;
-    QueueItem it = valid ? in[idx] : QueueItem{0u, 0u};
-    if (!valid) it.binary = 0;  // a leaf: one child, no allocation
+    // The grid follows an upper bound of the queue (the host reads the length
+    // back every few levels), so whole workgroups and waves past the queue
+    // leave at once (nothing in the kernel waits on a barrier); the other
+    // lanes past it stay for the wave's allocations with no children.
+    const uint32_t count = *in_count;
+    if (blockIdx.x * blockDim.x >= count) return;
+    const bool valid = idx < count;
+    if (!__ballot(valid)) return;
+    const QueueItem it = valid ? in[idx] : QueueItem{0u, 0u};
     uint32_t c[kWidth];
     int m = 0;
-    if (is_leaf(t, it.binary)) {
+    if (!valid) {
+        m = 0;
+    } else if (is_leaf(t, it.binary)) {
         // a single-primitive tree: one leaf child
         c[m++] = it.binary;
     } else {
