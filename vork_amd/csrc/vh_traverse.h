@@ -187,7 +187,7 @@ __device__ __forceinline__ void child_groups(const NodeBits& nb, f3 o, f3 inv, u
     const uint32_t nz0 = pz ? nb.v[3].x : nb.v[4].z, nz1 = pz ? nb.v[3].y : nb.v[4].w;
     const uint32_t fz0 = pz ? nb.v[4].z : nb.v[3].x, fz1 = pz ? nb.v[4].w : nb.v[3].y;
     const uint32_t imask = n.imask;
-    uint32_t h = 0, it = 0;
+    uint32_t hm = 0;  // bit c: child c's box meets the ray
 #pragma unroll
     for (int c = 0; c < 8; c++) {
         const uint32_t sh = (c & 3) * 8;
@@ -202,14 +202,21 @@ __device__ __forceinline__ void child_groups(const NodeBits& nb, f3 o, f3 inv, u
         const float tfz = __builtin_fmaf((float)((fzw >> sh) & 0xffu), az, bfz);
         const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tmin));
         const float tf = fminf(fminf(tfx, tfy), fminf(tfz, tmax));
-        if (tn <= tf) {
-            const uint32_t meta = ((c < 4 ? nb.v[1].z : nb.v[1].w) >> sh) & 0xffu;
-            if ((imask >> c) & 1u) {
-                h |= 1u << ((uint32_t)c ^ oct);
-            } else if (!(meta & kMetaNoItems)) {
-                it |= ((1u << ((meta >> 5) + 1u)) - 1u) << (meta & 31u);
-            }
-        }
+        hm |= tn <= tf ? 1u << c : 0u;
+    }
+    // the hit internal children, bit c moved to c ^ oct (a swap of
+    // neighbouring bits, pairs and nibbles for the octant's three bits)
+    uint32_t h = hm & imask;
+    if (oct & 1u) h = ((h & 0x55u) << 1) | ((h >> 1) & 0x55u);
+    if (oct & 2u) h = ((h & 0x33u) << 2) | ((h >> 2) & 0x33u);
+    if (oct & 4u) h = ((h & 0x0fu) << 4) | (h >> 4);
+    // the hit leaf children's items (internal and empty slots hold no items)
+    uint32_t it = 0, leaves = hm & ~imask;
+    while (leaves) {
+        const uint32_t c = (uint32_t)__builtin_ctz(leaves);
+        leaves &= leaves - 1u;
+        const uint32_t meta = ((c < 4 ? nb.v[1].z : nb.v[1].w) >> ((c & 3u) * 8u)) & 0xffu;
+        if (!(meta & kMetaNoItems)) it |= ((1u << ((meta >> 5) + 1u)) - 1u) << (meta & 31u);
     }
     hits = h;
     items = it;
