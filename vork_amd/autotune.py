@@ -55,6 +55,7 @@ KNOBS = {
     "VH_TRACE_CLAIM_CHUNK": [0, 1],
     "VH_ANY_REFILL_MIN": [16, 32, 48, 56],
     "VH_ANY_CLAIM_CHUNK": [0, 1],
+    "VH_TRACE_ITEMS_PER_STEP": [1, 2],
     "VH_WAVE_SAMPLES": [16, 32, 64],
     "VH_GRID_BLOCKS_PER_CU": [4, 6, 8, 12, 16],
     "VH_GRID_MIN_BLOCKS_PER_CU": [2, 4, 6],
